@@ -1,0 +1,1773 @@
+/* CPU oracle: GFN2-xTB energy, gradient and Mulliken charges.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  This restates, in plain C, what
+ * the reference obtains from tblite v0.2.1's xtb_singlepoint (called at
+ * src/tblite.f90:136, absent from this container) for the GFN2 Hamiltonian:
+ *   - GFN CN (double-exponential counting) and CN-shifted self-energies
+ *   - STO-nG overlap, dipole and quadrupole integrals, H0
+ *   - repulsion, isotropic ES2 (gamma with exponent 2, arithmetic hardness
+ *     mean), shell-resolved third order, anisotropic multipole ES (AES2) with
+ *     on-site XC kernels, self-consistent D4 (two-body with SCC charges, ATM
+ *     with zero charges) following the reference's in-tree D4 code
+ *     (src/dftd4.f90:292-1337, 1463-1780)
+ *   - SCC with modified Broyden mixing (Johnson, PRB 38, 12807 (1988)),
+ *     Fermi smearing at etemp with uhf = min(mult-1,0) = 0, nspin = 1
+ *     (src/tblite.f90:111,133), free energy E - TS
+ *   - analytic gradient: P dH0 - W dS + potential-weighted integral
+ *     derivatives + explicit pair terms + CN chain rules.
+ * The generalized eigenproblem uses Cholesky + Householder tridiagonalization
+ * + implicit QL (Wilkinson & Reinsch, Handbook for Automatic Computation II).
+ */
+#include "oracle.h"
+#include "gfn2_params_oracle.h"
+#include "d4_data_oracle.h"
+#include "sto_ng_oracle.h"
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EVTOAU (1.0 / ORC_AUTOEV)
+#define PI 3.14159265358979323846
+
+static int elem_index(int z)
+{
+    switch (z) {
+    case 1: return 0;
+    case 6: return 1;
+    case 7: return 2;
+    case 8: return 3;
+    default: return -1;
+    }
+}
+int orc_gfn2_supported(int z) { return elem_index(z) >= 0; }
+
+/* ------------------------------------------------------------------ D4 data */
+/* The reference's in-tree D4 tables are default-real (REAL(4)) literals widened
+ * to REAL(8).  orc_d4_literal_mode(1) rounds the data through float so that the
+ * oracle can be pinned bit-for-bit against the compiled reference routines;
+ * the default (0) uses the decimal values as written. */
+static int d4_float_mode = 0;
+void orc_d4_literal_mode(int on) { d4_float_mode = on; }
+static double lit(double v) { return d4_float_mode ? (double)(float)v : v; }
+
+static double d4_refal[4][7][23];
+static int d4_refc[4][7];
+static int d4_init_mode = -1;
+static pthread_mutex_t d4_lock = PTHREAD_MUTEX_INITIALIZER;
+
+static double d4_zeta(double a, double c, double qref, double qmod)
+{ /* dftd4.f90:413-429 */
+    if (qmod < 0.0) return exp(a);
+    return exp(a * (1.0 - exp(c * (1.0 - qref / qmod))));
+}
+static double d4_dzeta(double a, double c, double qref, double qmod)
+{ /* dftd4.f90:431-449 */
+    if (qmod < 0.0) return 0.0;
+    return -a * c * exp(c * (1.0 - qref / qmod)) * d4_zeta(a, c, qref, qmod) * qref /
+           (qmod * qmod);
+}
+static double d4_trapzd(const double *pol)
+{ /* dftd4.f90:451-497 */
+    static const double freq[23] = {0.000001, 0.050000, 0.100000, 0.200000, 0.300000,
+                                     0.400000, 0.500000, 0.600000, 0.700000, 0.800000,
+                                     0.900000, 1.000000, 1.200000, 1.400000, 1.600000,
+                                     1.800000, 2.000000, 2.500000, 3.000000, 4.000000,
+                                     5.000000, 7.500000, 10.00000};
+    double t1 = 0.0, t2 = 0.0;
+    for (int k = 0; k < 11; ++k) {
+        int i = 2 * k; /* pairs (1,2),(3,4),... in 1-based */
+        t1 += (freq[i + 1] - freq[i]) * (pol[i + 1] + pol[i]);
+    }
+    for (int k = 0; k < 11; ++k) {
+        int i = 2 * k + 1; /* pairs (2,3),(4,5),... */
+        t2 += (freq[i + 1] - freq[i]) * (pol[i + 1] + pol[i]);
+    }
+    return 0.5 * t1 + 0.5 * t2;
+}
+static double d4_gam(int z) { return lit(orc_d4_gam[z - 1]); }
+static double d4_zeff(int z) { return orc_d4_zeff[z - 1]; }
+static double d4_en(int z) { return lit(orc_d4_en[z - 1]); }
+static double d4_rcov(int z) { return lit(orc_d4_rcov[z - 1]); }
+static double d4_r4r2(int z) { return lit(orc_d4_r4r2[z - 1]); }
+
+static void d4_setup(void)
+{ /* dftd4.f90:524-595 (d4dim with the GFN2 charge set) */
+    pthread_mutex_lock(&d4_lock);
+    if (d4_init_mode == d4_float_mode) {
+        pthread_mutex_unlock(&d4_lock);
+        return;
+    }
+    for (int e = 0; e < 4; ++e) {
+        int cncount[16] = {0};
+        cncount[0] = 1;
+        int nref = orc_d4_refn[e];
+        for (int j = 0; j < nref; ++j) {
+            int is = orc_d4_refsys[e][j];
+            double iz = d4_zeff(is);
+            double zt = d4_zeta(ORC_D4_GA, d4_gam(is) * ORC_D4_GC, lit(orc_d4_secq[is - 1]) + iz,
+                                lit(orc_d4_refh[e][j]) + iz);
+            int icn = (int)lrint(lit(orc_d4_refcn[e][j]));
+            cncount[icn] += 1;
+            for (int k = 0; k < 23; ++k) {
+                double sec = lit(orc_d4_sscale[is - 1]) * lit(orc_d4_secaiw[is - 1][k]) * zt;
+                double v = lit(orc_d4_ascale[e][j]) *
+                           (lit(orc_d4_alphaiw[e][j][k]) - lit(orc_d4_hcount[e][j]) * sec);
+                d4_refal[e][j][k] = v > 0.0 ? v : 0.0;
+            }
+        }
+        for (int j = 0; j < nref; ++j) {
+            int icn = cncount[(int)lrint(lit(orc_d4_refcn[e][j]))];
+            d4_refc[e][j] = icn * (icn + 1) / 2;
+        }
+    }
+    d4_init_mode = d4_float_mode;
+    pthread_mutex_unlock(&d4_lock);
+}
+
+/* exposed for pinning against the reference's d4dim/trapzd */
+void orc_d4_reference(int z, int *nref, double *refal, int *refc, double *refcovcn,
+                      double *refq)
+{
+    d4_setup();
+    int e = elem_index(z);
+    *nref = orc_d4_refn[e];
+    for (int j = 0; j < 7; ++j) {
+        refc[j] = d4_refc[e][j];
+        refcovcn[j] = lit(orc_d4_refcovcn[e][j]);
+        refq[j] = lit(orc_d4_refq[e][j]);
+        for (int k = 0; k < 23; ++k) refal[23 * j + k] = d4_refal[e][j][k];
+    }
+}
+double orc_d4_trapzd(const double *pol) { return d4_trapzd(pol); }
+double orc_d4_zeta_fn(double a, double c, double qref, double qmod)
+{
+    return d4_zeta(a, c, qref, qmod);
+}
+double orc_d4_dzeta_fn(double a, double c, double qref, double qmod)
+{
+    return d4_dzeta(a, c, qref, qmod);
+}
+
+/* reference C6 between reference systems ii of element ea and jj of eb */
+static double d4_c6ref(int ea, int ii, int eb, int jj)
+{
+    double pol[23];
+    for (int k = 0; k < 23; ++k) pol[k] = d4_refal[ea][ii][k] * d4_refal[eb][jj][k];
+    return 3.0 / PI * d4_trapzd(pol);
+}
+
+/* ------------------------------------------------------------ coordination */
+/* D4 covalent CN, dftd4.f90:292-325 and 1036-1076.  dcn (optional) holds the
+ * pair derivative d(count_ij)/dr_ij in dcn[i*nat+j]. */
+void orc_d4_cn(int nat, const int32_t *num, const double *xyz, double *cn, double *dcn)
+{
+    for (int i = 0; i < nat; ++i) cn[i] = 0.0;
+    if (dcn)
+        for (int i = 0; i < nat * nat; ++i) dcn[i] = 0.0;
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j < i; ++j) {
+            double rij[3] = {xyz[3 * j] - xyz[3 * i], xyz[3 * j + 1] - xyz[3 * i + 1],
+                             xyz[3 * j + 2] - xyz[3 * i + 2]};
+            double r2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+            if (r2 > ORC_D4_CN_THR) continue;
+            double r = sqrt(r2);
+            double rco = d4_rcov(num[j]) + d4_rcov(num[i]);
+            double den = ORC_D4_K4 *
+                         exp(-pow(fabs(d4_en(num[i]) - d4_en(num[j])) + ORC_D4_K5, 2) / ORC_D4_K6);
+            double ex = exp(-ORC_D4_K1 * (rco / r - 1.0));
+            double c = den / (1.0 + ex);
+            cn[i] += c;
+            cn[j] += c;
+            if (dcn) {
+                double d = (-ORC_D4_K1 * rco * ex * den) / (r2 * (1.0 + ex) * (1.0 + ex));
+                dcn[i * nat + j] = d;
+                dcn[j * nat + i] = d;
+            }
+        }
+}
+
+/* GFN2 coordination number: product of two exponential counting functions
+ * with D3 covalent radii (tblite gfn_ncoord).  dcn as above. */
+void orc_gfn2_cn(int nat, const int32_t *num, const double *xyz, double *cn, double *dcn)
+{
+    const double cut2 = ORC_GFN2_CN_CUTOFF * ORC_GFN2_CN_CUTOFF;
+    for (int i = 0; i < nat; ++i) cn[i] = 0.0;
+    if (dcn)
+        for (int i = 0; i < nat * nat; ++i) dcn[i] = 0.0;
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j < i; ++j) {
+            double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1],
+                   dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > cut2) continue;
+            double r = sqrt(r2);
+            double rc = orc_covrad_d3(num[i]) + orc_covrad_d3(num[j]);
+            double e1 = exp(-ORC_GFN2_CN_KA * (rc / r - 1.0));
+            double e2 = exp(-ORC_GFN2_CN_KB * ((rc + ORC_GFN2_CN_RSHIFT) / r - 1.0));
+            double f1 = 1.0 / (1.0 + e1), f2 = 1.0 / (1.0 + e2);
+            cn[i] += f1 * f2;
+            cn[j] += f1 * f2;
+            if (dcn) {
+                double d1 = -ORC_GFN2_CN_KA * rc * e1 * f1 * f1 / r2;
+                double d2 = -ORC_GFN2_CN_KB * (rc + ORC_GFN2_CN_RSHIFT) * e2 * f2 * f2 / r2;
+                double d = d1 * f2 + f1 * d2;
+                dcn[i * nat + j] = d;
+                dcn[j * nat + i] = d;
+            }
+        }
+}
+
+/* ------------------------------------------------------------------ basis */
+typedef struct {
+    int at, l, ng, ao0, nao;
+    double alpha[6], coeff[6];
+} shell_t;
+
+int orc_sto_ng(int ng, int n, int l, double zeta, double *alpha, double *coeff)
+{
+    if (ng < 1 || ng > 6 || n < 1 || n > 3 || l < 0 || l >= n) return -1;
+    for (int i = 0; i < ng; ++i) {
+        alpha[i] = orc_sto_alpha[n - 1][l][ng - 1][i] * zeta * zeta;
+        double a = alpha[i];
+        double norm = pow(2.0 * a / PI, 0.75);
+        if (l == 1) norm *= sqrt(4.0 * a);
+        coeff[i] = orc_sto_coeff[n - 1][l][ng - 1][i] * norm;
+    }
+    /* exact normalisation of the contraction */
+    double s = 0.0;
+    for (int i = 0; i < ng; ++i)
+        for (int j = 0; j < ng; ++j) {
+            double g = alpha[i] + alpha[j];
+            double v = pow(PI / g, 1.5);
+            if (l == 1) v /= 2.0 * g;
+            s += coeff[i] * coeff[j] * v;
+        }
+    for (int i = 0; i < ng; ++i) coeff[i] /= sqrt(s);
+    return 0;
+}
+
+/* 1D Gaussian moment integrals E[i][m] = int (x-A)^i (x-B)^m exp(-a(x-A)^2-b(x-B)^2) */
+static void e1d(double a, double b, double ax, double bx, double E[4][5])
+{
+    double g = a + b, p = (a * ax + b * bx) / g;
+    double pa = p - ax, pb = p - bx, d = ax - bx;
+    double k = exp(-a * b / g * d * d);
+    double m[7];
+    m[0] = sqrt(PI / g) * k;
+    m[1] = 0.0;
+    m[2] = m[0] / (2.0 * g);
+    m[3] = 0.0;
+    m[4] = 3.0 * m[2] / (2.0 * g);
+    m[5] = 0.0;
+    m[6] = 5.0 * m[4] / (2.0 * g);
+    static const int binom[5][5] = {
+        {1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0}, {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
+    for (int i = 0; i < 4; ++i)
+        for (int mm = 0; mm < 5; ++mm) {
+            if (i + mm > 6) {
+                E[i][mm] = 0.0;
+                continue;
+            }
+            double s = 0.0;
+            for (int u = 0; u <= i; ++u)
+                for (int v = 0; v <= mm; ++v) {
+                    if ((u + v) & 1) continue;
+                    s += binom[i][u] * binom[mm][v] * pow(pa, i - u) * pow(pb, mm - v) *
+                         m[u + v];
+                }
+            E[i][mm] = s;
+        }
+}
+
+/* Cartesian exponents of the AO components: s; p = x,y,z */
+static const int cart[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+
+/* Shell-pair integrals over contracted Gaussians (shell A at ra, shell B at rb).
+ * Outputs (row = component of A, column = component of B, leading dims 3x3):
+ *   s[ia*3+ib], dpb[c][..] = <a|(r-B)_c|b>, qpb[k][..] = <a|(r-B)_a(r-B)_b|b> raw,
+ *   and, when ds != NULL, derivatives with respect to ra: ds[x][..],
+ *   ddp[x][c][..], dqp[x][k][..]. */
+static const int qidx[6][2] = {{0, 0}, {0, 1}, {1, 1}, {0, 2}, {1, 2}, {2, 2}};
+
+static void shellpair(const shell_t *sa, const shell_t *sb, const double *ra, const double *rb,
+                      double *s, double *dp, double *qp, double *ds, double *ddp, double *dqp)
+{
+    int na = sa->nao, nb = sb->nao;
+    int offa = sa->l ? 1 : 0, offb = sb->l ? 1 : 0;
+    memset(s, 0, sizeof(double) * 9);
+    memset(dp, 0, sizeof(double) * 27);
+    memset(qp, 0, sizeof(double) * 54);
+    if (ds) {
+        memset(ds, 0, sizeof(double) * 27);
+        memset(ddp, 0, sizeof(double) * 81);
+        memset(dqp, 0, sizeof(double) * 162);
+    }
+    for (int ip = 0; ip < sa->ng; ++ip)
+        for (int jp = 0; jp < sb->ng; ++jp) {
+            double a = sa->alpha[ip], b = sb->alpha[jp];
+            double cc = sa->coeff[ip] * sb->coeff[jp];
+            double E[3][4][5];
+            for (int x = 0; x < 3; ++x) e1d(a, b, ra[x], rb[x], E[x]);
+            for (int ia = 0; ia < na; ++ia)
+                for (int ib = 0; ib < nb; ++ib) {
+                    const int *pa = cart[ia + offa], *pb = cart[ib + offb];
+                    int o = ia * 3 + ib;
+                    double ex[3], e1[3], e2[3];
+                    for (int x = 0; x < 3; ++x) {
+                        ex[x] = E[x][pa[x]][pb[x]];
+                        e1[x] = E[x][pa[x]][pb[x] + 1];
+                        e2[x] = E[x][pa[x]][pb[x] + 2];
+                    }
+                    s[o] += cc * ex[0] * ex[1] * ex[2];
+                    for (int c = 0; c < 3; ++c) {
+                        double v = cc;
+                        for (int x = 0; x < 3; ++x) v *= (x == c) ? e1[x] : ex[x];
+                        dp[c * 9 + o] += v;
+                    }
+                    for (int k = 0; k < 6; ++k) {
+                        int u = qidx[k][0], w = qidx[k][1];
+                        double v = cc;
+                        for (int x = 0; x < 3; ++x) {
+                            int pw = (x == u) + (x == w);
+                            v *= pw == 0 ? ex[x] : (pw == 1 ? e1[x] : e2[x]);
+                        }
+                        qp[k * 9 + o] += v;
+                    }
+                    if (!ds) continue;
+                    /* d/dA_x of (x-A)^i exp(-a(x-A)^2) = 2a (x-A)^(i+1) - i (x-A)^(i-1) */
+                    double dex[3], de1[3], de2[3];
+                    for (int x = 0; x < 3; ++x) {
+                        int i = pa[x], m = pb[x];
+                        dex[x] = 2.0 * a * E[x][i + 1][m] - (i ? i * E[x][i - 1][m] : 0.0);
+                        de1[x] = 2.0 * a * E[x][i + 1][m + 1] - (i ? i * E[x][i - 1][m + 1] : 0.0);
+                        de2[x] = 2.0 * a * E[x][i + 1][m + 2] - (i ? i * E[x][i - 1][m + 2] : 0.0);
+                    }
+                    for (int d = 0; d < 3; ++d) {
+                        double v = cc;
+                        for (int x = 0; x < 3; ++x) v *= (x == d) ? dex[x] : ex[x];
+                        ds[d * 9 + o] += v;
+                        for (int c = 0; c < 3; ++c) {
+                            double w2 = cc;
+                            for (int x = 0; x < 3; ++x) {
+                                if (x == c) w2 *= (x == d) ? de1[x] : e1[x];
+                                else w2 *= (x == d) ? dex[x] : ex[x];
+                            }
+                            ddp[(d * 3 + c) * 9 + o] += w2;
+                        }
+                        for (int k = 0; k < 6; ++k) {
+                            int u = qidx[k][0], w = qidx[k][1];
+                            double w2 = cc;
+                            for (int x = 0; x < 3; ++x) {
+                                int pw = (x == u) + (x == w);
+                                if (x == d)
+                                    w2 *= pw == 0 ? dex[x] : (pw == 1 ? de1[x] : de2[x]);
+                                else
+                                    w2 *= pw == 0 ? ex[x] : (pw == 1 ? e1[x] : e2[x]);
+                            }
+                            dqp[(d * 6 + k) * 9 + o] += w2;
+                        }
+                    }
+                }
+        }
+}
+
+/* traceless quadrupole operator 3/2 r_a r_b - 1/2 r^2 delta_ab from raw moments */
+static void traceless(const double *raw, double *out, int stride)
+{
+    for (int o = 0; o < stride; ++o) {
+        double tr = 0.5 * (raw[0 * stride + o] + raw[2 * stride + o] + raw[5 * stride + o]);
+        out[0 * stride + o] = 1.5 * raw[0 * stride + o] - tr;
+        out[1 * stride + o] = 1.5 * raw[1 * stride + o];
+        out[2 * stride + o] = 1.5 * raw[2 * stride + o] - tr;
+        out[3 * stride + o] = 1.5 * raw[3 * stride + o];
+        out[4 * stride + o] = 1.5 * raw[4 * stride + o];
+        out[5 * stride + o] = 1.5 * raw[5 * stride + o] - tr;
+    }
+}
+
+/* Moments about A from moments about B: shift by t = B - A.
+ * dA = dB + t S;  mA_ab = mB_ab + t_a dB_b + t_b dB_a + t_a t_b S */
+static void shift_moments(const double *t, const double *s, const double *dpb, const double *qpb,
+                          double *dpa, double *qpa)
+{
+    for (int o = 0; o < 9; ++o) {
+        for (int c = 0; c < 3; ++c) dpa[c * 9 + o] = dpb[c * 9 + o] + t[c] * s[o];
+        for (int k = 0; k < 6; ++k) {
+            int u = qidx[k][0], w = qidx[k][1];
+            qpa[k * 9 + o] = qpb[k * 9 + o] + t[u] * dpb[w * 9 + o] + t[w] * dpb[u * 9 + o] +
+                             t[u] * t[w] * s[o];
+        }
+    }
+}
+
+/* exposed for unit pinning: s (9), dpb (27, about B), qpb (54 raw, about B) */
+void orc_overlap_shellpair(int la, int ga, const double *aa, const double *ca, int lb, int gb,
+                           const double *ab, const double *cb, const double *ra, const double *rb,
+                           double *s, double *dpb, double *qpb)
+{
+    shell_t sa = {0}, sb = {0};
+    sa.l = la; sa.ng = ga; sa.nao = la ? 3 : 1;
+    sb.l = lb; sb.ng = gb; sb.nao = lb ? 3 : 1;
+    for (int i = 0; i < ga; ++i) { sa.alpha[i] = aa[i]; sa.coeff[i] = ca[i]; }
+    for (int i = 0; i < gb; ++i) { sb.alpha[i] = ab[i]; sb.coeff[i] = cb[i]; }
+    shellpair(&sa, &sb, ra, rb, s, dpb, qpb, NULL, NULL, NULL);
+}
+
+/* ------------------------------------------------------------- linear algebra */
+/* Householder reduction of symmetric a (n x n, row-major) to tridiagonal form,
+ * accumulating the orthogonal transformation in a; d diagonal, e off-diagonal. */
+static void tred2(int n, double *a, double *d, double *e)
+{
+#define A(i, j) a[(size_t)(i) * n + (j)]
+    for (int i = n - 1; i > 0; --i) {
+        int l = i - 1;
+        double h = 0.0, scale = 0.0;
+        if (l > 0) {
+            for (int k = 0; k <= l; ++k) scale += fabs(A(i, k));
+            if (scale == 0.0) {
+                e[i] = A(i, l);
+            } else {
+                for (int k = 0; k <= l; ++k) {
+                    A(i, k) /= scale;
+                    h += A(i, k) * A(i, k);
+                }
+                double f = A(i, l);
+                double g = (f >= 0.0) ? -sqrt(h) : sqrt(h);
+                e[i] = scale * g;
+                h -= f * g;
+                A(i, l) = f - g;
+                f = 0.0;
+                for (int j = 0; j <= l; ++j) {
+                    A(j, i) = A(i, j) / h;
+                    double gg = 0.0;
+                    for (int k = 0; k <= j; ++k) gg += A(j, k) * A(i, k);
+                    for (int k = j + 1; k <= l; ++k) gg += A(k, j) * A(i, k);
+                    e[j] = gg / h;
+                    f += e[j] * A(i, j);
+                }
+                double hh = f / (h + h);
+                for (int j = 0; j <= l; ++j) {
+                    double ff = A(i, j);
+                    double gg = e[j] - hh * ff;
+                    e[j] = gg;
+                    for (int k = 0; k <= j; ++k) A(j, k) -= (ff * e[k] + gg * A(i, k));
+                }
+            }
+        } else {
+            e[i] = A(i, l);
+        }
+        d[i] = h;
+    }
+    d[0] = 0.0;
+    e[0] = 0.0;
+    for (int i = 0; i < n; ++i) {
+        int l = i - 1;
+        if (d[i] != 0.0) {
+            for (int j = 0; j <= l; ++j) {
+                double g = 0.0;
+                for (int k = 0; k <= l; ++k) g += A(i, k) * A(k, j);
+                for (int k = 0; k <= l; ++k) A(k, j) -= g * A(k, i);
+            }
+        }
+        d[i] = A(i, i);
+        A(i, i) = 1.0;
+        for (int j = 0; j <= l; ++j) A(j, i) = A(i, j) = 0.0;
+    }
+#undef A
+}
+
+/* implicit QL with Wilkinson shifts on the tridiagonal (d, e); z accumulates */
+static int tql2(int n, double *d, double *e, double *z)
+{
+    for (int i = 1; i < n; ++i) e[i - 1] = e[i];
+    e[n - 1] = 0.0;
+    for (int l = 0; l < n; ++l) {
+        int iter = 0, m;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                double dd = fabs(d[m]) + fabs(d[m + 1]);
+                if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 100) return -1;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = hypot(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                for (i = m - 1; i >= l; --i) {
+                    double f = s * e[i], b = c * e[i];
+                    e[i + 1] = (r = hypot(f, g));
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[m] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - b;
+                    for (int k = 0; k < n; ++k) {
+                        double ff = z[(size_t)k * n + i + 1];
+                        z[(size_t)k * n + i + 1] = s * z[(size_t)k * n + i] + c * ff;
+                        z[(size_t)k * n + i] = c * z[(size_t)k * n + i] - s * ff;
+                    }
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+    return 0;
+}
+
+/* Generalized symmetric eigenproblem F C = S C e.  On exit f holds C (columns
+ * are eigenvectors, ascending eigenvalues), s is destroyed.  work: 3n^2+2n. */
+void orc_sygv(int n, double *f, double *s, double *eval, double *work)
+{
+    double *linv = work, *t = work + (size_t)n * n, *z = work + 2 * (size_t)n * n;
+    double *e = work + 3 * (size_t)n * n;
+    /* Cholesky S = L L^T (lower, in s) */
+    for (int j = 0; j < n; ++j) {
+        double d = s[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= s[(size_t)j * n + k] * s[(size_t)j * n + k];
+        d = sqrt(d);
+        s[(size_t)j * n + j] = d;
+        for (int i = j + 1; i < n; ++i) {
+            double v = s[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) v -= s[(size_t)i * n + k] * s[(size_t)j * n + k];
+            s[(size_t)i * n + j] = v / d;
+        }
+    }
+    /* L^-1 */
+    memset(linv, 0, sizeof(double) * n * n);
+    for (int i = 0; i < n; ++i) {
+        linv[(size_t)i * n + i] = 1.0 / s[(size_t)i * n + i];
+        for (int j = 0; j < i; ++j) {
+            double v = 0.0;
+            for (int k = j; k < i; ++k) v += s[(size_t)i * n + k] * linv[(size_t)k * n + j];
+            linv[(size_t)i * n + j] = -v / s[(size_t)i * n + i];
+        }
+    }
+    /* t = L^-1 F ; z = t L^-T */
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double v = 0.0;
+            for (int k = 0; k <= i; ++k) v += linv[(size_t)i * n + k] * f[(size_t)k * n + j];
+            t[(size_t)i * n + j] = v;
+        }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double v = 0.0;
+            for (int k = 0; k <= j; ++k) v += t[(size_t)i * n + k] * linv[(size_t)j * n + k];
+            z[(size_t)i * n + j] = v;
+            z[(size_t)j * n + i] = v;
+        }
+    tred2(n, z, eval, e);
+    tql2(n, eval, e, z);
+    /* sort ascending */
+    for (int i = 0; i < n - 1; ++i) {
+        int k = i;
+        for (int j = i + 1; j < n; ++j)
+            if (eval[j] < eval[k]) k = j;
+        if (k != i) {
+            double tmp = eval[i];
+            eval[i] = eval[k];
+            eval[k] = tmp;
+            for (int r = 0; r < n; ++r) {
+                tmp = z[(size_t)r * n + i];
+                z[(size_t)r * n + i] = z[(size_t)r * n + k];
+                z[(size_t)r * n + k] = tmp;
+            }
+        }
+    }
+    /* C = L^-T z */
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double v = 0.0;
+            for (int k = i; k < n; ++k) v += linv[(size_t)k * n + i] * z[(size_t)k * n + j];
+            f[(size_t)i * n + j] = v;
+        }
+}
+
+/* ------------------------------------------------------------- the system */
+typedef struct {
+    int nat, nsh, nao, nel_int, charge;
+    double nel, kt;
+    const int32_t *num;
+    const double *xyz;
+    int *eidx;
+    shell_t *sh;
+    int *ao2at, *ao2sh;
+    /* geometry-dependent */
+    double *cn, *dcn, *cnd4, *dcnd4;
+    double *hself;                 /* shell self-energies (Eh) */
+    double *S, *H0, *Drow, *Qrow;  /* n^2, n^2, 3n^2, 6n^2 */
+    double *gam;                   /* nsh^2 */
+    double *mrad;
+    /* D4 */
+    int ndim;
+    int *d4off;
+    double *gw, *dgw, *c6ref;      /* ndim, ndim, ndim^2 */
+    double e_rep;
+} sys_t;
+
+static double hardness(const sys_t *y, int is)
+{
+    const orc_gfn2_elem *p = &orc_gfn2_elems[y->eidx[y->sh[is].at]];
+    return p->hubbard * (1.0 + p->shell_hubbard[y->sh[is].l]);
+}
+
+static int sys_init(sys_t *y, int nat, const int32_t *num, const double *xyz, int charge,
+                    double etemp)
+{
+    memset(y, 0, sizeof(*y));
+    y->nat = nat;
+    y->num = num;
+    y->xyz = xyz;
+    y->charge = charge;
+    y->eidx = (int *)malloc(sizeof(int) * nat);
+    int nsh = 0, nao = 0;
+    double nel = 0.0;
+    for (int i = 0; i < nat; ++i) {
+        int e = elem_index(num[i]);
+        if (e < 0) {
+            free(y->eidx);
+            return 5;
+        }
+        y->eidx[i] = e;
+        nsh += orc_gfn2_elems[e].nsh;
+        for (int k = 0; k < orc_gfn2_elems[e].nsh; ++k) nao += orc_gfn2_elems[e].l[k] ? 3 : 1;
+        nel += orc_gfn2_elems[e].nval;
+    }
+    y->nsh = nsh;
+    y->nao = nao;
+    y->nel = nel - charge;
+    y->nel_int = (int)lrint(y->nel);
+    y->kt = etemp * 3.166808578545117e-06; /* tblite.f90:43 */
+    y->sh = (shell_t *)calloc(nsh, sizeof(shell_t));
+    y->ao2at = (int *)malloc(sizeof(int) * nao);
+    y->ao2sh = (int *)malloc(sizeof(int) * nao);
+    int is = 0, ia = 0;
+    for (int i = 0; i < nat; ++i) {
+        const orc_gfn2_elem *p = &orc_gfn2_elems[y->eidx[i]];
+        for (int k = 0; k < p->nsh; ++k) {
+            shell_t *s = &y->sh[is];
+            s->at = i;
+            s->l = p->l[k];
+            s->ng = p->ng[k];
+            s->nao = s->l ? 3 : 1;
+            s->ao0 = ia;
+            orc_sto_ng(s->ng, p->n[k], s->l, p->zeta[k], s->alpha, s->coeff);
+            for (int c = 0; c < s->nao; ++c) {
+                y->ao2at[ia] = i;
+                y->ao2sh[ia] = is;
+                ++ia;
+            }
+            ++is;
+        }
+    }
+    return 0;
+}
+
+static void sys_free(sys_t *y)
+{
+    free(y->eidx); free(y->sh); free(y->ao2at); free(y->ao2sh);
+    free(y->cn); free(y->dcn); free(y->cnd4); free(y->dcnd4); free(y->hself);
+    free(y->S); free(y->H0); free(y->Drow); free(y->Qrow); free(y->gam); free(y->mrad);
+    free(y->d4off); free(y->gw); free(y->dgw); free(y->c6ref);
+}
+
+/* multipole radius and its CN derivative */
+static void mp_radius(const orc_gfn2_elem *p, double cn, double *r, double *drdcn)
+{
+    double arg = cn - p->mpvcn - ORC_GFN2_AES_SHIFT;
+    double t = exp(-ORC_GFN2_AES_KEXP * arg);
+    *r = p->mprad + (ORC_GFN2_AES_RMAX - p->mprad) / (1.0 + t);
+    *drdcn = (ORC_GFN2_AES_RMAX - p->mprad) * ORC_GFN2_AES_KEXP * t / ((1.0 + t) * (1.0 + t));
+}
+
+static double kshell(int la, int lb)
+{
+    double ka = la ? ORC_GFN2_KSHELL_P : ORC_GFN2_KSHELL_S;
+    double kb = lb ? ORC_GFN2_KSHELL_P : ORC_GFN2_KSHELL_S;
+    return 0.5 * (ka + kb);
+}
+
+/* off-site H0 scaling without the S factor and without (h_a + h_b)/2:
+ * K * X(EN) * Y(zeta) * Pi(R); returns also dPi/dR through *dpi */
+static double h0_scale(const sys_t *y, int isa, int isb, double r, double *pi_r,
+                       double *dpi_dr, double *kxy_out)
+{
+    const shell_t *sa = &y->sh[isa], *sb = &y->sh[isb];
+    const orc_gfn2_elem *pa = &orc_gfn2_elems[y->eidx[sa->at]],
+                        *pb = &orc_gfn2_elems[y->eidx[sb->at]];
+    int ka = (sa->l == pa->l[0]) ? 0 : 1, kb = (sb->l == pb->l[0]) ? 0 : 1;
+    double za = pa->zeta[ka], zb = pb->zeta[kb];
+    double yz = pow(2.0 * sqrt(za * zb) / (za + zb), ORC_GFN2_WEXP);
+    double den = pa->en - pb->en;
+    double xen = 1.0 + ORC_GFN2_ENSCALE * den * den;
+    double rab = orc_aatoau() * (pa->atomic_rad_aa + pb->atomic_rad_aa);
+    double sq = sqrt(r / rab);
+    double fa = 1.0 + pa->shpoly[ka] * sq, fb = 1.0 + pb->shpoly[kb] * sq;
+    *pi_r = fa * fb;
+    double dsq = (r > 0.0) ? 0.5 / sqrt(r * rab) : 0.0;
+    *dpi_dr = (pa->shpoly[ka] * fb + pb->shpoly[kb] * fa) * dsq;
+    double kxy = kshell(sa->l, sb->l) * xen * yz;
+    *kxy_out = kxy;
+    return kxy * (*pi_r);
+}
+
+static void build_geometry(sys_t *y)
+{
+    int nat = y->nat, n = y->nao, nsh = y->nsh;
+    y->cn = (double *)malloc(sizeof(double) * nat);
+    y->dcn = (double *)malloc(sizeof(double) * nat * nat);
+    y->cnd4 = (double *)malloc(sizeof(double) * nat);
+    y->dcnd4 = (double *)malloc(sizeof(double) * nat * nat);
+    orc_gfn2_cn(nat, y->num, y->xyz, y->cn, y->dcn);
+    orc_d4_cn(nat, y->num, y->xyz, y->cnd4, y->dcnd4);
+    y->hself = (double *)malloc(sizeof(double) * nsh);
+    for (int is = 0; is < nsh; ++is) {
+        const orc_gfn2_elem *p = &orc_gfn2_elems[y->eidx[y->sh[is].at]];
+        int k = (y->sh[is].l == p->l[0]) ? 0 : 1;
+        y->hself[is] = (p->selfenergy[k] - p->kcn[k] * y->cn[y->sh[is].at]) * EVTOAU;
+    }
+    y->S = (double *)calloc((size_t)n * n, sizeof(double));
+    y->H0 = (double *)calloc((size_t)n * n, sizeof(double));
+    y->Drow = (double *)calloc(3 * (size_t)n * n, sizeof(double));
+    y->Qrow = (double *)calloc(6 * (size_t)n * n, sizeof(double));
+    double s[9], dpb[27], qpb[54], dpa[27], qpa[54], qtb[54], qta[54];
+    for (int ia = 0; ia < nsh; ++ia)
+        for (int ib = 0; ib <= ia; ++ib) {
+            const shell_t *sa = &y->sh[ia], *sb = &y->sh[ib];
+            const double *ra = y->xyz + 3 * sa->at, *rb = y->xyz + 3 * sb->at;
+            shellpair(sa, sb, ra, rb, s, dpb, qpb, NULL, NULL, NULL);
+            double t[3] = {rb[0] - ra[0], rb[1] - ra[1], rb[2] - ra[2]};
+            shift_moments(t, s, dpb, qpb, dpa, qpa);
+            traceless(qpb, qtb, 9);
+            traceless(qpa, qta, 9);
+            double r = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+            double hs = 0.0;
+            if (sa->at != sb->at) {
+                double pir, dpir, kxy;
+                hs = 0.5 * (y->hself[ia] + y->hself[ib]) * h0_scale(y, ia, ib, r, &pir, &dpir, &kxy);
+            }
+            for (int a = 0; a < sa->nao; ++a)
+                for (int b = 0; b < sb->nao; ++b) {
+                    int mu = sa->ao0 + a, nu = sb->ao0 + b, o = a * 3 + b;
+                    y->S[(size_t)mu * n + nu] = y->S[(size_t)nu * n + mu] = s[o];
+                    if (sa->at != sb->at) {
+                        y->H0[(size_t)mu * n + nu] = y->H0[(size_t)nu * n + mu] = hs * s[o];
+                    }
+                    for (int c = 0; c < 3; ++c) {
+                        y->Drow[(size_t)c * n * n + (size_t)mu * n + nu] = dpa[c * 9 + o];
+                        y->Drow[(size_t)c * n * n + (size_t)nu * n + mu] = dpb[c * 9 + o];
+                    }
+                    for (int k = 0; k < 6; ++k) {
+                        y->Qrow[(size_t)k * n * n + (size_t)mu * n + nu] = qta[k * 9 + o];
+                        y->Qrow[(size_t)k * n * n + (size_t)nu * n + mu] = qtb[k * 9 + o];
+                    }
+                }
+        }
+    for (int is = 0; is < nsh; ++is)
+        for (int a = 0; a < y->sh[is].nao; ++a) {
+            int mu = y->sh[is].ao0 + a;
+            y->H0[(size_t)mu * n + mu] = y->hself[is];
+        }
+    /* ES2 gamma */
+    y->gam = (double *)malloc(sizeof(double) * nsh * nsh);
+    for (int ia = 0; ia < nsh; ++ia)
+        for (int ib = 0; ib < nsh; ++ib) {
+            const double *ra = y->xyz + 3 * y->sh[ia].at, *rb = y->xyz + 3 * y->sh[ib].at;
+            double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
+            double r2 = dx * dx + dy * dy + dz * dz;
+            double eta = 0.5 * (hardness(y, ia) + hardness(y, ib));
+            y->gam[ia * nsh + ib] = 1.0 / sqrt(r2 + 1.0 / (eta * eta));
+        }
+    y->mrad = (double *)malloc(sizeof(double) * nat);
+    for (int i = 0; i < nat; ++i) {
+        double d;
+        mp_radius(&orc_gfn2_elems[y->eidx[i]], y->cn[i], &y->mrad[i], &d);
+    }
+    /* repulsion */
+    double erep = 0.0;
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j < i; ++j) {
+            const orc_gfn2_elem *pi = &orc_gfn2_elems[y->eidx[i]], *pj = &orc_gfn2_elems[y->eidx[j]];
+            double dx = y->xyz[3 * i] - y->xyz[3 * j], dy = y->xyz[3 * i + 1] - y->xyz[3 * j + 1],
+                   dz = y->xyz[3 * i + 2] - y->xyz[3 * j + 2];
+            double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (r > ORC_GFN2_REP_CUTOFF) continue;
+            double kexp = (y->num[i] <= 2 && y->num[j] <= 2) ? ORC_GFN2_REP_KEXP_LIGHT
+                                                               : ORC_GFN2_REP_KEXP;
+            erep += pi->rep_zeff * pj->rep_zeff / r *
+                    exp(-sqrt(pi->rep_alpha * pj->rep_alpha) * pow(r, kexp));
+        }
+    y->e_rep = erep;
+    /* D4 reference weights */
+    d4_setup();
+    y->d4off = (int *)malloc(sizeof(int) * (nat + 1));
+    int nd = 0;
+    for (int i = 0; i < nat; ++i) {
+        y->d4off[i] = nd;
+        nd += orc_d4_refn[y->eidx[i]];
+    }
+    y->d4off[nat] = nd;
+    y->ndim = nd;
+    y->gw = (double *)malloc(sizeof(double) * nd);
+    y->dgw = (double *)malloc(sizeof(double) * nd);
+    y->c6ref = (double *)malloc(sizeof(double) * nd * nd);
+    for (int i = 0; i < nat; ++i) {
+        int e = y->eidx[i], nref = orc_d4_refn[e];
+        double cn = y->cnd4[i], norm = 0.0, dnorm = 0.0;
+        for (int ii = 0; ii < nref; ++ii)
+            for (int iii = 1; iii <= d4_refc[e][ii]; ++iii) {
+                double twf = iii * ORC_D4_WF, rc = lit(orc_d4_refcovcn[e][ii]);
+                double tgw = exp(-twf * (cn - rc) * (cn - rc));
+                norm += tgw;
+                dnorm += 2 * twf * (rc - cn) * tgw;
+            }
+        norm = 1.0 / norm;
+        for (int ii = 0; ii < nref; ++ii) {
+            double expw = 0.0, dexpw = 0.0, rc = lit(orc_d4_refcovcn[e][ii]);
+            for (int iii = 1; iii <= d4_refc[e][ii]; ++iii) {
+                double twf = iii * ORC_D4_WF;
+                double tgw = exp(-twf * (cn - rc) * (cn - rc));
+                expw += tgw;
+                dexpw += 2 * twf * (rc - cn) * tgw;
+            }
+            int k = y->d4off[i] + ii;
+            double g = expw * norm;
+            if (g != g) {
+                double mx = -1e300;
+                for (int jj = 0; jj < nref; ++jj)
+                    if (lit(orc_d4_refcovcn[e][jj]) > mx) mx = lit(orc_d4_refcovcn[e][jj]);
+                g = (mx == rc) ? 1.0 : 0.0;
+            }
+            double dg = dexpw * norm - expw * dnorm * norm * norm;
+            if (dg != dg) dg = 0.0;
+            y->gw[k] = g;
+            y->dgw[k] = dg;
+        }
+    }
+    for (int i = 0; i < nat; ++i)
+        for (int ii = 0; ii < orc_d4_refn[y->eidx[i]]; ++ii)
+            for (int j = 0; j < nat; ++j)
+                for (int jj = 0; jj < orc_d4_refn[y->eidx[j]]; ++jj)
+                    y->c6ref[(y->d4off[i] + ii) * nd + y->d4off[j] + jj] =
+                        d4_c6ref(y->eidx[i], ii, y->eidx[j], jj);
+}
+
+/* ------------------------------------------------------------------- D4 */
+static void d4_zvec(const sys_t *y, const double *q, double *z, double *dz)
+{
+    for (int i = 0; i < y->nat; ++i) {
+        int e = y->eidx[i], zi = y->num[i];
+        double iz = d4_zeff(zi), gi = d4_gam(zi) * ORC_D4_GC;
+        for (int ii = 0; ii < orc_d4_refn[e]; ++ii) {
+            int k = y->d4off[i] + ii;
+            double qi = q ? q[i] : 0.0;
+            z[k] = d4_zeta(ORC_D4_GA, gi, lit(orc_d4_refq[e][ii]) + iz, qi + iz);
+            if (dz) dz[k] = d4_dzeta(ORC_D4_GA, gi, lit(orc_d4_refq[e][ii]) + iz, qi + iz);
+        }
+    }
+}
+
+/* Two-body D4 at atomic charges q (dispgrad, dftd4.f90:1084-1337).  dedq:
+ * SCC potential dE/dq; grad (optional): explicit gradient at fixed q with
+ * the CN chain rule added. */
+static double d4_twobody(const sys_t *y, const double *q, double *dedq, double *grad,
+                         double *dedcn)
+{
+    int nat = y->nat, nd = y->ndim;
+    double *z = (double *)malloc(sizeof(double) * nd), *dz = (double *)malloc(sizeof(double) * nd);
+    d4_zvec(y, q, z, dz);
+    double e = 0.0;
+    if (dedq)
+        for (int i = 0; i < nat; ++i) dedq[i] = 0.0;
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j < i; ++j) {
+            double rij[3] = {y->xyz[3 * j] - y->xyz[3 * i], y->xyz[3 * j + 1] - y->xyz[3 * i + 1],
+                             y->xyz[3 * j + 2] - y->xyz[3 * i + 2]};
+            double r2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+            if (r2 > ORC_D4_R_THR) continue;
+            double c6 = 0.0, dqi = 0.0, dqj = 0.0, dci = 0.0, dcj = 0.0;
+            for (int ii = y->d4off[i]; ii < y->d4off[i + 1]; ++ii)
+                for (int jj = y->d4off[j]; jj < y->d4off[j + 1]; ++jj) {
+                    double c = y->c6ref[ii * nd + jj];
+                    double zi = z[ii] * y->gw[ii], zj = z[jj] * y->gw[jj];
+                    c6 += zi * zj * c;
+                    dqi += dz[ii] * y->gw[ii] * zj * c;
+                    dqj += zi * dz[jj] * y->gw[jj] * c;
+                    dci += z[ii] * y->dgw[ii] * zj * c;
+                    dcj += zi * z[jj] * y->dgw[jj] * c;
+                }
+            double r = sqrt(r2);
+            double r4r2ij = 3.0 * d4_r4r2(y->num[i]) * d4_r4r2(y->num[j]);
+            double r0 = ORC_D4_A1 * sqrt(r4r2ij) + ORC_D4_A2;
+            double oor6 = 1.0 / (r2 * r2 * r2 + pow(r0, 6));
+            double oor8 = 1.0 / (r2 * r2 * r2 * r2 + pow(r0, 8));
+            double disp = ORC_D4_S6 * oor6 + ORC_D4_S8 * r4r2ij * oor8;
+            e -= c6 * disp;
+            if (dedq) {
+                dedq[i] -= dqi * disp;
+                dedq[j] -= dqj * disp;
+            }
+            if (dedcn) {
+                dedcn[i] -= dci * disp;
+                dedcn[j] -= dcj * disp;
+            }
+            if (grad) {
+                double door6 = -6.0 * r2 * r2 * r * oor6 * oor6;
+                double door8 = -8.0 * r2 * r2 * r2 * r * oor8 * oor8;
+                double dedr = -c6 * (ORC_D4_S6 * door6 + ORC_D4_S8 * r4r2ij * door8);
+                for (int x = 0; x < 3; ++x) {
+                    grad[3 * i + x] -= dedr * rij[x] / r;
+                    grad[3 * j + x] += dedr * rij[x] / r;
+                }
+            }
+        }
+    free(z);
+    free(dz);
+    return e;
+}
+
+/* ATM three-body at zero charges (abcappr/dabcappr, dftd4.f90:1463-1780) */
+static double d4_atm(const sys_t *y, double *grad, double *dedcn)
+{
+    int nat = y->nat, nd = y->ndim;
+    double *z = (double *)malloc(sizeof(double) * nd);
+    d4_zvec(y, NULL, z, NULL);
+    double *c6 = (double *)calloc((size_t)nat * nat, sizeof(double));
+    double *dc6 = (double *)calloc((size_t)nat * nat, sizeof(double)); /* dC6_ij/dCN_i */
+    double *r2m = (double *)calloc((size_t)nat * nat, sizeof(double));
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j < nat; ++j) {
+            if (i == j) continue;
+            double dx = y->xyz[3 * i] - y->xyz[3 * j], dy = y->xyz[3 * i + 1] - y->xyz[3 * j + 1],
+                   dz = y->xyz[3 * i + 2] - y->xyz[3 * j + 2];
+            r2m[i * nat + j] = dx * dx + dy * dy + dz * dz;
+            double c = 0.0, dc = 0.0;
+            for (int ii = y->d4off[i]; ii < y->d4off[i + 1]; ++ii)
+                for (int jj = y->d4off[j]; jj < y->d4off[j + 1]; ++jj) {
+                    double cr = y->c6ref[ii * nd + jj];
+                    c += z[ii] * y->gw[ii] * z[jj] * y->gw[jj] * cr;
+                    dc += z[ii] * y->dgw[ii] * z[jj] * y->gw[jj] * cr;
+                }
+            c6[i * nat + j] = c;
+            dc6[i * nat + j] = dc;
+        }
+    double e = 0.0;
+    const double alp3 = ORC_D4_ALP / 3.0;
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j < i; ++j) {
+            double u = r2m[i * nat + j];
+            if (u > ORC_D4_ATM_THR) continue;
+            for (int k = 0; k < j; ++k) {
+                double w = r2m[i * nat + k], v = r2m[j * nat + k];
+                if (w > ORC_D4_ATM_THR || v > ORC_D4_ATM_THR) continue;
+                int zi = y->num[i], zj = y->num[j], zk = y->num[k];
+                double cij = ORC_D4_A1 * sqrt(3.0 * d4_r4r2(zi) * d4_r4r2(zj)) + ORC_D4_A2;
+                double cik = ORC_D4_A1 * sqrt(3.0 * d4_r4r2(zi) * d4_r4r2(zk)) + ORC_D4_A2;
+                double cjk = ORC_D4_A1 * sqrt(3.0 * d4_r4r2(zj) * d4_r4r2(zk)) + ORC_D4_A2;
+                double uvw = u * v * w, rijk = sqrt(uvw), cijk = cij * cjk * cik;
+                double a = u + v - w, b = u + w - v, c = w + v - u;
+                double tt = a * b * c / uvw;
+                double atm = (0.375 * tt + 1.0) / (rijk * rijk * rijk);
+                double x = pow(cijk / rijk, alp3);
+                double fdmp = 1.0 / (1.0 + 6.0 * x);
+                double c9 = ORC_D4_S9 * sqrt(c6[i * nat + j] * c6[j * nat + k] * c6[i * nat + k]);
+                double eabc = c9 * atm * fdmp;
+                e += eabc;
+                if (dedcn) {
+                    dedcn[i] += 0.5 * eabc * (dc6[i * nat + j] / c6[i * nat + j] +
+                                              dc6[i * nat + k] / c6[i * nat + k]);
+                    dedcn[j] += 0.5 * eabc * (dc6[j * nat + i] / c6[i * nat + j] +
+                                              dc6[j * nat + k] / c6[j * nat + k]);
+                    dedcn[k] += 0.5 * eabc * (dc6[k * nat + i] / c6[i * nat + k] +
+                                              dc6[k * nat + j] / c6[j * nat + k]);
+                }
+                if (grad) {
+                    /* derivatives with respect to the squared distances u, v, w */
+                    double r3 = pow(uvw, -1.5);
+                    double dtt_du = (b * c + a * c - a * b) / uvw - tt / u;
+                    double dtt_dv = (b * c - a * c + a * b) / uvw - tt / v;
+                    double dtt_dw = (-b * c + a * c + a * b) / uvw - tt / w;
+                    double datm_du = 0.375 * dtt_du * r3 - 1.5 * (0.375 * tt + 1.0) * r3 / u;
+                    double datm_dv = 0.375 * dtt_dv * r3 - 1.5 * (0.375 * tt + 1.0) * r3 / v;
+                    double datm_dw = 0.375 * dtt_dw * r3 - 1.5 * (0.375 * tt + 1.0) * r3 / w;
+                    double df = fdmp * fdmp * 6.0 * alp3 * x * 0.5; /* times 1/u etc. */
+                    double de_du = c9 * (datm_du * fdmp + atm * df / u);
+                    double de_dv = c9 * (datm_dv * fdmp + atm * df / v);
+                    double de_dw = c9 * (datm_dw * fdmp + atm * df / w);
+                    /* u = |ri-rj|^2, v = |rj-rk|^2, w = |ri-rk|^2 */
+                    for (int xx = 0; xx < 3; ++xx) {
+                        double dij = y->xyz[3 * i + xx] - y->xyz[3 * j + xx];
+                        double djk = y->xyz[3 * j + xx] - y->xyz[3 * k + xx];
+                        double dik = y->xyz[3 * i + xx] - y->xyz[3 * k + xx];
+                        grad[3 * i + xx] += 2.0 * (de_du * dij + de_dw * dik);
+                        grad[3 * j + xx] += 2.0 * (-de_du * dij + de_dv * djk);
+                        grad[3 * k + xx] += 2.0 * (-de_dv * djk - de_dw * dik);
+                    }
+                }
+            }
+        }
+    free(z);
+    free(c6);
+    free(dc6);
+    free(r2m);
+    return e;
+}
+
+double orc_d4_twobody(int nat, const int32_t *num, const double *xyz, const double *q,
+                      double *grad, double *dedq)
+{
+    sys_t y;
+    if (sys_init(&y, nat, num, xyz, 0, 300.0)) return NAN;
+    build_geometry(&y);
+    double *dedcn = (double *)calloc(nat, sizeof(double));
+    double e = d4_twobody(&y, q, dedq, grad, dedcn);
+    if (grad)
+        for (int i = 0; i < nat; ++i)
+            for (int j = 0; j < i; ++j) {
+                double d = y.dcnd4[i * nat + j];
+                if (d == 0.0) continue;
+                double rij[3], r2 = 0.0;
+                for (int x = 0; x < 3; ++x) {
+                    rij[x] = xyz[3 * i + x] - xyz[3 * j + x];
+                    r2 += rij[x] * rij[x];
+                }
+                double r = sqrt(r2), f = (dedcn[i] + dedcn[j]) * d / r;
+                for (int x = 0; x < 3; ++x) {
+                    grad[3 * i + x] += f * rij[x];
+                    grad[3 * j + x] -= f * rij[x];
+                }
+            }
+    free(dedcn);
+    sys_free(&y);
+    return e;
+}
+
+double orc_d4_atm(int nat, const int32_t *num, const double *xyz, double *grad)
+{
+    sys_t y;
+    if (sys_init(&y, nat, num, xyz, 0, 300.0)) return NAN;
+    build_geometry(&y);
+    double *dedcn = (double *)calloc(nat, sizeof(double));
+    double e = d4_atm(&y, grad, dedcn);
+    if (grad)
+        for (int i = 0; i < nat; ++i)
+            for (int j = 0; j < i; ++j) {
+                double d = y.dcnd4[i * nat + j];
+                if (d == 0.0) continue;
+                double rij[3], r2 = 0.0;
+                for (int x = 0; x < 3; ++x) {
+                    rij[x] = xyz[3 * i + x] - xyz[3 * j + x];
+                    r2 += rij[x] * rij[x];
+                }
+                double r = sqrt(r2), f = (dedcn[i] + dedcn[j]) * d / r;
+                for (int x = 0; x < 3; ++x) {
+                    grad[3 * i + x] += f * rij[x];
+                    grad[3 * j + x] -= f * rij[x];
+                }
+            }
+    free(dedcn);
+    sys_free(&y);
+    return e;
+}
+
+/* ---------------------------------------------------------- electrostatics */
+/* Variables x = [qsh(nsh), dp(3 nat), qp(6 nat)].  Computes the interaction
+ * energy (ES2 + ES3 + AES + AXC + D4 two-body) and the potentials
+ * vs = dE/dqsh, vd = dE/ddp, vq = dE/dqp. */
+typedef struct {
+    double es2, es3, aes, axc, d4;
+} eparts_t;
+
+static void aes_pair_terms(double r, double r0, double *h3, double *h5)
+{
+    *h3 = 1.0 / (r * r * r + 6.0 * r0 * r0 * r0);
+    *h5 = 1.0 / (r * (r * r * r * r + 6.0 * r0 * r0 * r0 * r0));
+}
+
+static double theta_rr(const double *t, const double *rv)
+{
+    return t[0] * rv[0] * rv[0] + 2 * t[1] * rv[0] * rv[1] + t[2] * rv[1] * rv[1] +
+           2 * t[3] * rv[0] * rv[2] + 2 * t[4] * rv[1] * rv[2] + t[5] * rv[2] * rv[2];
+}
+
+static void interaction(const sys_t *y, const double *x, double *vs, double *vd, double *vq,
+                        eparts_t *ep)
+{
+    int nat = y->nat, nsh = y->nsh;
+    const double *qsh = x, *dp = x + nsh, *qp = x + nsh + 3 * nat;
+    double *qat = (double *)calloc(nat, sizeof(double));
+    double *vat = (double *)calloc(nat, sizeof(double));
+    for (int is = 0; is < nsh; ++is) qat[y->sh[is].at] += qsh[is];
+    memset(vd, 0, sizeof(double) * 3 * nat);
+    memset(vq, 0, sizeof(double) * 6 * nat);
+    double es2 = 0.0, es3 = 0.0, aes = 0.0, axc = 0.0;
+    for (int a = 0; a < nsh; ++a) {
+        double v = 0.0;
+        for (int b = 0; b < nsh; ++b) v += y->gam[a * nsh + b] * qsh[b];
+        vs[a] = v;
+        es2 += 0.5 * qsh[a] * v;
+        const orc_gfn2_elem *p = &orc_gfn2_elems[y->eidx[y->sh[a].at]];
+        double g3 = p->hubbard_deriv * (y->sh[a].l ? ORC_GFN2_THIRD_P : ORC_GFN2_THIRD_S);
+        es3 += g3 * qsh[a] * qsh[a] * qsh[a] / 3.0;
+        vs[a] += g3 * qsh[a] * qsh[a];
+    }
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j < i; ++j) {
+            double rv[3] = {y->xyz[3 * i] - y->xyz[3 * j], y->xyz[3 * i + 1] - y->xyz[3 * j + 1],
+                            y->xyz[3 * i + 2] - y->xyz[3 * j + 2]}; /* R_i - R_j */
+            double r = sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+            double r0 = 0.5 * (y->mrad[i] + y->mrad[j]), h3, h5;
+            aes_pair_terms(r, r0, &h3, &h5);
+            double h5b = r * r * h5;
+            const double *mi = dp + 3 * i, *mj = dp + 3 * j, *ti = qp + 6 * i, *tj = qp + 6 * j;
+            double mjr = mj[0] * rv[0] + mj[1] * rv[1] + mj[2] * rv[2];
+            double mir = mi[0] * rv[0] + mi[1] * rv[1] + mi[2] * rv[2];
+            double mimj = mi[0] * mj[0] + mi[1] * mj[1] + mi[2] * mj[2];
+            double tjrr = theta_rr(tj, rv), tirr = theta_rr(ti, rv);
+            /* q_i mu_j.R_ij + q_j mu_i.R_ji + dipole-dipole + q_i Th_j:RR + q_j Th_i:RR */
+            aes += qat[i] * mjr * h3 - qat[j] * mir * h3 + mimj * h5b - 3.0 * mir * mjr * h5 +
+                   qat[i] * tjrr * h5 + qat[j] * tirr * h5;
+            vat[i] += mjr * h3 + tjrr * h5;
+            vat[j] += -mir * h3 + tirr * h5;
+            for (int c = 0; c < 3; ++c) {
+                vd[3 * i + c] += -qat[j] * rv[c] * h3 + mj[c] * h5b - 3.0 * rv[c] * mjr * h5;
+                vd[3 * j + c] += qat[i] * rv[c] * h3 + mi[c] * h5b - 3.0 * rv[c] * mir * h5;
+            }
+            double rr[6] = {rv[0] * rv[0], 2 * rv[0] * rv[1], rv[1] * rv[1],
+                            2 * rv[0] * rv[2], 2 * rv[1] * rv[2], rv[2] * rv[2]};
+            for (int k = 0; k < 6; ++k) {
+                vq[6 * i + k] += qat[j] * rr[k] * h5;
+                vq[6 * j + k] += qat[i] * rr[k] * h5;
+            }
+        }
+    for (int i = 0; i < nat; ++i) {
+        const orc_gfn2_elem *p = &orc_gfn2_elems[y->eidx[i]];
+        const double *mi = dp + 3 * i, *ti = qp + 6 * i;
+        axc += p->dkernel * (mi[0] * mi[0] + mi[1] * mi[1] + mi[2] * mi[2]);
+        axc += p->qkernel * (ti[0] * ti[0] + ti[2] * ti[2] + ti[5] * ti[5] +
+                             2.0 * (ti[1] * ti[1] + ti[3] * ti[3] + ti[4] * ti[4]));
+        for (int c = 0; c < 3; ++c) vd[3 * i + c] += 2.0 * p->dkernel * mi[c];
+        vq[6 * i + 0] += 2.0 * p->qkernel * ti[0];
+        vq[6 * i + 1] += 4.0 * p->qkernel * ti[1];
+        vq[6 * i + 2] += 2.0 * p->qkernel * ti[2];
+        vq[6 * i + 3] += 4.0 * p->qkernel * ti[3];
+        vq[6 * i + 4] += 4.0 * p->qkernel * ti[4];
+        vq[6 * i + 5] += 2.0 * p->qkernel * ti[5];
+    }
+    double *dedq = (double *)calloc(nat, sizeof(double));
+    double ed4 = d4_twobody(y, qat, dedq, NULL, NULL);
+    for (int i = 0; i < nat; ++i) vat[i] += dedq[i];
+    for (int a = 0; a < nsh; ++a) vs[a] += vat[y->sh[a].at];
+    ep->es2 = es2;
+    ep->es3 = es3;
+    ep->aes = aes;
+    ep->axc = axc;
+    ep->d4 = ed4;
+    free(qat);
+    free(vat);
+    free(dedq);
+}
+
+/* Mulliken shell charges and atomic multipoles from P */
+static void moments(const sys_t *y, const double *P, double *x)
+{
+    int n = y->nao, nat = y->nat, nsh = y->nsh;
+    double *qsh = x, *dp = x + nsh, *qp = x + nsh + 3 * nat;
+    for (int is = 0; is < nsh; ++is) {
+        const orc_gfn2_elem *p = &orc_gfn2_elems[y->eidx[y->sh[is].at]];
+        int k = (y->sh[is].l == p->l[0]) ? 0 : 1;
+        qsh[is] = p->refocc[k];
+    }
+    memset(dp, 0, sizeof(double) * 9 * nat);
+    for (int mu = 0; mu < n; ++mu) {
+        int a = y->ao2at[mu], s = y->ao2sh[mu];
+        double pop = 0.0, d[3] = {0, 0, 0}, q[6] = {0, 0, 0, 0, 0, 0};
+        for (int nu = 0; nu < n; ++nu) {
+            double pmn = P[(size_t)mu * n + nu];
+            pop += pmn * y->S[(size_t)mu * n + nu];
+            for (int c = 0; c < 3; ++c) d[c] += pmn * y->Drow[(size_t)c * n * n + (size_t)mu * n + nu];
+            for (int k = 0; k < 6; ++k) q[k] += pmn * y->Qrow[(size_t)k * n * n + (size_t)mu * n + nu];
+        }
+        qsh[s] -= pop;
+        for (int c = 0; c < 3; ++c) dp[3 * a + c] -= d[c];
+        for (int k = 0; k < 6; ++k) qp[6 * a + k] -= q[k];
+    }
+}
+
+static void fock(const sys_t *y, const double *vs, const double *vd, const double *vq, double *F)
+{
+    int n = y->nao, nn = n * n;
+    for (int mu = 0; mu < n; ++mu) {
+        int am = y->ao2at[mu], sm = y->ao2sh[mu];
+        for (int nu = 0; nu < n; ++nu) {
+            int an = y->ao2at[nu], sn = y->ao2sh[nu];
+            size_t o = (size_t)mu * n + nu, ot = (size_t)nu * n + mu;
+            double v = y->S[o] * (vs[sm] + vs[sn]);
+            for (int c = 0; c < 3; ++c)
+                v += y->Drow[c * nn + o] * vd[3 * am + c] + y->Drow[c * nn + ot] * vd[3 * an + c];
+            for (int k = 0; k < 6; ++k)
+                v += y->Qrow[k * nn + o] * vq[6 * am + k] + y->Qrow[k * nn + ot] * vq[6 * an + k];
+            F[o] = y->H0[o] - 0.5 * v;
+        }
+    }
+}
+
+/* Fermi occupations (per spin) for nalpha electrons; returns -T*S (both spins) */
+static double fermi(int n, const double *e, double kt, double nalpha, double *f)
+{
+    double lo = e[0] - 50.0 * kt - 1.0, hi = e[n - 1] + 50.0 * kt + 1.0, mu = 0.0;
+    for (int it = 0; it < 200; ++it) {
+        mu = 0.5 * (lo + hi);
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += 1.0 / (1.0 + exp((e[i] - mu) / kt));
+        if (s > nalpha) hi = mu;
+        else lo = mu;
+        if (hi - lo < 1e-15 * (fabs(mu) + 1.0)) break;
+    }
+    /* Newton polish */
+    for (int it = 0; it < 5; ++it) {
+        double s = 0.0, ds = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double fi = 1.0 / (1.0 + exp((e[i] - mu) / kt));
+            s += fi;
+            ds += fi * (1.0 - fi) / kt;
+        }
+        if (ds <= 0.0) break;
+        mu += (nalpha - s) / ds;
+    }
+    double ts = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double fi = 1.0 / (1.0 + exp((e[i] - mu) / kt));
+        f[i] = fi;
+        if (fi > 0.0 && fi < 1.0) ts += fi * log(fi) + (1.0 - fi) * log(1.0 - fi);
+    }
+    return 2.0 * kt * ts;
+}
+
+/* ------------------------------------------------------------ Broyden mixer */
+typedef struct {
+    int n, mem, iter;
+    double damp;
+    double *df, *u, *a, *omega, *qlast, *dqlast;
+} broyden_t;
+
+static void broyden_init(broyden_t *b, int n, int mem, double damp)
+{
+    b->n = n;
+    b->mem = mem;
+    b->iter = 0;
+    b->damp = damp;
+    b->df = (double *)calloc((size_t)n * mem, sizeof(double));
+    b->u = (double *)calloc((size_t)n * mem, sizeof(double));
+    b->a = (double *)calloc((size_t)mem * mem, sizeof(double));
+    b->omega = (double *)calloc(mem, sizeof(double));
+    b->qlast = (double *)calloc(n, sizeof(double));
+    b->dqlast = (double *)calloc(n, sizeof(double));
+}
+static void broyden_free(broyden_t *b)
+{
+    free(b->df); free(b->u); free(b->a); free(b->omega); free(b->qlast); free(b->dqlast);
+}
+
+/* invert small symmetric positive matrix (Gauss-Jordan with partial pivoting) */
+static int small_inverse(int m, double *a, double *inv)
+{
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) inv[i * m + j] = (i == j);
+    for (int c = 0; c < m; ++c) {
+        int p = c;
+        for (int r = c + 1; r < m; ++r)
+            if (fabs(a[r * m + c]) > fabs(a[p * m + c])) p = r;
+        if (a[p * m + c] == 0.0) return -1;
+        if (p != c)
+            for (int k = 0; k < m; ++k) {
+                double t = a[c * m + k]; a[c * m + k] = a[p * m + k]; a[p * m + k] = t;
+                t = inv[c * m + k]; inv[c * m + k] = inv[p * m + k]; inv[p * m + k] = t;
+            }
+        double d = 1.0 / a[c * m + c];
+        for (int k = 0; k < m; ++k) { a[c * m + k] *= d; inv[c * m + k] *= d; }
+        for (int r = 0; r < m; ++r) {
+            if (r == c) continue;
+            double f = a[r * m + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < m; ++k) { a[r * m + k] -= f * a[c * m + k]; inv[r * m + k] -= f * inv[c * m + k]; }
+        }
+    }
+    return 0;
+}
+
+/* One modified-Broyden step: q (input vector, updated in place), dq = out - in */
+static void broyden_step(broyden_t *b, double *q, const double *dq)
+{
+    const double omega0 = 0.01, minw = 1.0, maxw = 100000.0, wfac = 0.01;
+    int n = b->n;
+    b->iter += 1;
+    if (b->iter == 1) {
+        for (int i = 0; i < n; ++i) {
+            b->dqlast[i] = dq[i];
+            b->qlast[i] = q[i];
+            q[i] = q[i] + b->damp * dq[i];
+        }
+        return;
+    }
+    int it1 = b->iter - 1; /* number of history vectors, 1-based index of newest */
+    if (it1 > b->mem) it1 = b->mem; /* history full: keep using the last slot */
+    int k1 = it1 - 1;
+    double *df = b->df + (size_t)k1 * n, *u = b->u + (size_t)k1 * n;
+    double nrm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        df[i] = dq[i] - b->dqlast[i];
+        nrm += df[i] * df[i];
+    }
+    double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+    for (int i = 0; i < n; ++i) {
+        df[i] *= inv;
+        u[i] = b->damp * df[i] + inv * (q[i] - b->qlast[i]);
+    }
+    double w = 0.0;
+    for (int i = 0; i < n; ++i) w += dq[i] * dq[i];
+    w = sqrt(w);
+    w = (w > wfac / maxw) ? wfac / w : maxw;
+    if (w < minw) w = minw;
+    b->omega[k1] = w;
+    for (int i = 0; i < it1; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += b->df[(size_t)i * n + j] * df[j];
+        b->a[i * b->mem + k1] = b->a[k1 * b->mem + i] = s;
+    }
+    double *beta = (double *)malloc(sizeof(double) * it1 * it1);
+    double *binv = (double *)malloc(sizeof(double) * it1 * it1);
+    double *cvec = (double *)malloc(sizeof(double) * it1);
+    for (int i = 0; i < it1; ++i) {
+        for (int j = 0; j < it1; ++j)
+            beta[i * it1 + j] = b->omega[i] * b->omega[j] * b->a[i * b->mem + j];
+        beta[i * it1 + i] += omega0 * omega0;
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += b->df[(size_t)i * n + j] * dq[j];
+        cvec[i] = b->omega[i] * s;
+    }
+    small_inverse(it1, beta, binv);
+    for (int i = 0; i < n; ++i) {
+        b->dqlast[i] = dq[i];
+        b->qlast[i] = q[i];
+    }
+    for (int i = 0; i < n; ++i) q[i] += b->damp * dq[i];
+    for (int i = 0; i < it1; ++i) {
+        double g = 0.0;
+        for (int k = 0; k < it1; ++k) g += cvec[k] * binv[k * it1 + i];
+        double f = b->omega[i] * g;
+        for (int j = 0; j < n; ++j) q[j] -= f * b->u[(size_t)i * n + j];
+    }
+    free(beta);
+    free(binv);
+    free(cvec);
+}
+
+void orc_scc_default_opts(orc_scc_opts *o)
+{
+    o->econv = 1.0e-9;
+    o->pconv = 1.0e-8;
+    o->maxiter = 250;
+    o->broyden_damp = 0.4;
+    o->cold_start = 1;
+}
+
+/* --------------------------------------------------------------- driver */
+static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, double etemp,
+                    const orc_scc_opts *opts_in, double *energy, double *grad, double *qat_out,
+                    double *dpat, double *qpat, orc_gfn2_info *info)
+{
+    orc_scc_opts opts;
+    if (opts_in) opts = *opts_in;
+    else orc_scc_default_opts(&opts);
+    sys_t y;
+    int st = sys_init(&y, nat, num, xyz, charge, etemp);
+    if (st) return st;
+    build_geometry(&y);
+    int n = y.nao, nsh = y.nsh, nx = nsh + 9 * nat;
+    double *x = (double *)calloc(nx, sizeof(double)), *xo = (double *)calloc(nx, sizeof(double));
+    double *dx = (double *)calloc(nx, sizeof(double));
+    double *vs = (double *)calloc(nsh, sizeof(double)), *vd = (double *)calloc(3 * nat, sizeof(double));
+    double *vq = (double *)calloc(6 * nat, sizeof(double));
+    double *F = (double *)malloc(sizeof(double) * n * n), *Sw = (double *)malloc(sizeof(double) * n * n);
+    double *P = (double *)malloc(sizeof(double) * n * n);
+    double *eps = (double *)malloc(sizeof(double) * n), *focc = (double *)malloc(sizeof(double) * n);
+    double *work = (double *)malloc(sizeof(double) * (3 * n * n + 2 * n));
+    broyden_t br;
+    broyden_init(&br, nx, opts.maxiter + 1, opts.broyden_damp);
+    double nalpha = 0.5 * y.nel;
+    double eold = 0.0, etot = 0.0, ts = 0.0;
+    eparts_t ep = {0};
+    int conv = 0, it;
+    for (it = 1; it <= opts.maxiter; ++it) {
+        interaction(&y, x, vs, vd, vq, &ep);
+        fock(&y, vs, vd, vq, F);
+        memcpy(Sw, y.S, sizeof(double) * n * n);
+        orc_sygv(n, F, Sw, eps, work);
+        ts = fermi(n, eps, y.kt, nalpha, focc);
+        for (int mu = 0; mu < n; ++mu)
+            for (int nu = 0; nu <= mu; ++nu) {
+                double v = 0.0;
+                for (int k = 0; k < n; ++k) v += F[(size_t)mu * n + k] * focc[k] * F[(size_t)nu * n + k];
+                P[(size_t)mu * n + nu] = P[(size_t)nu * n + mu] = 2.0 * v;
+            }
+        moments(&y, P, xo);
+        interaction(&y, xo, vs, vd, vq, &ep);
+        double eband = 0.0;
+        for (int k = 0; k < n * n; ++k) eband += P[k] * y.H0[k];
+        etot = eband + ep.es2 + ep.es3 + ep.aes + ep.axc + ep.d4 + ts;
+        double err = 0.0;
+        for (int k = 0; k < nx; ++k) {
+            dx[k] = xo[k] - x[k];
+            err += dx[k] * dx[k];
+        }
+        err = sqrt(err / nx);
+        double de = etot - eold;
+        eold = etot;
+        if (it > 1 && fabs(de) < opts.econv && err < opts.pconv) {
+            conv = 1;
+            break;
+        }
+        broyden_step(&br, x, dx);
+    }
+    if (it > opts.maxiter) it = opts.maxiter;
+    /* final state: moments xo from P; potentials vs/vd/vq at xo (computed above) */
+    double eatm_dummy = 0.0;
+    double *dedcn_d4 = (double *)calloc(nat, sizeof(double));
+    double eatm = d4_atm(&y, grad ? grad : NULL, dedcn_d4);
+    (void)eatm_dummy;
+    double eband = 0.0;
+    for (int k = 0; k < n * n; ++k) eband += P[k] * y.H0[k];
+    double etotal = etot + eatm + y.e_rep;
+    if (info) {
+        info->energy = etotal;
+        info->e_rep = y.e_rep;
+        info->e_disp = ep.d4 + eatm;
+        info->e_es2 = ep.es2;
+        info->e_es3 = ep.es3;
+        info->e_aes = ep.aes;
+        info->e_axc = ep.axc;
+        info->e_band = eband;
+        info->e_ts = ts;
+        info->iterations = it;
+        info->converged = conv;
+        info->nao = n;
+        info->nsh = nsh;
+        info->nel_int = y.nel_int;
+    }
+    *energy = etotal;
+    double *qat = (double *)calloc(nat, sizeof(double));
+    for (int is = 0; is < nsh; ++is) qat[y.sh[is].at] += xo[is];
+    if (qat_out) memcpy(qat_out, qat, sizeof(double) * nat);
+    if (dpat) memcpy(dpat, xo + nsh, sizeof(double) * 3 * nat);
+    if (qpat) memcpy(qpat, xo + nsh + 3 * nat, sizeof(double) * 6 * nat);
+
+    if (grad) {
+        /* grad already holds the ATM pair terms (without the CN chain) */
+        double *dedcn = (double *)calloc(nat, sizeof(double)); /* GFN2 CN */
+        /* W = 2 C diag(f e) C^T */
+        double *W = work; /* n^2 */
+        for (int mu = 0; mu < n; ++mu)
+            for (int nu = 0; nu <= mu; ++nu) {
+                double v = 0.0;
+                for (int k = 0; k < n; ++k)
+                    v += F[(size_t)mu * n + k] * focc[k] * eps[k] * F[(size_t)nu * n + k];
+                W[(size_t)mu * n + nu] = W[(size_t)nu * n + mu] = 2.0 * v;
+            }
+        /* repulsion */
+        for (int i = 0; i < nat; ++i)
+            for (int j = 0; j < i; ++j) {
+                const orc_gfn2_elem *pi = &orc_gfn2_elems[y.eidx[i]], *pj = &orc_gfn2_elems[y.eidx[j]];
+                double rv[3] = {xyz[3 * i] - xyz[3 * j], xyz[3 * i + 1] - xyz[3 * j + 1],
+                                xyz[3 * i + 2] - xyz[3 * j + 2]};
+                double r = sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+                if (r > ORC_GFN2_REP_CUTOFF) continue;
+                double kexp = (num[i] <= 2 && num[j] <= 2) ? ORC_GFN2_REP_KEXP_LIGHT : ORC_GFN2_REP_KEXP;
+                double sa = sqrt(pi->rep_alpha * pj->rep_alpha);
+                double e = pi->rep_zeff * pj->rep_zeff / r * exp(-sa * pow(r, kexp));
+                double dedr = e * (-1.0 / r - sa * kexp * pow(r, kexp - 1.0));
+                for (int c = 0; c < 3; ++c) {
+                    grad[3 * i + c] += dedr * rv[c] / r;
+                    grad[3 * j + c] -= dedr * rv[c] / r;
+                }
+            }
+        /* D4 two-body explicit + D4 CN chain (ATM dedcn already in dedcn_d4) */
+        double *qatv = qat;
+        d4_twobody(&y, qatv, NULL, grad, dedcn_d4);
+        for (int i = 0; i < nat; ++i)
+            for (int j = 0; j < i; ++j) {
+                double d = y.dcnd4[i * nat + j];
+                if (d == 0.0) continue;
+                double rv[3], r2 = 0.0;
+                for (int c = 0; c < 3; ++c) {
+                    rv[c] = xyz[3 * i + c] - xyz[3 * j + c];
+                    r2 += rv[c] * rv[c];
+                }
+                double f = (dedcn_d4[i] + dedcn_d4[j]) * d / sqrt(r2);
+                for (int c = 0; c < 3; ++c) {
+                    grad[3 * i + c] += f * rv[c];
+                    grad[3 * j + c] -= f * rv[c];
+                }
+            }
+        /* ES2 gamma derivative */
+        for (int a = 0; a < nsh; ++a)
+            for (int b = 0; b < a; ++b) {
+                int ia = y.sh[a].at, ib = y.sh[b].at;
+                if (ia == ib) continue;
+                double rv[3] = {xyz[3 * ia] - xyz[3 * ib], xyz[3 * ia + 1] - xyz[3 * ib + 1],
+                                xyz[3 * ia + 2] - xyz[3 * ib + 2]};
+                double g = y.gam[a * nsh + b];
+                /* d gamma / d r_vec = -r_vec gamma^3 */
+                double f = -xo[a] * xo[b] * g * g * g;
+                for (int c = 0; c < 3; ++c) {
+                    grad[3 * ia + c] += f * rv[c];
+                    grad[3 * ib + c] -= f * rv[c];
+                }
+            }
+        /* AES explicit terms (fixed moments) and multipole-radius CN chain */
+        const double *dp = xo + nsh, *qp = xo + nsh + 3 * nat;
+        for (int i = 0; i < nat; ++i)
+            for (int j = 0; j < i; ++j) {
+                double rv[3] = {xyz[3 * i] - xyz[3 * j], xyz[3 * i + 1] - xyz[3 * j + 1],
+                                xyz[3 * i + 2] - xyz[3 * j + 2]};
+                double r = sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+                double r0 = 0.5 * (y.mrad[i] + y.mrad[j]), h3, h5;
+                aes_pair_terms(r, r0, &h3, &h5);
+                double r3 = r * r * r, r4 = r3 * r, r04 = r0 * r0 * r0 * r0;
+                double dh3 = -3.0 * r * r * h3 * h3;            /* dh3/dr */
+                double dh5 = -(5.0 * r4 + 6.0 * r04) * h5 * h5;  /* dh5/dr */
+                double h5b = r * r * h5, dh5b = 2.0 * r * h5 + r * r * dh5;
+                double dh3r0 = -18.0 * r0 * r0 * h3 * h3;
+                double dh5r0 = -24.0 * r0 * r0 * r0 * r * h5 * h5;
+                double dh5br0 = r * r * dh5r0;
+                const double *mi = dp + 3 * i, *mj = dp + 3 * j, *ti = qp + 6 * i, *tj = qp + 6 * j;
+                double mjr = mj[0] * rv[0] + mj[1] * rv[1] + mj[2] * rv[2];
+                double mir = mi[0] * rv[0] + mi[1] * rv[1] + mi[2] * rv[2];
+                double mimj = mi[0] * mj[0] + mi[1] * mj[1] + mi[2] * mj[2];
+                double tjrr = theta_rr(tj, rv), tirr = theta_rr(ti, rv);
+                double qi = qat[i], qj = qat[j];
+                /* radial-scalar part: sum of f(r) derivatives times R/r */
+                double srad = (qi * mjr - qj * mir) * dh3 + mimj * dh5b - 3.0 * mir * mjr * dh5 +
+                              (qi * tjrr + qj * tirr) * dh5;
+                double tjr[3], tir[3];
+                for (int c = 0; c < 3; ++c) {
+                    int k0 = c == 0 ? 0 : (c == 1 ? 1 : 3), k1 = c == 0 ? 1 : (c == 1 ? 2 : 4),
+                        k2 = c == 0 ? 3 : (c == 1 ? 4 : 5);
+                    tjr[c] = tj[k0] * rv[0] + tj[k1] * rv[1] + tj[k2] * rv[2];
+                    tir[c] = ti[k0] * rv[0] + ti[k1] * rv[1] + ti[k2] * rv[2];
+                }
+                for (int c = 0; c < 3; ++c) {
+                    double gv = srad * rv[c] / r + (qi * mj[c] - qj * mi[c]) * h3 -
+                                3.0 * (mi[c] * mjr + mj[c] * mir) * h5 +
+                                2.0 * (qi * tjr[c] + qj * tir[c]) * h5;
+                    grad[3 * i + c] += gv;
+                    grad[3 * j + c] -= gv;
+                }
+                double der0 = (qi * mjr - qj * mir) * dh3r0 + mimj * dh5br0 -
+                              3.0 * mir * mjr * dh5r0 + (qi * tjrr + qj * tirr) * dh5r0;
+                double dri, drj;
+                double rtmp;
+                mp_radius(&orc_gfn2_elems[y.eidx[i]], y.cn[i], &rtmp, &dri);
+                mp_radius(&orc_gfn2_elems[y.eidx[j]], y.cn[j], &rtmp, &drj);
+                dedcn[i] += 0.5 * der0 * dri;
+                dedcn[j] += 0.5 * der0 * drj;
+            }
+        /* integral terms over shell pairs on different atoms */
+        double s[9], dpb[27], qpb[54], ds[27], ddp[81], dqp[162];
+        double dpa[27], qpa[54], qta[54], qtb[54], dqta[162], dqtb[162], ddpa[81], dqpa[162];
+        for (int ia = 0; ia < nsh; ++ia) {
+            /* on-site diagonal self-energy contribution to dE/dh */
+            const orc_gfn2_elem *pa = &orc_gfn2_elems[y.eidx[y.sh[ia].at]];
+            int ka = (y.sh[ia].l == pa->l[0]) ? 0 : 1;
+            double dedh = 0.0;
+            for (int a = 0; a < y.sh[ia].nao; ++a) {
+                int mu = y.sh[ia].ao0 + a;
+                dedh += P[(size_t)mu * n + mu];
+            }
+            dedcn[y.sh[ia].at] += dedh * (-pa->kcn[ka] * EVTOAU);
+        }
+        for (int ia = 0; ia < nsh; ++ia)
+            for (int ib = 0; ib < nsh; ++ib) {
+                const shell_t *sa = &y.sh[ia], *sb = &y.sh[ib];
+                if (sa->at <= sb->at) continue; /* unordered atom pairs, A != B */
+                const double *ra = xyz + 3 * sa->at, *rb = xyz + 3 * sb->at;
+                shellpair(sa, sb, ra, rb, s, dpb, qpb, ds, ddp, dqp);
+                double t[3] = {rb[0] - ra[0], rb[1] - ra[1], rb[2] - ra[2]};
+                double r = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+                shift_moments(t, s, dpb, qpb, dpa, qpa);
+                /* derivatives of the moments about A with respect to ra (t = rb - ra) */
+                for (int d = 0; d < 3; ++d)
+                    for (int o = 0; o < 9; ++o) {
+                        for (int c = 0; c < 3; ++c)
+                            ddpa[(d * 3 + c) * 9 + o] = ddp[(d * 3 + c) * 9 + o] -
+                                                        (c == d) * s[o] + t[c] * ds[d * 9 + o];
+                        for (int k = 0; k < 6; ++k) {
+                            int u = qidx[k][0], w = qidx[k][1];
+                            double v = dqp[(d * 6 + k) * 9 + o];
+                            v += -(u == d) * dpb[w * 9 + o] - (w == d) * dpb[u * 9 + o];
+                            v += t[u] * ddp[(d * 3 + w) * 9 + o] + t[w] * ddp[(d * 3 + u) * 9 + o];
+                            v += -(u == d) * t[w] * s[o] - (w == d) * t[u] * s[o];
+                            v += t[u] * t[w] * ds[d * 9 + o];
+                            dqpa[(d * 6 + k) * 9 + o] = v;
+                        }
+                    }
+                traceless(qpa, qta, 9);
+                traceless(qpb, qtb, 9);
+                for (int d = 0; d < 3; ++d) {
+                    traceless(dqpa + d * 54, dqta + d * 54, 9);
+                    traceless(dqp + d * 54, dqtb + d * 54, 9);
+                }
+                double pir, dpir, kxy;
+                double scal = h0_scale(&y, ia, ib, r, &pir, &dpir, &kxy);
+                double hav = 0.5 * (y.hself[ia] + y.hself[ib]);
+                int A = sa->at, B = sb->at;
+                double gA[3] = {0, 0, 0}, psum = 0.0;
+                for (int a = 0; a < sa->nao; ++a)
+                    for (int b = 0; b < sb->nao; ++b) {
+                        int mu = sa->ao0 + a, nu = sb->ao0 + b, o = a * 3 + b;
+                        double pmn = P[(size_t)mu * n + nu], wmn = W[(size_t)mu * n + nu];
+                        double ws = 2.0 * pmn * hav * scal - 2.0 * wmn - pmn * (xo ? 1.0 : 1.0) * (vs[ia] + vs[ib]);
+                        psum += pmn * s[o];
+                        for (int d = 0; d < 3; ++d) {
+                            double g = ws * ds[d * 9 + o];
+                            for (int c = 0; c < 3; ++c)
+                                g -= pmn * (vd[3 * A + c] * ddpa[(d * 3 + c) * 9 + o] +
+                                            vd[3 * B + c] * ddp[(d * 3 + c) * 9 + o]);
+                            for (int k = 0; k < 6; ++k)
+                                g -= pmn * (vq[6 * A + k] * dqta[d * 54 + k * 9 + o] +
+                                            vq[6 * B + k] * dqtb[d * 54 + k * 9 + o]);
+                            gA[d] += g;
+                        }
+                    }
+                /* polynomial Pi(R): dR/dra = (ra - rb)/r = -t/r */
+                double fpi = 2.0 * psum * hav * kxy * dpir;
+                for (int d = 0; d < 3; ++d) gA[d] += fpi * (-t[d] / r);
+                for (int d = 0; d < 3; ++d) {
+                    grad[3 * A + d] += gA[d];
+                    grad[3 * B + d] -= gA[d];
+                }
+                /* dE/dh for both shells: sum_{mu nu} 2 P (1/2) K X Y Pi S */
+                double dh = psum * scal;
+                const orc_gfn2_elem *pa = &orc_gfn2_elems[y.eidx[A]], *pb = &orc_gfn2_elems[y.eidx[B]];
+                int ka = (sa->l == pa->l[0]) ? 0 : 1, kb = (sb->l == pb->l[0]) ? 0 : 1;
+                dedcn[A] += dh * (-pa->kcn[ka] * EVTOAU);
+                dedcn[B] += dh * (-pb->kcn[kb] * EVTOAU);
+            }
+        /* GFN2 CN chain */
+        for (int i = 0; i < nat; ++i)
+            for (int j = 0; j < i; ++j) {
+                double d = y.dcn[i * nat + j];
+                if (d == 0.0) continue;
+                double rv[3], r2 = 0.0;
+                for (int c = 0; c < 3; ++c) {
+                    rv[c] = xyz[3 * i + c] - xyz[3 * j + c];
+                    r2 += rv[c] * rv[c];
+                }
+                double f = (dedcn[i] + dedcn[j]) * d / sqrt(r2);
+                for (int c = 0; c < 3; ++c) {
+                    grad[3 * i + c] += f * rv[c];
+                    grad[3 * j + c] -= f * rv[c];
+                }
+            }
+        free(dedcn);
+    }
+    free(dedcn_d4);
+    free(qat);
+    free(x); free(xo); free(dx); free(vs); free(vd); free(vq);
+    free(F); free(Sw); free(P); free(eps); free(focc); free(work);
+    broyden_free(&br);
+    sys_free(&y);
+    return conv ? 0 : -1;
+}
+
+int orc_gfn2_egrad(int nat, const int32_t *num, const double *xyz, int charge, double etemp,
+                   const orc_scc_opts *opts, double *energy, double *grad, double *qat,
+                   double *dipole_at, double *qpole_at, orc_gfn2_info *info)
+{
+    if (grad) memset(grad, 0, sizeof(double) * 3 * nat);
+    return gfn2_run(nat, num, xyz, charge, etemp, opts, energy, grad, qat, dipole_at, qpole_at,
+                    info);
+}
+
+int orc_gfn2_energy(int nat, const int32_t *num, const double *xyz, int charge, double etemp,
+                    const orc_scc_opts *opts, double *energy, double *qat)
+{
+    return gfn2_run(nat, num, xyz, charge, etemp, opts, energy, NULL, qat, NULL, NULL, NULL);
+}
+
+/* matrices for unit tests: S, H0 (n^2), Drow (3 n^2), Qrow (6 n^2) */
+int orc_gfn2_matrices(int nat, const int32_t *num, const double *xyz, double *s, double *h0,
+                      double *dpint, double *qpint, int *nao_out)
+{
+    sys_t y;
+    int st = sys_init(&y, nat, num, xyz, 0, 300.0);
+    if (st) return st;
+    *nao_out = y.nao;
+    if (s || h0 || dpint || qpint) {
+        build_geometry(&y);
+        size_t nn = (size_t)y.nao * y.nao;
+        if (s) memcpy(s, y.S, sizeof(double) * nn);
+        if (h0) memcpy(h0, y.H0, sizeof(double) * nn);
+        if (dpint) memcpy(dpint, y.Drow, sizeof(double) * 3 * nn);
+        if (qpint) memcpy(qpint, y.Qrow, sizeof(double) * 6 * nn);
+    }
+    sys_free(&y);
+    return 0;
+}
+
+/* ------------------------------------------------------------ batch driver */
+typedef struct {
+    int nsys;
+    const int32_t *offsets, *num, *charge;
+    const double *xyz, *etemp;
+    const orc_scc_opts *opts;
+    double *energy, *grad, *qat;
+    int32_t *stat;
+    int next;
+    pthread_mutex_t lock;
+} batch_t;
+
+static void *batch_worker(void *arg)
+{
+    batch_t *b = (batch_t *)arg;
+    for (;;) {
+        pthread_mutex_lock(&b->lock);
+        int s = b->next++;
+        pthread_mutex_unlock(&b->lock);
+        if (s >= b->nsys) break;
+        int o = b->offsets[s], nat = b->offsets[s + 1] - o;
+        b->stat[s] = orc_gfn2_egrad(nat, b->num + o, b->xyz + 3 * o, b->charge[s], b->etemp[s],
+                                    b->opts, &b->energy[s], b->grad + 3 * o, b->qat + o, NULL,
+                                    NULL, NULL);
+    }
+    return NULL;
+}
+
+int orc_gfn2_egrad_batch(int nsys, const int32_t *offsets, const int32_t *num, const double *xyz,
+                         const int32_t *charge, const double *etemp, const orc_scc_opts *opts,
+                         double *energy, double *grad, double *qat, int32_t *stat, int nthreads)
+{
+    batch_t b = {nsys, offsets, num, charge, xyz, etemp, opts, energy, grad, qat, stat, 0,
+                 PTHREAD_MUTEX_INITIALIZER};
+    d4_setup();
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, batch_worker, &b);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    return 0;
+}

@@ -1,0 +1,124 @@
+/* CPU oracle for the QCxMS GFN2-xTB molecular-dynamics hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm, used by tests/ as the parity checker, by __graft_entry__.smoke()
+ * as the checker, and by bench.py's cpu_baseline leg as the CPU baseline.
+ * The product (qcxms_amd/) never includes, links or calls anything here.
+ *
+ * Units follow the reference: Bohr, Hartree, atomic time units, masses in
+ * electron masses (xtb_mctc_convert.f90:24-63, mass.f90:3-22).
+ */
+#ifndef QCXMS_ORACLE_H
+#define QCXMS_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants (xtb_mctc_constants.f90:35, xtb_mctc_convert.f90:24-63) ---- */
+#define ORC_KB      3.166808578545117e-06
+#define ORC_AUTOAA  0.52917726
+#define ORC_AUTOEV  27.21138505
+#define ORC_FSTOAU  41.3413733365614
+
+double orc_aatoau(void);
+double orc_amutoau(void);
+double orc_autoamu(void);
+double orc_rad(int z);            /* covalent_radii.f90:89-114 (Bohr) */
+double orc_covrad_d3(int z);      /* covalent_radii.f90:83-84 (Bohr) */
+double orc_atomic_mass(int z);    /* atomic_masses.f90:24-67 (m_e) */
+
+/* ---- MD bookkeeping (bit-exact restatements) ---- */
+void orc_leapfrog(int nat, const double *grad, const double *mass, double tstp,
+                  double *xyz, double *vel, double *ke);                 /* md.f90:749-773 */
+void orc_ekinet(int nat, const double *velo, const double *mass,
+                double *e, double *t);                                   /* mdinit.f90:57-76 */
+int  orc_impactscale(int nuc, double *velo, const double *mass, const double *velof,
+                     double eimp, double ff, double e0);                 /* impact.f90:12-55 */
+double orc_setetemp(int nfrag, double eimp, double ax, double ieetemp);  /* utility.f90:69-86 */
+int  orc_getspin(int nat, const int32_t *ic, int chrg);                  /* utility.f90:449-464 */
+double orc_gnorm(int nuc, const double *grad);                           /* iniqm.f90:718-732 */
+int  orc_checkqc(int nuc, double *e, const double *grad, const double *qat,
+                 int mchrg);                                             /* iniqm.f90:684-715 */
+void orc_fragment_structure(int nat, const int32_t *oz, const double *xyz, double rcut,
+                            int at1, int at2, int32_t *frag);            /* fragments.f90:93-182 */
+void orc_fragmass(int nat, const int32_t *iat, const int32_t *list, const double *mass,
+                  const int32_t *imass, int32_t *nfrag, double *fragm,
+                  int32_t *fragat);                                      /* fragments.f90:10-84 */
+void orc_intenergy(int nuc, const int32_t *list, const double *mass, const double *velo,
+                   int nfrag, double *t);                                /* md.f90:715-741 */
+
+/* ---- MD driver: the EI production loop of md.f90:285-682 (it>0, method<3) ---- */
+typedef int (*orc_egrad_fn)(void *ctx, int nat, const int32_t *iat, const double *xyz,
+                            int chrg, double etemp, double *e, double *grad,
+                            double *qat);
+
+typedef struct {
+    /* inputs */
+    int nat, chrg, nfragexit, max_steps;
+    double tstep, tadd, eimp, ax, ieetemp, etempin;
+    /* outputs (md.f90:684-700) */
+    int nstep, nfrag, fragstate, mdok, exit_reason;
+    double tav, epav, ekav, ttime, dtime, atlast, epot, ekin, eerror, etemp;
+    double fragm[10], fragT[10];
+} orc_md_state;
+
+/* exit_reason codes shared with the product's MD engine */
+enum { ORC_MD_RUNNING = 0, ORC_MD_ERR_SCF = 1, ORC_MD_ERR_DRIFT = 2, ORC_MD_NFRAG6 = 3,
+       ORC_MD_NFRAGEXIT = 4, ORC_MD_FCONST = 5, ORC_MD_MORESTEP = 6, ORC_MD_MAXSTEP = 7,
+       ORC_MD_HEAT_FAIL = 8 };
+
+int orc_md_run(orc_md_state *st, const int32_t *iat, const int32_t *imass,
+               const double *mass, const double *velof, double *xyz, double *velo,
+               double *grad, double *achrg, int32_t *list, orc_egrad_fn egrad, void *ctx,
+               int steps_limit);
+
+/* ---- GFN2-xTB single point with analytic gradient ---- */
+typedef struct {
+    double econv, pconv;     /* SCC convergence: |dE| and RMS charge/moment change */
+    int maxiter;
+    double broyden_damp;     /* damping of the modified Broyden mixer */
+    int cold_start;          /* 1: start from zero charges (reference behaviour) */
+} orc_scc_opts;
+
+typedef struct {
+    double energy, e_rep, e_disp, e_es2, e_es3, e_aes, e_axc, e_band, e_ts;
+    int iterations, converged, nao, nsh, nel_int;
+} orc_gfn2_info;
+
+void orc_scc_default_opts(orc_scc_opts *o);
+/* returns 0 on success, 5 unsupported element, -1 SCC not converged */
+int orc_gfn2_egrad(int nat, const int32_t *num, const double *xyz, int charge, double etemp,
+                   const orc_scc_opts *opts, double *energy, double *grad, double *qat,
+                   double *dipole_at, double *qpole_at, orc_gfn2_info *info);
+/* energy only (no gradient), same model */
+int orc_gfn2_energy(int nat, const int32_t *num, const double *xyz, int charge,
+                    double etemp, const orc_scc_opts *opts, double *energy, double *qat);
+
+/* building blocks exposed for unit pinning */
+int  orc_gfn2_supported(int z);
+void orc_gfn2_cn(int nat, const int32_t *num, const double *xyz, double *cn, double *dcn);
+void orc_d4_cn(int nat, const int32_t *num, const double *xyz, double *cn, double *dcn);
+int  orc_sto_ng(int ng, int n, int l, double zeta, double *alpha, double *coeff);
+void orc_overlap_shellpair(int la, int ga, const double *aa, const double *ca,
+                           int lb, int gb, const double *ab, const double *cb,
+                           const double *ra, const double *rb, double *s, double *dpb,
+                           double *qpb);
+int  orc_gfn2_matrices(int nat, const int32_t *num, const double *xyz, double *s,
+                       double *h0, double *dpint, double *qpint, int *nao_out);
+void orc_sygv(int n, double *f, double *s, double *eval, double *work);
+double orc_d4_twobody(int nat, const int32_t *num, const double *xyz, const double *q,
+                      double *grad, double *dedq);
+double orc_d4_atm(int nat, const int32_t *num, const double *xyz, double *grad);
+
+/* batched CPU baseline: nthreads workers, each owning whole systems */
+int orc_gfn2_egrad_batch(int nsys, const int32_t *offsets, const int32_t *num,
+                         const double *xyz, const int32_t *charge, const double *etemp,
+                         const orc_scc_opts *opts, double *energy, double *grad,
+                         double *qat, int32_t *stat, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,189 @@
+! C-callable wrappers around QCxMS's own Fortran routines, compiled from the
+! reference sources in place (see oracle/Makefile) into oracle/_ref/.
+! Test infrastructure only: the test suite uses these to pin the C oracle and
+! to write golden fixtures.  Nothing in the product links this.
+!
+! Wrapped routines (reference file:line):
+!   fragment_structure   src/fragments.f90:93-182
+!   fragmass             src/fragments.f90:10-84
+!   ekinet               src/mdinit.f90:57-76
+!   impactscale          src/impact.f90:12-55
+!   Rad                  src/covalent_radii.f90:89-114
+!   covalent_radius_d3   src/covalent_radii.f90:83-84
+!   atomic_mass          src/atomic_masses.f90:58-67
+!   zeta/dzeta/trapzd/cngw/d4dim/d4/covncoord/dispgrad
+!                        src/dftd4.f90:292-1337, 1463-1780
+module qcxms_ref_bind
+   use iso_c_binding
+   use xtb_mctc_accuracy, only: wp
+   use qcxms_fragments, only: fragment_structure, fragmass
+   use qcxms_mdinit, only: ekinet
+   use qcxms_impact, only: impactscale
+   use covalent_radii, only: Rad, covalent_radius_d3
+   use xtb_atomic_masses, only: atomic_mass
+   use dftd4
+   implicit none
+contains
+
+subroutine ref_fragment_structure(nat, oz, xyz, rcut, at1, at2, frag) bind(C)
+   integer(c_int), value :: nat, at1, at2
+   integer(c_int), intent(in) :: oz(nat)
+   real(c_double), intent(in) :: xyz(3, nat)
+   real(c_double), value :: rcut
+   integer(c_int), intent(out) :: frag(nat)
+   integer :: oz_(nat), frag_(nat)
+   real(wp) :: r
+   oz_ = oz
+   r = rcut
+   call fragment_structure(int(nat), oz_, xyz, r, int(at1), int(at2), frag_)
+   frag = frag_
+end subroutine
+
+subroutine ref_fragmass(nat, iat, list, mass, imass, nfrag, fragm, fragat) bind(C)
+   integer(c_int), value :: nat
+   integer(c_int), intent(in) :: iat(nat), list(nat), imass(nat)
+   real(c_double), intent(in) :: mass(nat)
+   integer(c_int), intent(out) :: nfrag
+   real(c_double), intent(out) :: fragm(10)
+   integer(c_int), intent(out) :: fragat(200, 10)
+   integer :: iat_(nat), list_(nat), imass_(nat), nfrag_, fragat_(200, 10)
+   real(wp) :: mass_(nat), fragm_(10)
+   character(len=80) :: fragf(max(nat, 10))
+   iat_ = iat; list_ = list; imass_ = imass; mass_ = mass
+   fragm_ = 0.0_wp
+   fragat_ = 0
+   call fragmass(int(nat), iat_, list_, mass_, imass_, nfrag_, fragm_, fragf, fragat_)
+   nfrag = nfrag_
+   fragm = fragm_
+   fragat = fragat_
+end subroutine
+
+subroutine ref_ekinet(nat, velo, mass, e, t) bind(C)
+   integer(c_int), value :: nat
+   real(c_double), intent(in) :: velo(3, nat), mass(nat)
+   real(c_double), intent(out) :: e, t
+   real(wp) :: e_, t_
+   call ekinet(int(nat), velo, mass, e_, t_)
+   e = e_
+   t = t_
+end subroutine
+
+subroutine ref_impactscale(nuc, velo, mass, velof, eimp, ff, e0) bind(C)
+   integer(c_int), value :: nuc
+   real(c_double), intent(inout) :: velo(3, nuc)
+   real(c_double), intent(in) :: mass(nuc), velof(nuc)
+   real(c_double), value :: eimp, ff, e0
+   real(wp) :: v_(3, nuc), m_(nuc), vf_(nuc), eimp_, ff_, e0_
+   v_ = velo; m_ = mass; vf_ = velof; eimp_ = eimp; ff_ = ff; e0_ = e0
+   call impactscale(int(nuc), v_, m_, vf_, eimp_, ff_, e0_)
+   velo = v_
+end subroutine
+
+function ref_rad(z) result(r) bind(C)
+   integer(c_int), value :: z
+   real(c_double) :: r
+   r = Rad(z)
+end function
+
+function ref_covrad_d3(z) result(r) bind(C)
+   integer(c_int), value :: z
+   real(c_double) :: r
+   r = covalent_radius_d3(z)
+end function
+
+function ref_atomic_mass(z) result(m) bind(C)
+   integer(c_int), value :: z
+   real(c_double) :: m
+   m = atomic_mass(int(z))
+end function
+
+function ref_zeta(a, c, qref, qmod) result(z) bind(C)
+   real(c_double), value :: a, c, qref, qmod
+   real(c_double) :: z
+   z = zeta(a, c, qref, qmod)
+end function
+
+function ref_dzeta(a, c, qref, qmod) result(z) bind(C)
+   real(c_double), value :: a, c, qref, qmod
+   real(c_double) :: z
+   z = dzeta(a, c, qref, qmod)
+end function
+
+function ref_trapzd(pol) result(t) bind(C)
+   real(c_double), intent(in) :: pol(23)
+   real(c_double) :: t
+   t = trapzd(pol)
+end function
+
+function ref_cngw(wf, cn, cnref) result(w) bind(C)
+   real(c_double), value :: wf, cn, cnref
+   real(c_double) :: w
+   w = cngw(wf, cn, cnref)
+end function
+
+!> Reference-system table as d4dim leaves it (mode selects the charge set).
+subroutine ref_d4dim(nat, at, g_a, g_c, mode, ndim) bind(C)
+   integer(c_int), value :: nat, mode
+   integer(c_int), intent(in) :: at(nat)
+   real(c_double), value :: g_a, g_c
+   integer(c_int), intent(out) :: ndim
+   integer :: at_(nat), nd
+   at_ = at
+   call d4dim(int(nat), at_, g_a, g_c, int(mode), nd)
+   ndim = nd
+end subroutine
+
+subroutine ref_d4_tables(z, nref, refq_out, refcovcn_out, refc_out, refal_out) bind(C)
+   integer(c_int), value :: z
+   integer(c_int), intent(out) :: nref
+   real(c_double), intent(out) :: refq_out(7), refcovcn_out(7), refal_out(23, 7)
+   integer(c_int), intent(out) :: refc_out(7)
+   nref = refn(z)
+   refq_out = refq(:, z)
+   refcovcn_out = refcovcn(:, z)
+   refc_out = refc(:, z)
+   refal_out = refal(:, :, z)
+end subroutine
+
+subroutine ref_d4(nat, ndim, at, wf, g_a, g_c, covcn, gw, c6abns) bind(C)
+   integer(c_int), value :: nat, ndim
+   integer(c_int), intent(in) :: at(nat)
+   real(c_double), value :: wf, g_a, g_c
+   real(c_double), intent(in) :: covcn(nat)
+   real(c_double), intent(out) :: gw(ndim), c6abns(ndim, ndim)
+   integer :: at_(nat)
+   at_ = at
+   call d4(int(nat), int(ndim), at_, wf, g_a, g_c, covcn, gw, c6abns)
+end subroutine
+
+subroutine ref_covncoord(nat, at, xyz, cn, thr) bind(C)
+   integer(c_int), value :: nat
+   integer(c_int), intent(in) :: at(nat)
+   real(c_double), intent(in) :: xyz(3, nat)
+   real(c_double), value :: thr
+   real(c_double), intent(out) :: cn(nat)
+   integer :: at_(nat)
+   at_ = at
+   call covncoord(int(nat), at_, xyz, cn, thr)
+end subroutine
+
+!> Full D4 energy (two-body at charges q + approximate ATM at q=0) and gradient.
+subroutine ref_dispgrad(nat, ndim, at, q, xyz, s6, s8, s9, a1, a2, alp, wf, g_a, g_c, &
+      & g, eout) bind(C)
+   integer(c_int), value :: nat, ndim, alp
+   integer(c_int), intent(in) :: at(nat)
+   real(c_double), intent(in) :: q(nat), xyz(3, nat)
+   real(c_double), value :: s6, s8, s9, a1, a2, wf, g_a, g_c
+   real(c_double), intent(inout) :: g(3, nat)
+   real(c_double), intent(out) :: eout
+   integer :: at_(nat)
+   real(wp), allocatable :: gw(:), c6abns(:, :), cn(:)
+   at_ = at
+   allocate(gw(ndim), c6abns(ndim, ndim), cn(nat))
+   call covncoord(int(nat), at_, xyz, cn, 1600.0_wp)
+   call d4(int(nat), int(ndim), at_, wf, g_a, g_c, cn, gw, c6abns)
+   call dispgrad(int(nat), int(ndim), at_, q, xyz, s6, s8, s9, a1, a2, int(alp), wf, &
+      & g_a, g_c, c6abns, 3, g, eout)
+end subroutine
+
+end module qcxms_ref_bind
