@@ -1,0 +1,172 @@
+"""ctypes bindings to the CPU oracle (oracle/_build/liboracle.so) and to the
+reference's own Fortran routines compiled in place (oracle/_ref/libqcxms_ref.so).
+Test infrastructure: used by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg only."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libqcxms_ref.so")
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int32)
+
+
+def ptr(a, t=dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class SccOpts(C.Structure):
+    _fields_ = [("econv", C.c_double), ("pconv", C.c_double), ("maxiter", C.c_int),
+                ("broyden_damp", C.c_double), ("cold_start", C.c_int)]
+
+
+class Gfn2Info(C.Structure):
+    _fields_ = ([(n, C.c_double) for n in ("energy", "e_rep", "e_disp", "e_es2", "e_es3",
+                                           "e_aes", "e_axc", "e_band", "e_ts")]
+                + [(n, C.c_int) for n in ("iterations", "converged", "nao", "nsh", "nel_int")])
+
+
+class MdState(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("nat", "chrg", "nfragexit", "max_steps")]
+                + [(n, C.c_double) for n in ("tstep", "tadd", "eimp", "ax", "ieetemp", "etempin")]
+                + [(n, C.c_int) for n in ("nstep", "nfrag", "fragstate", "mdok", "exit_reason")]
+                + [(n, C.c_double) for n in ("tav", "epav", "ekav", "ttime", "dtime", "atlast",
+                                             "epot", "ekin", "eerror", "etemp")]
+                + [("fragm", C.c_double * 10), ("fragT", C.c_double * 10)])
+
+
+EGRAD_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, ip, dp, C.c_int, C.c_double, dp, dp, dp)
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        L = C.CDLL(ORACLE_SO)
+        for name in ("orc_aatoau", "orc_amutoau", "orc_autoamu", "orc_gnorm", "orc_setetemp",
+                     "orc_d4_trapzd", "orc_d4_zeta_fn", "orc_d4_dzeta_fn", "orc_d4_twobody",
+                     "orc_d4_atm"):
+            getattr(L, name).restype = C.c_double
+        for name in ("orc_rad", "orc_covrad_d3", "orc_atomic_mass"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_int]
+        L.orc_setetemp.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        L.orc_d4_zeta_fn.argtypes = [C.c_double] * 4
+        L.orc_d4_dzeta_fn.argtypes = [C.c_double] * 4
+        L.orc_gfn2_egrad.argtypes = [C.c_int, ip, dp, C.c_int, C.c_double, C.c_void_p, dp, dp,
+                                     dp, dp, dp, C.c_void_p]
+        L.orc_gfn2_egrad_batch.argtypes = [C.c_int, ip, ip, dp, ip, dp, C.c_void_p, dp, dp, dp,
+                                           ip, C.c_int]
+        L.orc_md_run.argtypes = [C.c_void_p, ip, ip, dp, dp, dp, dp, dp, dp, ip, EGRAD_FN,
+                                 C.c_void_p, C.c_int]
+        _oracle = L
+    return _oracle
+
+
+def ref():
+    """Reference Fortran library, or None when it was not built."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_SO):
+            return None
+        L = C.CDLL(REF_SO, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        for name in ("ref_rad", "ref_covrad_d3", "ref_atomic_mass"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_int]
+        for name in ("ref_zeta", "ref_dzeta", "ref_cngw"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_double] * 4 if name != "ref_cngw" else [C.c_double] * 3
+        L.ref_trapzd.restype = C.c_double
+        _ref = L
+    return _ref
+
+
+def default_opts(**kw):
+    o = SccOpts()
+    oracle().orc_scc_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def gfn2_egrad(num, xyz, charge=0, etemp=300.0, opts=None, grad=True, moments=False):
+    """Oracle GFN2 single point.  Returns dict(stat, energy, grad, qat, info[, dp, qp])."""
+    L = oracle()
+    num = np.ascontiguousarray(num, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    nat = len(num)
+    e = C.c_double()
+    g = np.zeros((nat, 3)) if grad else None
+    q = np.zeros(nat)
+    dpm = np.zeros((nat, 3)) if moments else None
+    qpm = np.zeros((nat, 6)) if moments else None
+    info = Gfn2Info()
+    o = opts if opts is not None else default_opts()
+    st = L.orc_gfn2_egrad(nat, ptr(num, ip), ptr(xyz), int(charge), float(etemp), C.byref(o),
+                          C.byref(e), ptr(g), ptr(q), ptr(dpm), ptr(qpm), C.byref(info))
+    out = dict(stat=st, energy=e.value, grad=g, qat=q, info=info)
+    if moments:
+        out["dp"] = dpm
+        out["qp"] = qpm
+    return out
+
+
+def gfn2_matrices(num, xyz):
+    L = oracle()
+    num = np.ascontiguousarray(num, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    nao = C.c_int()
+    L.orc_gfn2_matrices(len(num), ptr(num, ip), ptr(xyz), None, None, None, None, C.byref(nao))
+    n = nao.value
+    s = np.zeros((n, n)); h0 = np.zeros((n, n)); d = np.zeros((3, n, n)); qd = np.zeros((6, n, n))
+    L.orc_gfn2_matrices(len(num), ptr(num, ip), ptr(xyz), ptr(s), ptr(h0), ptr(d), ptr(qd),
+                        C.byref(nao))
+    return s, h0, d, qd
+
+
+def fragment_structure(lib, prefix, oz, xyz, rcut=3.0, at1=1, at2=0):
+    oz = np.ascontiguousarray(oz, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    frag = np.zeros(len(oz), dtype=np.int32)
+    getattr(lib, prefix + "fragment_structure")(len(oz), ptr(oz, ip), ptr(xyz), C.c_double(rcut),
+                                                at1, at2, ptr(frag, ip))
+    return frag
+
+
+def fragmass(lib, prefix, iat, lst, mass, imass=None):
+    iat = np.ascontiguousarray(iat, dtype=np.int32)
+    lst = np.ascontiguousarray(lst, dtype=np.int32)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    imass = np.full(len(iat), -1, dtype=np.int32) if imass is None else \
+        np.ascontiguousarray(imass, dtype=np.int32)
+    nfrag = C.c_int32()
+    fragm = np.zeros(10)
+    fragat = np.zeros((10, 200), dtype=np.int32)
+    getattr(lib, prefix + "fragmass")(len(iat), ptr(iat, ip), ptr(lst, ip), ptr(mass),
+                                      ptr(imass, ip), C.byref(nfrag), ptr(fragm), ptr(fragat, ip))
+    return nfrag.value, fragm, fragat
+
+
+def ekinet(lib, prefix, velo, mass):
+    velo = np.ascontiguousarray(velo, dtype=np.float64)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    e = C.c_double(); t = C.c_double()
+    getattr(lib, prefix + "ekinet")(len(mass), ptr(velo), ptr(mass), C.byref(e), C.byref(t))
+    return e.value, t.value
+
+
+def impactscale(lib, prefix, velo, mass, velof, eimp, ff, e0):
+    v = np.array(velo, dtype=np.float64, order="C")
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    velof = np.ascontiguousarray(velof, dtype=np.float64)
+    fn = getattr(lib, prefix + "impactscale")
+    fn.restype = C.c_int
+    k = fn(len(mass), ptr(v), ptr(mass), ptr(velof), C.c_double(eimp), C.c_double(ff),
+           C.c_double(e0))
+    return v, k

@@ -1,0 +1,149 @@
+"""CPU oracle MD bookkeeping pinned bit-for-bit to the reference's Fortran
+(golden vectors from oracle/_ref; live cross-check when it is built)."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+
+def split(a, off, k):
+    return a[off[k]:off[k + 1]]
+
+
+def test_tables_match_reference(golden):
+    L = ob.oracle()
+    for z in range(1, 119):
+        assert L.orc_rad(z) == golden["rad"][z - 1]
+        assert L.orc_atomic_mass(z) == golden["mass"][z - 1]
+        assert L.orc_covrad_d3(z) == golden["covrad_d3"][z - 1]
+
+
+def test_fragment_structure_golden(golden):
+    L = ob.oracle()
+    off = golden["frag_off"]
+    for k in range(len(off) - 1):
+        oz = split(golden["frag_oz"], off, k).astype(np.int32)
+        xyz = golden["frag_xyz"][off[k]:off[k + 1]]
+        want = split(golden["frag_out"], off, k)
+        got = ob.fragment_structure(L, "orc_", oz, xyz)
+        assert np.array_equal(got, want), k
+
+
+def test_fragment_labels_are_canonical_components(golden):
+    """Labels equal connected components numbered by lowest atom index."""
+    L = ob.oracle()
+    rng = np.random.default_rng(7)
+    rad = golden["rad"]
+    for _ in range(50):
+        nat = int(rng.integers(1, 40))
+        oz = rng.choice([1, 6, 7, 8], size=nat).astype(np.int32)
+        xyz = rng.normal(0, 5.0, size=(nat, 3))
+        got = ob.fragment_structure(L, "orc_", oz, xyz)
+        d = np.sqrt(((xyz[:, None] - xyz[None]) ** 2).sum(-1))
+        adj = d < 3.0 * 0.5 * (rad[oz - 1][:, None] + rad[oz - 1][None])
+        comp = -np.ones(nat, int)
+        lab = 0
+        for i in range(nat):
+            if comp[i] >= 0:
+                continue
+            lab += 1
+            stack = [i]
+            comp[i] = lab
+            while stack:
+                a = stack.pop()
+                for b in np.nonzero(adj[a])[0]:
+                    if comp[b] < 0:
+                        comp[b] = lab
+                        stack.append(b)
+        assert np.array_equal(got, comp)
+
+
+def test_fragmass_golden(golden):
+    L = ob.oracle()
+    off = golden["frag_off"]
+    for k in range(len(off) - 1):
+        if golden["fm_nfrag"][k] < 0:
+            continue
+        oz = split(golden["frag_oz"], off, k).astype(np.int32)
+        frag = split(golden["frag_out"], off, k).astype(np.int32)
+        mass = golden["mass"][oz - 1]
+        nf, fm, fa = ob.fragmass(L, "orc_", oz, frag, mass)
+        assert nf == golden["fm_nfrag"][k]
+        assert np.array_equal(fm, golden["fm_fragm"][k])
+        assert np.array_equal(fa, golden["fm_fragat"][k])
+
+
+def test_ekinet_impactscale_golden(golden):
+    L = ob.oracle()
+    off = golden["ek_off"]
+    for k in range(len(off) - 1):
+        v = golden["ek_velo"][off[k]:off[k + 1]]
+        m = golden["ek_mass"][off[k]:off[k + 1]]
+        e, t = ob.ekinet(L, "orc_", v, m)
+        assert e == golden["ek_out"][k][0] and t == golden["ek_out"][k][1]
+    off = golden["is_off"]
+    for k in range(len(off) - 1):
+        v = golden["is_velo"][off[k]:off[k + 1]]
+        m = golden["is_mass"][off[k]:off[k + 1]]
+        f = golden["is_velof"][off[k]:off[k + 1]]
+        eimp, ff, e0 = golden["is_par"][k]
+        got, kk = ob.impactscale(L, "orc_", v, m, f, eimp, ff, e0)
+        assert kk < 20000
+        assert np.array_equal(got, golden["is_out"][off[k]:off[k + 1]])
+
+
+def test_leapfrog_closed_forms():
+    """Zero force: linear motion; constant force: exact quadratic positions."""
+    import ctypes as C
+    L = ob.oracle()
+    rng = np.random.default_rng(3)
+    nat = 7
+    mass = rng.uniform(1800, 30000, nat)
+    x0 = rng.normal(size=(nat, 3))
+    v0 = rng.normal(scale=1e-3, size=(nat, 3))
+    g = rng.normal(scale=1e-2, size=(nat, 3))
+    dt = 0.5 * 41.3413733365614
+    x, v, ke = x0.copy(), v0.copy(), C.c_double()
+    nsteps = 50
+    for _ in range(nsteps):
+        L.orc_leapfrog(nat, ob.ptr(g), ob.ptr(mass), C.c_double(dt), ob.ptr(x), ob.ptr(v), C.byref(ke))
+    a = -g / mass[:, None]
+    # v_n = v0 + n dt a ; x_n = x0 + dt sum_{k=1..n} v_k = x0 + n dt v0 + dt^2 a n(n+1)/2
+    np.testing.assert_allclose(v, v0 + nsteps * dt * a, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(x, x0 + nsteps * dt * v0 + dt * dt * a * nsteps * (nsteps + 1) / 2,
+                               rtol=1e-11, atol=1e-12)
+    vm = v - 0.5 * dt * a
+    assert abs(ke.value - 0.5 * np.sum(mass[:, None] * vm * vm)) < 1e-12 * abs(ke.value)
+
+
+def test_checkqc_and_getspin():
+    import ctypes as C
+    L = ob.oracle()
+    g = np.zeros((3, 3)); g[0, 1] = 1.0; g[1, 2] = 30.0  # z never enters gnorm
+    L.orc_gnorm.argtypes = [C.c_int, ob.dp]
+    assert L.orc_gnorm(3, ob.ptr(g)) == np.sqrt(2.0)
+    num = np.array([6, 1, 1, 1, 1], np.int32)
+    assert L.orc_getspin(5, ob.ptr(num, ob.ip), 0) == 1
+    assert L.orc_getspin(5, ob.ptr(num, ob.ip), 1) == 2
+    h = np.array([1], np.int32)
+    assert L.orc_getspin(1, ob.ptr(h, ob.ip), 1) == -1
+    e = C.c_double(-5.0)
+    q = np.array([0.5, 0.5, 0.0])
+    g2 = np.ones((3, 3)) * 0.01
+    assert L.orc_checkqc(3, C.byref(e), ob.ptr(g2), ob.ptr(q), 1) == 1
+    g3 = np.ones((3, 3)) * 20.0
+    assert L.orc_checkqc(3, C.byref(e), ob.ptr(g3), ob.ptr(q), 1) == 0 and e.value == 0.0
+    assert L.orc_setetemp(1, 0.3, 0.0, 0.0) == 5000.0
+
+
+@pytest.mark.skipif(ob.ref() is None, reason="reference Fortran not built")
+def test_live_against_reference_fortran():
+    L, R = ob.oracle(), ob.ref()
+    rng = np.random.default_rng(11)
+    rad = np.array([R.ref_rad(z) for z in range(1, 119)])
+    for _ in range(100):
+        nat = int(rng.integers(1, 70))
+        oz = rng.integers(1, 87, size=nat).astype(np.int32)
+        xyz = rng.normal(0, float(rng.choice([2.0, 5.0, 10.0])), size=(nat, 3))
+        assert np.array_equal(ob.fragment_structure(L, "orc_", oz, xyz),
+                              ob.fragment_structure(R, "ref_", oz, xyz))
