@@ -1,0 +1,90 @@
+/* C ABI of the MI355X GFN2-xTB molecular-dynamics engine (libqcxms_gpu.so).
+ *
+ * Plain pointers and sizes only.  Units follow the reference: coordinates in
+ * Bohr, energies in Hartree, gradients in Hartree/Bohr, temperatures in K,
+ * masses in electron masses, time in atomic units.  Arrays are row-major
+ * [atom][xyz], which is the memory layout of the Fortran xyz(3,nat).
+ * Status codes follow src/tblite.f90:55-58: 0 success, 5 unknown or
+ * unsupported method/element, -1 calculation failed (SCC not converged).
+ * Every function returns an int status; -2 reports an invalid argument or a
+ * HIP runtime error, with the message available from qcxms_gpu_last_error().
+ */
+#ifndef QCXMS_GPU_H
+#define QCXMS_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QCXMS_OK 0
+#define QCXMS_UNKNOWN_METHOD 5
+#define QCXMS_FATAL (-1)
+#define QCXMS_INVALID (-2)
+
+/* method selectors, src/tblite.f90:34-40 */
+#define QCXMS_GFN1_XTB 1
+#define QCXMS_GFN2_XTB 2
+#define QCXMS_IPEA1_XTB 11
+
+/* Select the GPU and build the parameter tables.  Called implicitly by the
+ * other entry points with device 0; no counterpart in the reference. */
+int qcxms_gpu_init(int device);
+/* Message of the last failure on this thread ("" if none). */
+const char *qcxms_gpu_last_error(void);
+
+/* Drop-in for get_xtb_egrad (src/tblite.f90:65-93): one structure, fresh
+ * wavefunction per call.  multiplicity enters as uhf = min(mult-1, 0) = 0
+ * (src/tblite.f90:111), i.e. a spin-restricted Fermi-smeared solution.
+ * qat[nat], gradient[3*nat] are written; *stat receives the reference's
+ * status code (0, 5 or -1) and is also the return value. */
+int qcxms_get_xtb_egrad(int32_t nat, const int32_t *num, const double *xyz, int32_t charge,
+                        int32_t multiplicity, int32_t method, double etemp, double *qat,
+                        double *energy, double *gradient, int32_t *stat);
+
+/* Batched single points over ragged systems (the per-trajectory egrad calls
+ * of iniqm.f90:641-655, many at once).  System s owns atoms
+ * offsets[s] .. offsets[s+1]-1 of num/xyz/qat/gradient.  stat[nsys]. */
+int qcxms_egrad_batch(int32_t nsys, const int32_t *offsets, const int32_t *num,
+                      const double *xyz, const int32_t *charge, const int32_t *multiplicity,
+                      int32_t method, const double *etemp, double *qat, double *energy,
+                      double *gradient, int32_t *stat);
+
+/* ---- persistent batches: device-resident state for repeated evaluations --- */
+typedef struct qcxms_batch qcxms_batch;
+
+/* Create a batch of nsys systems (layout as qcxms_egrad_batch).  Returns NULL
+ * on failure (see qcxms_gpu_last_error). */
+qcxms_batch *qcxms_batch_create(int32_t nsys, const int32_t *offsets, const int32_t *num,
+                                const int32_t *charge, int32_t method);
+void qcxms_batch_destroy(qcxms_batch *b);
+/* SCC controls: energy and RMS moment-change thresholds, iteration cap,
+ * Broyden damping, Jacobi off-diagonal tolerance.  Defaults: 1e-9, 1e-8, 100,
+ * 0.4, 1e-11. */
+int qcxms_batch_set_scc(qcxms_batch *b, double econv, double pconv, int32_t maxiter,
+                        double damping, double jacobi_tol);
+/* warm != 0: start each SCC from the previous call's converged charges,
+ * moments and orbitals (same fixed point, fewer iterations).  Default 0
+ * (fresh wavefunction per call, as src/tblite.f90:133). */
+int qcxms_batch_set_warm_start(qcxms_batch *b, int32_t warm);
+/* Energy, gradient and Mulliken charges for all systems at xyz (host
+ * arrays).  etemp[nsys] in K.  Outputs may be NULL when not needed. */
+int qcxms_batch_egrad(qcxms_batch *b, const double *xyz, const double *etemp, double *energy,
+                      double *gradient, double *qat, int32_t *stat);
+/* Per-system SCC iteration counts and energy parts
+ * (band, es2, es3, aes, axc, disp2, atm, rep, -TS, electronic) [nsys*10]. */
+int qcxms_batch_info(qcxms_batch *b, int32_t *iterations, double *eparts);
+/* Atomic dipoles [3*natoms] and traceless quadrupoles [6*natoms, xx xy yy xz yz zz]. */
+int qcxms_batch_multipoles(qcxms_batch *b, double *dipoles, double *quadrupoles);
+
+/* ---- timing hooks (HIP events on the library's stream) ---- */
+/* enable != 0 records events around every kernel launch of this batch. */
+int qcxms_batch_set_profiling(qcxms_batch *b, int32_t enable);
+/* Accumulated kernel time (ms) and launch count for kernel names
+ * "setup", "scc", "grad", "md_pre", "md_post"; counters reset on read. */
+int qcxms_batch_kernel_time(qcxms_batch *b, const char *kernel, double *ms, int32_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,11 @@
+"""MI355X-native GFN2-xTB molecular-dynamics hot path of QCxMS.
+
+The compute runs in libqcxms_gpu.so (HIP, gfx950) through the C ABI declared
+in include/qcxms_gpu.h; this package is its Python host layer, mirroring the
+reference's Fortran interface (src/tblite.f90 get_xtb_egrad, src/md.f90).
+"""
+from .gpu import (GFN1_XTB, GFN2_XTB, IPEA1_XTB, Batch, LibraryMissing, egrad_batch,
+                  get_xtb_egrad, library, library_path)
+
+__all__ = ["GFN1_XTB", "GFN2_XTB", "IPEA1_XTB", "Batch", "LibraryMissing", "egrad_batch",
+           "get_xtb_egrad", "library", "library_path"]

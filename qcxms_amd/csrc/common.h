@@ -1,0 +1,156 @@
+// Shared device-side definitions of the MI355X GFN2-xTB engine.
+//
+// One workgroup owns one system (trajectory).  Per-system data lives in flat
+// device arrays addressed through SysInfo offsets; per-system n x n matrices
+// are stored row-major with a leading dimension ld = round_up(nao, 16) and
+// zero padding, so that 16 x 16 FP64 MFMA tiles cover them exactly.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gfn2_params.h"
+
+#define QX_BLOCK 256
+#define QX_MAX_NAT 64
+#define QX_MAX_NSH 128
+#define QX_MAX_NAO 256
+#define QX_MAX_REF 7
+#define QX_LDS_MAXM 64  // Jacobi matrices up to this order live in LDS
+
+#define QX_DEV __device__ __forceinline__
+
+struct ElemParams {
+    int z, nsh, nprim[2], pqn[2];
+    double alpha[2][6], coeff[2][6];  // scaled exponents, normalised contraction coefficients
+    double slater[2], hself0[2], kcn[2], shpoly[2], refocc[2];  // self-energies/kcn in Eh
+    double hard[2], third[2];                                     // shell hardness, Gamma_l
+    double rep_alpha, rep_zeff, en, rad, dkernel, qkernel, mprad, mpvcn, nval, covrad;
+    // D4
+    int nref, refc[QX_MAX_REF];
+    double refq[QX_MAX_REF], refcovcn[QX_MAX_REF], zeff, gam, r4r2, rcov_d4, en_d4;
+};
+
+struct DevTables {
+    ElemParams e[gfn2::kElem];
+    double c6ref[gfn2::kElem][QX_MAX_REF][gfn2::kElem][QX_MAX_REF];
+};
+
+extern __constant__ DevTables g_tab;
+
+struct SysInfo {
+    int nat, nsh, nao, ld, ndim, nx;
+    int atom_off, shell_off, ref_off, x_off;
+    long long mat_off;   // one ld*ld slot
+    long long gam_off;   // nsh*nsh
+    long long d4_off;    // ndim*ndim
+    long long hist_off;  // Broyden history: 2*mem*nx + mem*mem
+    long long pair_off;  // nat*nat*8 gradient scratch
+    int charge, pad0;
+    double nel, kt;
+};
+
+struct SccOpts {
+    double econv, pconv, damp, jacobi_tol;
+    int maxiter, mem;
+};
+
+// Device-resident batch state.  Every pointer is a flat array over all systems.
+struct BatchPtrs {
+    int nsys;
+    const SysInfo* info;
+    const int* elem;  // per atom: element slot
+    double* xyz;      // 3 per atom
+    double* grad;     // 3 per atom
+    double* qat;      // per atom
+    double* cn;       // per atom (GFN2 CN)
+    double* cnd4;     // per atom (D4 covalent CN)
+    double* mrad;     // per atom
+    double* dmrad;    // per atom
+    double* hself;    // per shell
+    double* gw;       // per D4 reference
+    double* dgw;      // per D4 reference
+    double* S;        // ld^2 per system
+    double* H0;
+    double* D;        // 3 ld^2 per system (dipole integrals, origin at row atom)
+    double* Q;        // 6 ld^2 per system (traceless quadrupole, origin at row atom)
+    double* L;        // Cholesky factor of S (lower)
+    double* X;        // L^-T (upper): X^T S X = 1
+    double* C;        // eigenvectors (AO basis)
+    double* T;        // scratch
+    double* P;        // density
+    double* W;        // energy-weighted density
+    double* U;        // eigenvectors in the orthogonalised basis (warm start)
+    double* gam;      // nsh^2 per system
+    double* wd4;      // ndim^2 per system
+    double* x;        // mixing vector [qsh, dp, qp] (converged, warm start)
+    double* xo;       // output moments of the last iteration
+    double* vs;       // per shell potential at the output moments
+    double* vd;       // 3 per atom
+    double* vq;       // 6 per atom
+    double* eps;      // orbital energies, ld per system
+    double* focc;     // occupations, ld per system
+    double* hist;     // Broyden history
+    double* pairbuf;  // gradient scratch
+    double* energy;   // per system: total
+    double* eparts;   // per system: 10 parts (band, es2, es3, aes, axc, d4, atm, rep, ts, scc)
+    int* iters;       // per system
+    int* status;      // per system: 0 ok, -1 not converged
+    int* warm;        // per system: 1 if U/x hold a previous solution
+    const int* active;  // per system: 0 skips the system (finished trajectories), may be null
+};
+
+QX_DEV int pad16(int n) { return (n + 15) & ~15; }
+
+// block-wide sum (all threads must call); smem needs QX_BLOCK/64 doubles
+QX_DEV double block_sum(double v, double* smem)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) smem[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < QX_BLOCK / 64; ++i) s += smem[i];
+    __syncthreads();
+    return s;
+}
+
+QX_DEV double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Shell descriptor built per system in LDS
+struct ShellDesc {
+    int at, l, el, k, ao0, nao;  // k: shell index within the element
+};
+
+// Fill shell and AO tables of one system (called by all threads).
+QX_DEV void build_shell_tables(const SysInfo& si, const int* elem, ShellDesc* sh, int* ao_at,
+                               int* ao_sh)
+{
+    if (threadIdx.x == 0) {
+        int is = 0, ia = 0;
+        for (int a = 0; a < si.nat; ++a) {
+            int el = elem[si.atom_off + a];
+            for (int k = 0; k < g_tab.e[el].nsh; ++k) {
+                ShellDesc d;
+                d.at = a;
+                d.l = k;
+                d.el = el;
+                d.k = k;
+                d.ao0 = ia;
+                d.nao = k ? 3 : 1;
+                sh[is] = d;
+                for (int c = 0; c < d.nao; ++c) {
+                    ao_at[ia] = a;
+                    ao_sh[ia] = is;
+                    ++ia;
+                }
+                ++is;
+            }
+        }
+    }
+    __syncthreads();
+}

@@ -1,0 +1,364 @@
+// Analytic GFN2-xTB gradient (the adjoint of the converged SCC energy), one
+// workgroup per system.
+//
+//   dE/dR = sum_{AB blocks} [2 P dH0 - 2 W dS - P (v_a + v_b) dS
+//                            - P (vd . dD + vq : dQ)]      (integral terms)
+//         + explicit pair terms (repulsion, ES2 gamma, AES kernels, D4)
+//         + CN chain rules (self-energies and multipole radii through the
+//           GFN2 CN, D4 weights through the covalent D4 CN).
+// Derivative integral blocks are recomputed on the fly (never stored).
+// All reductions are ordered: every partial result has one owner thread, so
+// the gradient is bit-reproducible from run to run.
+#pragma once
+#include "k_scc.h"
+
+namespace qx {
+
+constexpr int kPairStride = 16;
+
+__global__ __launch_bounds__(QX_BLOCK) void k_grad(BatchPtrs B)
+{
+    const int sys = blockIdx.x;
+    if (B.active && !B.active[sys]) return;
+    const SysInfo si = B.info[sys];
+    const int tid = threadIdx.x, nat = si.nat, nsh = si.nsh, ld = si.ld;
+    __shared__ SysShared ss;
+    __shared__ double dedcn[QX_MAX_NAT], dedcn4[QX_MAX_NAT], gacc[3 * QX_MAX_NAT];
+    load_system(si, B, ss);
+    const double* P = B.P + si.mat_off;
+    const double* W = B.W + si.mat_off;
+    const double* xo = B.xo + si.x_off;
+    const double* qsh = xo;
+    const double* dp = xo + nsh;
+    const double* qp = xo + nsh + 3 * nat;
+    const double* vs = B.vs + si.shell_off;
+    const double* vd = B.vd + 3 * (size_t)si.atom_off;
+    const double* vq = B.vq + 6 * (size_t)si.atom_off;
+    const double* hself = B.hself + si.shell_off;
+    const double* gw = B.gw + si.ref_off;
+    const double* dgw = B.dgw + si.ref_off;
+    const double* cn = B.cn + si.atom_off;
+    const double* cnd4 = B.cnd4 + si.atom_off;
+    const double* dmrad = B.dmrad + si.atom_off;
+    double* pb = B.pairbuf + si.pair_off;
+    for (int a = tid; a < nat; a += QX_BLOCK) ss.qat[a] = B.qat[si.atom_off + a];
+    __syncthreads();
+
+    // D4 charge scaling at the SCC charges (zr, dzr unused here) and at zero
+    for (int k = tid; k < si.ndim; k += QX_BLOCK) {
+        int a = 0;
+        while (ss.refoff[a + 1] <= k) ++a;
+        const ElemParams& e = g_tab.e[ss.sel[a]];
+        const int r = k - ss.refoff[a];
+        const double c = e.gam * gfn2::kD4Gc;
+        ss.zr[k] = d4_zeta(gfn2::kD4Ga, c, e.refq[r] + e.zeff, ss.qat[a] + e.zeff);
+        ss.yr[k] = d4_zeta(gfn2::kD4Ga, c, e.refq[r] + e.zeff, e.zeff);
+    }
+    __syncthreads();
+    // pre-phase: zero-charge pair C6 and its CN derivative for ATM
+    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
+        const int i = p / nat, j = p % nat;
+        double c = 0.0, dc = 0.0;
+        if (i != j) {
+            const ElemParams& ei = g_tab.e[ss.sel[i]];
+            const ElemParams& ej = g_tab.e[ss.sel[j]];
+            for (int ii = 0; ii < ei.nref; ++ii) {
+                const int k = ss.refoff[i] + ii;
+                for (int jj = 0; jj < ej.nref; ++jj) {
+                    const int l = ss.refoff[j] + jj;
+                    const double cr = g_tab.c6ref[ss.sel[i]][ii][ss.sel[j]][jj];
+                    c += ss.yr[k] * gw[k] * ss.yr[l] * gw[l] * cr;
+                    dc += ss.yr[k] * dgw[k] * ss.yr[l] * gw[l] * cr;
+                }
+            }
+        }
+        pb[(size_t)p * kPairStride + 10] = c;
+        pb[(size_t)p * kPairStride + 11] = dc;
+    }
+    __syncthreads();
+
+    // Phase A: integral derivative blocks for unordered atom pairs A > B
+    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
+        const int A = p / nat, Bt = p % nat;
+        if (Bt >= A) continue;
+        const double* ra = ss.sx + 3 * A;
+        const double* rb = ss.sx + 3 * Bt;
+        const double t[3] = {rb[0] - ra[0], rb[1] - ra[1], rb[2] - ra[2]};
+        const double r = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        double gA[3] = {0, 0, 0}, dhA = 0.0, dhB = 0.0;
+        for (int ia = ss.shoff[A]; ia < ss.shoff[A + 1]; ++ia) {
+            const ShellDesc sa = ss.sh[ia];
+            const ElemParams& ea = g_tab.e[sa.el];
+            for (int ib = ss.shoff[Bt]; ib < ss.shoff[Bt + 1]; ++ib) {
+                const ShellDesc sb = ss.sh[ib];
+                const ElemParams& eb = g_tab.e[sb.el];
+                double pir, dpir, kxy;
+                const double scal = h0_scale(ea, sa.k, eb, sb.k, r, pir, dpir, kxy);
+                const double hav = 0.5 * (hself[ia] + hself[ib]);
+                double psum = 0.0;
+                for (int a = 0; a < sa.nao; ++a)
+                    for (int b = 0; b < sb.nao; ++b) {
+                        int la[3], lb[3];
+                        ao_cart(sa.l, a, la);
+                        ao_cart(sb.l, b, lb);
+                        double s, dpb[3], qpb[6], ds[3], ddp[3][3], dqp[3][6];
+                        ao_pair_derivs(ea, sa.k, eb, sb.k, ra, rb, la, lb, s, dpb, qpb, ds, ddp, dqp);
+                        const int mu = sa.ao0 + a, nu = sb.ao0 + b;
+                        const double pmn = P[(size_t)mu * ld + nu], wmn = W[(size_t)mu * ld + nu];
+                        psum += pmn * s;
+                        const double wS = 2.0 * pmn * hav * scal - 2.0 * wmn - pmn * (vs[ia] + vs[ib]);
+                        double wDA[3], wDB[3], wQA[6], wQB[6], tA[6], tB[6];
+                        for (int c = 0; c < 3; ++c) {
+                            wDA[c] = -pmn * vd[3 * A + c];
+                            wDB[c] = -pmn * vd[3 * Bt + c];
+                        }
+                        for (int k = 0; k < 6; ++k) {
+                            wQA[k] = -pmn * vq[6 * A + k];
+                            wQB[k] = -pmn * vq[6 * Bt + k];
+                        }
+                        traceless6_t(wQA, tA);
+                        traceless6_t(wQB, tB);
+                        for (int d = 0; d < 3; ++d) {
+                            double g = wS * ds[d];
+                            for (int c = 0; c < 3; ++c) {
+                                g += wDB[c] * ddp[d][c];
+                                g += wDA[c] * (ddp[d][c] - (c == d ? s : 0.0) + t[c] * ds[d]);
+                            }
+                            for (int k = 0; k < 6; ++k) {
+                                const int u = qu(k), w = qw(k);
+                                g += tB[k] * dqp[d][k];
+                                double v = dqp[d][k];
+                                v += -(u == d ? dpb[w] : 0.0) - (w == d ? dpb[u] : 0.0);
+                                v += t[u] * ddp[d][w] + t[w] * ddp[d][u];
+                                v += -(u == d ? t[w] * s : 0.0) - (w == d ? t[u] * s : 0.0);
+                                v += t[u] * t[w] * ds[d];
+                                g += tA[k] * v;
+                            }
+                            gA[d] += g;
+                        }
+                    }
+                const double fpi = 2.0 * psum * hav * kxy * dpir;
+                for (int d = 0; d < 3; ++d) gA[d] += fpi * (-t[d] / r);
+                dhA += psum * scal * (-ea.kcn[sa.k]);
+                dhB += psum * scal * (-eb.kcn[sb.k]);
+            }
+        }
+        double* o = pb + (size_t)p * kPairStride;
+        o[0] = gA[0];
+        o[1] = gA[1];
+        o[2] = gA[2];
+        o[3] = dhA;
+        o[4] = dhB;
+    }
+
+    // Phase B: explicit pair terms, contribution to atom a from ordered pair (a, b)
+    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
+        const int a = p / nat, b = p % nat;
+        double* o = pb + (size_t)p * kPairStride;
+        if (a == b) {
+            o[5] = o[6] = o[7] = o[8] = o[9] = 0.0;
+            continue;
+        }
+        const ElemParams& ea = g_tab.e[ss.sel[a]];
+        const ElemParams& eb = g_tab.e[ss.sel[b]];
+        const double rv[3] = {ss.sx[3 * a] - ss.sx[3 * b], ss.sx[3 * a + 1] - ss.sx[3 * b + 1],
+                              ss.sx[3 * a + 2] - ss.sx[3 * b + 2]};
+        const double r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+        const double r = sqrt(r2);
+        double g[3] = {0, 0, 0};
+        double dcn_a = 0.0, dcn4_a = 0.0;
+        double sr = 0.0;  // radial factor times 1/r
+        // repulsion
+        if (r <= gfn2::kRepCutoff) {
+            const double kexp = (ea.z <= 2 && eb.z <= 2) ? gfn2::kRepKexpLight : gfn2::kRepKexp;
+            const double sa = sqrt(ea.rep_alpha * eb.rep_alpha);
+            const double e = ea.rep_zeff * eb.rep_zeff / r * exp(-sa * pow(r, kexp));
+            sr += e * (-1.0 / r - sa * kexp * pow(r, kexp - 1.0)) / r;
+        }
+        // ES2: -q_s q_t gamma^3 R
+        for (int s = ss.shoff[a]; s < ss.shoff[a + 1]; ++s) {
+            const double ha = g_tab.e[ss.sh[s].el].hard[ss.sh[s].k];
+            for (int tt = ss.shoff[b]; tt < ss.shoff[b + 1]; ++tt) {
+                const double eta = 0.5 * (ha + g_tab.e[ss.sh[tt].el].hard[ss.sh[tt].k]);
+                const double gm = 1.0 / sqrt(r2 + 1.0 / (eta * eta));
+                sr += -qsh[s] * qsh[tt] * gm * gm * gm;
+            }
+        }
+        // AES at fixed moments
+        {
+            const double r0 = 0.5 * (ss.mrad[a] + ss.mrad[b]);
+            const double r03 = r0 * r0 * r0, r04 = r03 * r0;
+            const double h3 = 1.0 / (r2 * r + 6.0 * r03);
+            const double h5 = 1.0 / (r * (r2 * r2 + 6.0 * r04));
+            const double dh3 = -3.0 * r2 * h3 * h3;
+            const double dh5 = -(5.0 * r2 * r2 + 6.0 * r04) * h5 * h5;
+            const double h5b = r2 * h5, dh5b = 2.0 * r * h5 + r2 * dh5;
+            const double dh3r0 = -18.0 * r0 * r0 * h3 * h3;
+            const double dh5r0 = -24.0 * r03 * r * h5 * h5;
+            const double dh5br0 = r2 * dh5r0;
+            const double* mi = dp + 3 * a;
+            const double* mj = dp + 3 * b;
+            const double* ti = qp + 6 * a;
+            const double* tj = qp + 6 * b;
+            const double mjr = mj[0] * rv[0] + mj[1] * rv[1] + mj[2] * rv[2];
+            const double mir = mi[0] * rv[0] + mi[1] * rv[1] + mi[2] * rv[2];
+            const double mimj = mi[0] * mj[0] + mi[1] * mj[1] + mi[2] * mj[2];
+            const double tjr[3] = {tj[0] * rv[0] + tj[1] * rv[1] + tj[3] * rv[2],
+                                   tj[1] * rv[0] + tj[2] * rv[1] + tj[4] * rv[2],
+                                   tj[3] * rv[0] + tj[4] * rv[1] + tj[5] * rv[2]};
+            const double tir[3] = {ti[0] * rv[0] + ti[1] * rv[1] + ti[3] * rv[2],
+                                   ti[1] * rv[0] + ti[2] * rv[1] + ti[4] * rv[2],
+                                   ti[3] * rv[0] + ti[4] * rv[1] + ti[5] * rv[2]};
+            const double tjrr = tjr[0] * rv[0] + tjr[1] * rv[1] + tjr[2] * rv[2];
+            const double tirr = tir[0] * rv[0] + tir[1] * rv[1] + tir[2] * rv[2];
+            const double qi = ss.qat[a], qj = ss.qat[b];
+            const double srad = (qi * mjr - qj * mir) * dh3 + mimj * dh5b - 3.0 * mir * mjr * dh5 +
+                                (qi * tjrr + qj * tirr) * dh5;
+            sr += srad / r;
+            for (int c = 0; c < 3; ++c)
+                g[c] += (qi * mj[c] - qj * mi[c]) * h3 - 3.0 * (mi[c] * mjr + mj[c] * mir) * h5 +
+                        2.0 * (qi * tjr[c] + qj * tir[c]) * h5;
+            const double der0 = (qi * mjr - qj * mir) * dh3r0 + mimj * dh5br0 -
+                                3.0 * mir * mjr * dh5r0 + (qi * tjrr + qj * tirr) * dh5r0;
+            dcn_a += 0.5 * der0 * dmrad[a];
+        }
+        // D4 two-body at the SCC charges
+        if (r2 <= gfn2::kD4RThr) {
+            double c6 = 0.0, dc = 0.0;
+            for (int ii = 0; ii < ea.nref; ++ii) {
+                const int k = ss.refoff[a] + ii;
+                for (int jj = 0; jj < eb.nref; ++jj) {
+                    const int l = ss.refoff[b] + jj;
+                    const double cr = g_tab.c6ref[ss.sel[a]][ii][ss.sel[b]][jj];
+                    c6 += ss.zr[k] * gw[k] * ss.zr[l] * gw[l] * cr;
+                    dc += ss.zr[k] * dgw[k] * ss.zr[l] * gw[l] * cr;
+                }
+            }
+            double dd;
+            const double disp = d4_disp(ea, eb, r2, &dd);
+            sr += -c6 * dd / r;
+            dcn4_a += -dc * disp;
+        }
+        // ATM at zero charges: triples {a, b, c} with c > b, c != a
+        const double u = r2;
+        if (u <= gfn2::kD4AtmThr) {
+            const double cab = gfn2::kD4A1 * sqrt(3.0 * ea.r4r2 * eb.r4r2) + gfn2::kD4A2;
+            const double c6ab = pb[(size_t)(a * nat + b) * kPairStride + 10];
+            const double dc6ab = pb[(size_t)(a * nat + b) * kPairStride + 11];
+            for (int c = b + 1; c < nat; ++c) {
+                if (c == a) continue;
+                const double dac[3] = {ss.sx[3 * a] - ss.sx[3 * c], ss.sx[3 * a + 1] - ss.sx[3 * c + 1],
+                                       ss.sx[3 * a + 2] - ss.sx[3 * c + 2]};
+                const double dbc[3] = {ss.sx[3 * b] - ss.sx[3 * c], ss.sx[3 * b + 1] - ss.sx[3 * c + 1],
+                                       ss.sx[3 * b + 2] - ss.sx[3 * c + 2]};
+                const double w = dac[0] * dac[0] + dac[1] * dac[1] + dac[2] * dac[2];
+                const double v = dbc[0] * dbc[0] + dbc[1] * dbc[1] + dbc[2] * dbc[2];
+                if (w > gfn2::kD4AtmThr || v > gfn2::kD4AtmThr) continue;
+                const ElemParams& ec = g_tab.e[ss.sel[c]];
+                const double cac = gfn2::kD4A1 * sqrt(3.0 * ea.r4r2 * ec.r4r2) + gfn2::kD4A2;
+                const double cbc = gfn2::kD4A1 * sqrt(3.0 * eb.r4r2 * ec.r4r2) + gfn2::kD4A2;
+                const double uvw = u * v * w, rijk = sqrt(uvw), cijk = cab * cbc * cac;
+                const double A_ = u + v - w, B_ = u + w - v, C_ = w + v - u;
+                const double tt = A_ * B_ * C_ / uvw;
+                const double r3 = 1.0 / (uvw * rijk);
+                const double atm = (0.375 * tt + 1.0) * r3;
+                const double alp3 = gfn2::kD4Alp / 3.0;
+                const double x = pow(cijk / rijk, alp3);
+                const double fd = 1.0 / (1.0 + 6.0 * x);
+                const double c6ac = pb[(size_t)(a * nat + c) * kPairStride + 10];
+                const double c6bc = pb[(size_t)(b * nat + c) * kPairStride + 10];
+                const double c9 = gfn2::kD4S9 * sqrt(c6ab * c6bc * c6ac);
+                const double eabc = c9 * atm * fd;
+                const double dtt_du = (B_ * C_ + A_ * C_ - A_ * B_) / uvw - tt / u;
+                const double dtt_dw = (-B_ * C_ + A_ * C_ + A_ * B_) / uvw - tt / w;
+                const double datm_du = 0.375 * dtt_du * r3 - 1.5 * (0.375 * tt + 1.0) * r3 / u;
+                const double datm_dw = 0.375 * dtt_dw * r3 - 1.5 * (0.375 * tt + 1.0) * r3 / w;
+                const double df = fd * fd * 6.0 * alp3 * x * 0.5;
+                const double de_du = c9 * (datm_du * fd + atm * df / u);
+                const double de_dw = c9 * (datm_dw * fd + atm * df / w);
+                for (int q = 0; q < 3; ++q) g[q] += 2.0 * (de_du * rv[q] + de_dw * dac[q]);
+                const double dc6ac = pb[(size_t)(a * nat + c) * kPairStride + 11];
+                dcn4_a += 0.5 * eabc * (dc6ab / c6ab + dc6ac / c6ac);
+            }
+        }
+        o[5] = g[0] + sr * rv[0];
+        o[6] = g[1] + sr * rv[1];
+        o[7] = g[2] + sr * rv[2];
+        o[8] = dcn_a;
+        o[9] = dcn4_a;
+    }
+    __syncthreads();
+    // Phase C: per-atom reduction of forces and CN derivatives
+    for (int a = tid; a < nat; a += QX_BLOCK) {
+        double g[3] = {0, 0, 0}, dc = 0.0, dc4 = 0.0;
+        for (int b = 0; b < nat; ++b) {
+            if (b == a) continue;
+            const double* o = pb + (size_t)(a * nat + b) * kPairStride;
+            for (int c = 0; c < 3; ++c) g[c] += o[5 + c];
+            dc += o[8];
+            dc4 += o[9];
+            if (b < a) {
+                for (int c = 0; c < 3; ++c) g[c] += o[c];
+                dc += o[3];
+            } else {
+                const double* o2 = pb + (size_t)(b * nat + a) * kPairStride;
+                for (int c = 0; c < 3; ++c) g[c] -= o2[c];
+                dc += o2[4];
+            }
+        }
+        // on-site self-energies
+        for (int s = ss.shoff[a]; s < ss.shoff[a + 1]; ++s) {
+            double pd = 0.0;
+            for (int k = 0; k < ss.sh[s].nao; ++k) {
+                const int mu = ss.sh[s].ao0 + k;
+                pd += P[(size_t)mu * ld + mu];
+            }
+            dc += pd * (-g_tab.e[ss.sh[s].el].kcn[ss.sh[s].k]);
+        }
+        for (int c = 0; c < 3; ++c) gacc[3 * a + c] = g[c];
+        dedcn[a] = dc;
+        dedcn4[a] = dc4;
+    }
+    __syncthreads();
+    // Phase D: CN chain rules, ordered pairs
+    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
+        const int a = p / nat, b = p % nat;
+        double* o = pb + (size_t)p * kPairStride;
+        if (a == b) {
+            o[12] = o[13] = o[14] = 0.0;
+            continue;
+        }
+        const ElemParams& ea = g_tab.e[ss.sel[a]];
+        const ElemParams& eb = g_tab.e[ss.sel[b]];
+        const double rv[3] = {ss.sx[3 * a] - ss.sx[3 * b], ss.sx[3 * a + 1] - ss.sx[3 * b + 1],
+                              ss.sx[3 * a + 2] - ss.sx[3 * b + 2]};
+        const double r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+        const double r = sqrt(r2);
+        double f = 0.0, df = 0.0, fac = 0.0;
+        if (r2 <= gfn2::kCnCutoff * gfn2::kCnCutoff) {
+            gfn_count(ea.covrad + eb.covrad, r, f, df);
+            fac += (dedcn[a] + dedcn[b]) * df / r;
+        }
+        if (r2 <= gfn2::kD4CnThr) {
+            d4_count(ea, eb, r, f, df);
+            fac += (dedcn4[a] + dedcn4[b]) * df / r;
+        }
+        o[12] = fac * rv[0];
+        o[13] = fac * rv[1];
+        o[14] = fac * rv[2];
+    }
+    __syncthreads();
+    for (int a = tid; a < nat; a += QX_BLOCK) {
+        double g[3] = {gacc[3 * a], gacc[3 * a + 1], gacc[3 * a + 2]};
+        for (int b = 0; b < nat; ++b) {
+            if (b == a) continue;
+            const double* o = pb + (size_t)(a * nat + b) * kPairStride;
+            g[0] += o[12];
+            g[1] += o[13];
+            g[2] += o[14];
+        }
+        for (int c = 0; c < 3; ++c) B.grad[3 * (size_t)(si.atom_off + a) + c] = g[c];
+    }
+}
+
+}  // namespace qx

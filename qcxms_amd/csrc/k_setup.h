@@ -1,0 +1,394 @@
+// Geometry-dependent setup of one GFN2-xTB evaluation (one workgroup per
+// system): coordination numbers, D4 reference weights, repulsion and ATM
+// energies, S/H0/dipole/quadrupole integrals, the ES2 gamma matrix, the D4
+// charge-response matrix and the Cholesky orthogonaliser X = L^-T of S.
+#pragma once
+#include "integrals.h"
+
+namespace qx {
+
+QX_DEV double d4_zeta(double a, double c, double qref, double qmod)
+{
+    if (qmod < 0.0) return exp(a);
+    return exp(a * (1.0 - exp(c * (1.0 - qref / qmod))));
+}
+QX_DEV double d4_dzeta(double a, double c, double qref, double qmod)
+{
+    if (qmod < 0.0) return 0.0;
+    const double e = exp(c * (1.0 - qref / qmod));
+    return -a * c * e * exp(a * (1.0 - e)) * qref / (qmod * qmod);
+}
+
+// GFN2 coordination-number count and its derivative d(count)/dr
+QX_DEV void gfn_count(double rc, double r, double& f, double& df)
+{
+    const double e1 = exp(-gfn2::kCnKa * (rc / r - 1.0));
+    const double e2 = exp(-gfn2::kCnKb * ((rc + gfn2::kCnShift) / r - 1.0));
+    const double f1 = 1.0 / (1.0 + e1), f2 = 1.0 / (1.0 + e2);
+    const double r2 = r * r;
+    const double d1 = -gfn2::kCnKa * rc * e1 * f1 * f1 / r2;
+    const double d2 = -gfn2::kCnKb * (rc + gfn2::kCnShift) * e2 * f2 * f2 / r2;
+    f = f1 * f2;
+    df = d1 * f2 + f1 * d2;
+}
+
+// D4 covalent count (dftd4.f90:292-325) and derivative
+QX_DEV void d4_count(const ElemParams& ei, const ElemParams& ej, double r, double& f, double& df)
+{
+    const double rco = ei.rcov_d4 + ej.rcov_d4;
+    const double den0 = fabs(ei.en_d4 - ej.en_d4) + gfn2::kD4K5;
+    const double den = gfn2::kD4K4 * exp(-den0 * den0 / gfn2::kD4K6);
+    const double ex = exp(-gfn2::kD4K1 * (rco / r - 1.0));
+    f = den / (1.0 + ex);
+    df = (-gfn2::kD4K1 * rco * ex * den) / (r * r * (1.0 + ex) * (1.0 + ex));
+}
+
+QX_DEV void mp_radius(const ElemParams& e, double cn, double& r, double& drdcn)
+{
+    const double arg = cn - e.mpvcn - gfn2::kAesShift;
+    const double t = exp(-gfn2::kAesKexp * arg);
+    r = e.mprad + (gfn2::kAesRmax - e.mprad) / (1.0 + t);
+    drdcn = (gfn2::kAesRmax - e.mprad) * gfn2::kAesKexp * t / ((1.0 + t) * (1.0 + t));
+}
+
+// K * X(EN) * Y(zeta) and Pi(R) with dPi/dR for an off-site shell pair
+QX_DEV double h0_scale(const ElemParams& ea, int ka, const ElemParams& eb, int kb, double r,
+                       double& pir, double& dpir, double& kxy)
+{
+    const double za = ea.slater[ka], zb = eb.slater[kb];
+    const double yz = sqrt(2.0 * sqrt(za * zb) / (za + zb));  // wexp = 1/2
+    const double den = ea.en - eb.en;
+    const double xen = 1.0 + gfn2::kEnScale * den * den;
+    const double rab = ea.rad + eb.rad;
+    const double sq = sqrt(r / rab);
+    const double fa = 1.0 + ea.shpoly[ka] * sq, fb = 1.0 + eb.shpoly[kb] * sq;
+    pir = fa * fb;
+    dpir = (r > 0.0) ? (ea.shpoly[ka] * fb + eb.shpoly[kb] * fa) * 0.5 / sqrt(r * rab) : 0.0;
+    kxy = 0.5 * (gfn2::kShellK[ka] + gfn2::kShellK[kb]) * xen * yz;
+    return kxy * pir;
+}
+
+QX_DEV double d4_disp(const ElemParams& ei, const ElemParams& ej, double r2, double* ddisp_dr)
+{
+    const double r4r2ij = 3.0 * ei.r4r2 * ej.r4r2;
+    const double r0 = gfn2::kD4A1 * sqrt(r4r2ij) + gfn2::kD4A2;
+    const double r02 = r0 * r0, r06 = r02 * r02 * r02;
+    const double oor6 = 1.0 / (r2 * r2 * r2 + r06);
+    const double oor8 = 1.0 / (r2 * r2 * r2 * r2 + r06 * r02);
+    if (ddisp_dr) {
+        const double r = sqrt(r2);
+        const double door6 = -6.0 * r2 * r2 * r * oor6 * oor6;
+        const double door8 = -8.0 * r2 * r2 * r2 * r * oor8 * oor8;
+        *ddisp_dr = gfn2::kD4S6 * door6 + gfn2::kD4S8 * r4r2ij * door8;
+    }
+    return gfn2::kD4S6 * oor6 + gfn2::kD4S8 * r4r2ij * oor8;
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(QX_BLOCK) void k_setup(BatchPtrs B, int with_chol)
+{
+    const int sys = blockIdx.x;
+    if (B.active && !B.active[sys]) return;
+    const SysInfo si = B.info[sys];
+    const int tid = threadIdx.x;
+    const int nat = si.nat, nsh = si.nsh, n = si.nao, ld = si.ld;
+    __shared__ ShellDesc sh[QX_MAX_NSH];
+    __shared__ int ao_at[QX_MAX_NAO], ao_sh[QX_MAX_NAO];
+    __shared__ double sx[3 * QX_MAX_NAT];
+    __shared__ int sel[QX_MAX_NAT];
+    __shared__ double scn[QX_MAX_NAT], scnd4[QX_MAX_NAT];
+    __shared__ int refoff[QX_MAX_NAT + 1];
+    __shared__ double red[QX_BLOCK / 64];
+    __shared__ double bcast;
+    extern __shared__ double dyn[];
+
+    const double* gx = B.xyz + 3 * (size_t)si.atom_off;
+    for (int i = tid; i < 3 * nat; i += QX_BLOCK) sx[i] = gx[i];
+    for (int i = tid; i < nat; i += QX_BLOCK) sel[i] = B.elem[si.atom_off + i];
+    build_shell_tables(si, B.elem, sh, ao_at, ao_sh);
+    if (tid == 0) {
+        int o = 0;
+        for (int a = 0; a < nat; ++a) {
+            refoff[a] = o;
+            o += g_tab.e[sel[a]].nref;
+        }
+        refoff[nat] = o;
+    }
+    // 1. coordination numbers (each thread owns an atom: no atomics)
+    for (int a = tid; a < nat; a += QX_BLOCK) {
+        const ElemParams& ea = g_tab.e[sel[a]];
+        double cn = 0.0, cn4 = 0.0;
+        for (int b = 0; b < nat; ++b) {
+            if (b == a) continue;
+            const ElemParams& eb = g_tab.e[sel[b]];
+            const double dx = sx[3 * a] - sx[3 * b], dy = sx[3 * a + 1] - sx[3 * b + 1],
+                         dz = sx[3 * a + 2] - sx[3 * b + 2];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double r = sqrt(r2);
+            double f, df;
+            if (r2 <= gfn2::kCnCutoff * gfn2::kCnCutoff) {
+                gfn_count(ea.covrad + eb.covrad, r, f, df);
+                cn += f;
+            }
+            if (r2 <= gfn2::kD4CnThr) {
+                d4_count(ea, eb, r, f, df);
+                cn4 += f;
+            }
+        }
+        scn[a] = cn;
+        scnd4[a] = cn4;
+        B.cn[si.atom_off + a] = cn;
+        B.cnd4[si.atom_off + a] = cn4;
+        double mr, dmr;
+        mp_radius(ea, cn, mr, dmr);
+        B.mrad[si.atom_off + a] = mr;
+        B.dmrad[si.atom_off + a] = dmr;
+    }
+    __syncthreads();
+    // 2. self-energies and D4 weights
+    for (int s = tid; s < nsh; s += QX_BLOCK) {
+        const ElemParams& e = g_tab.e[sh[s].el];
+        B.hself[si.shell_off + s] = e.hself0[sh[s].k] - e.kcn[sh[s].k] * scn[sh[s].at];
+    }
+    for (int a = tid; a < nat; a += QX_BLOCK) {
+        const ElemParams& e = g_tab.e[sel[a]];
+        const double cn = scnd4[a];
+        double norm = 0.0, dnorm = 0.0;
+        for (int r = 0; r < e.nref; ++r)
+            for (int k = 1; k <= e.refc[r]; ++k) {
+                const double twf = k * gfn2::kD4Wf, d = cn - e.refcovcn[r];
+                const double t = exp(-twf * d * d);
+                norm += t;
+                dnorm += 2.0 * twf * (e.refcovcn[r] - cn) * t;
+            }
+        norm = 1.0 / norm;
+        double mx = -1e300;
+        for (int r = 0; r < e.nref; ++r) mx = fmax(mx, e.refcovcn[r]);
+        for (int r = 0; r < e.nref; ++r) {
+            double ex = 0.0, dex = 0.0;
+            for (int k = 1; k <= e.refc[r]; ++k) {
+                const double twf = k * gfn2::kD4Wf, d = cn - e.refcovcn[r];
+                const double t = exp(-twf * d * d);
+                ex += t;
+                dex += 2.0 * twf * (e.refcovcn[r] - cn) * t;
+            }
+            double g = ex * norm;
+            if (g != g) g = (mx == e.refcovcn[r]) ? 1.0 : 0.0;
+            double dg = dex * norm - ex * dnorm * norm * norm;
+            if (dg != dg) dg = 0.0;
+            B.gw[si.ref_off + refoff[a] + r] = g;
+            B.dgw[si.ref_off + refoff[a] + r] = dg;
+        }
+    }
+    // 3. clear the matrix slots (padding stays zero)
+    const size_t ms = (size_t)ld * ld;
+    double* S = B.S + si.mat_off;
+    double* H = B.H0 + si.mat_off;
+    double* D = B.D + 3 * si.mat_off;
+    double* Q = B.Q + 6 * si.mat_off;
+    for (size_t i = tid; i < ms; i += QX_BLOCK) {
+        S[i] = 0.0;
+        H[i] = 0.0;
+    }
+    for (size_t i = tid; i < 3 * ms; i += QX_BLOCK) D[i] = 0.0;
+    for (size_t i = tid; i < 6 * ms; i += QX_BLOCK) Q[i] = 0.0;
+    __syncthreads();
+    // 4. integrals over shell pairs (ia >= ib)
+    const int npair = nsh * (nsh + 1) / 2;
+    for (int p = tid; p < npair; p += QX_BLOCK) {
+        int ia = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+        while (ia * (ia + 1) / 2 > p) --ia;
+        while ((ia + 1) * (ia + 2) / 2 <= p) ++ia;
+        const int ib = p - ia * (ia + 1) / 2;
+        const ShellDesc sa = sh[ia], sb = sh[ib];
+        const ElemParams& ea = g_tab.e[sa.el];
+        const ElemParams& eb = g_tab.e[sb.el];
+        const double* ra = sx + 3 * sa.at;
+        const double* rb = sx + 3 * sb.at;
+        const double t[3] = {rb[0] - ra[0], rb[1] - ra[1], rb[2] - ra[2]};
+        double hs = 0.0;
+        if (sa.at != sb.at) {
+            const double r = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+            double pir, dpir, kxy;
+            hs = 0.5 * (B.hself[si.shell_off + ia] + B.hself[si.shell_off + ib]) *
+                 h0_scale(ea, sa.k, eb, sb.k, r, pir, dpir, kxy);
+        }
+        for (int a = 0; a < sa.nao; ++a)
+            for (int b = 0; b < sb.nao; ++b) {
+                int la[3], lb[3];
+                ao_cart(sa.l, a, la);
+                ao_cart(sb.l, b, lb);
+                double s, dpb[3], qpb[6], dpa[3], qpa[6], qta[6], qtb[6];
+                ao_pair_values(ea, sa.k, eb, sb.k, ra, rb, la, lb, s, dpb, qpb);
+                shift_to_a(t, s, dpb, qpb, dpa, qpa);
+                traceless6(qpa, qta);
+                traceless6(qpb, qtb);
+                const int mu = sa.ao0 + a, nu = sb.ao0 + b;
+                const size_t o1 = (size_t)mu * ld + nu, o2 = (size_t)nu * ld + mu;
+                S[o1] = s;
+                S[o2] = s;
+                if (sa.at != sb.at) {
+                    H[o1] = hs * s;
+                    H[o2] = hs * s;
+                }
+                for (int c = 0; c < 3; ++c) {
+                    D[c * ms + o1] = dpa[c];
+                    D[c * ms + o2] = dpb[c];
+                }
+                for (int k = 0; k < 6; ++k) {
+                    Q[k * ms + o1] = qta[k];
+                    Q[k * ms + o2] = qtb[k];
+                }
+            }
+    }
+    __syncthreads();
+    for (int s = tid; s < nsh; s += QX_BLOCK)
+        for (int a = 0; a < sh[s].nao; ++a) {
+            const int mu = sh[s].ao0 + a;
+            H[(size_t)mu * ld + mu] = B.hself[si.shell_off + s];
+        }
+    // 5. ES2 gamma matrix
+    double* G = B.gam + si.gam_off;
+    for (int p = tid; p < nsh * nsh; p += QX_BLOCK) {
+        const int a = p / nsh, b = p % nsh;
+        const double* ra = sx + 3 * sh[a].at;
+        const double* rb = sx + 3 * sh[b].at;
+        const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
+        const double eta = 0.5 * (g_tab.e[sh[a].el].hard[sh[a].k] + g_tab.e[sh[b].el].hard[sh[b].k]);
+        G[p] = 1.0 / sqrt(dx * dx + dy * dy + dz * dz + 1.0 / (eta * eta));
+    }
+    // 6. D4 charge-response matrix wd4[k][l] = gw_k gw_l C6ref_kl disp_ij (i != j)
+    const int nd = si.ndim;
+    double* WD = B.wd4 + si.d4_off;
+    const double* gw = B.gw + si.ref_off;
+    __syncthreads();
+    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
+        const int i = p / nat, j = p % nat;
+        const ElemParams& ei = g_tab.e[sel[i]];
+        const ElemParams& ej = g_tab.e[sel[j]];
+        const double dx = sx[3 * i] - sx[3 * j], dy = sx[3 * i + 1] - sx[3 * j + 1],
+                     dz = sx[3 * i + 2] - sx[3 * j + 2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double disp = (i != j && r2 <= gfn2::kD4RThr) ? d4_disp(ei, ej, r2, nullptr) : 0.0;
+        for (int ii = 0; ii < ei.nref; ++ii)
+            for (int jj = 0; jj < ej.nref; ++jj) {
+                const int k = refoff[i] + ii, l = refoff[j] + jj;
+                WD[(size_t)k * nd + l] =
+                    disp * gw[k] * gw[l] * g_tab.c6ref[sel[i]][ii][sel[j]][jj];
+            }
+    }
+    // 7. repulsion energy
+    double erep = 0.0;
+    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
+        const int i = p / nat, j = p % nat;
+        if (j >= i) continue;
+        const ElemParams& ei = g_tab.e[sel[i]];
+        const ElemParams& ej = g_tab.e[sel[j]];
+        const double dx = sx[3 * i] - sx[3 * j], dy = sx[3 * i + 1] - sx[3 * j + 1],
+                     dz = sx[3 * i + 2] - sx[3 * j + 2];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        if (r > gfn2::kRepCutoff) continue;
+        const double kexp = (ei.z <= 2 && ej.z <= 2) ? gfn2::kRepKexpLight : gfn2::kRepKexp;
+        erep += ei.rep_zeff * ej.rep_zeff / r * exp(-sqrt(ei.rep_alpha * ej.rep_alpha) * pow(r, kexp));
+    }
+    erep = block_sum(erep, red);
+    // 8. ATM energy at zero charges: pair C6 into scratch, then triples
+    double* c60 = B.pairbuf + si.pair_off;  // nat*nat
+    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
+        const int i = p / nat, j = p % nat;
+        const ElemParams& ei = g_tab.e[sel[i]];
+        const ElemParams& ej = g_tab.e[sel[j]];
+        double c = 0.0;
+        for (int ii = 0; ii < ei.nref; ++ii) {
+            const double zi = d4_zeta(gfn2::kD4Ga, ei.gam * gfn2::kD4Gc, ei.refq[ii] + ei.zeff, ei.zeff) *
+                              gw[refoff[i] + ii];
+            for (int jj = 0; jj < ej.nref; ++jj) {
+                const double zj = d4_zeta(gfn2::kD4Ga, ej.gam * gfn2::kD4Gc, ej.refq[jj] + ej.zeff,
+                                          ej.zeff) * gw[refoff[j] + jj];
+                c += zi * zj * g_tab.c6ref[sel[i]][ii][sel[j]][jj];
+            }
+        }
+        c60[p] = c;
+    }
+    __syncthreads();
+    double eatm = 0.0;
+    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
+        const int i = p / nat, j = p % nat;
+        if (j >= i) continue;
+        const double dij[3] = {sx[3 * i] - sx[3 * j], sx[3 * i + 1] - sx[3 * j + 1], sx[3 * i + 2] - sx[3 * j + 2]};
+        const double u = dij[0] * dij[0] + dij[1] * dij[1] + dij[2] * dij[2];
+        if (u > gfn2::kD4AtmThr) continue;
+        const ElemParams& ei = g_tab.e[sel[i]];
+        const ElemParams& ej = g_tab.e[sel[j]];
+        const double cij = gfn2::kD4A1 * sqrt(3.0 * ei.r4r2 * ej.r4r2) + gfn2::kD4A2;
+        for (int k = 0; k < j; ++k) {
+            const double dik[3] = {sx[3 * i] - sx[3 * k], sx[3 * i + 1] - sx[3 * k + 1], sx[3 * i + 2] - sx[3 * k + 2]};
+            const double djk[3] = {sx[3 * j] - sx[3 * k], sx[3 * j + 1] - sx[3 * k + 1], sx[3 * j + 2] - sx[3 * k + 2]};
+            const double w = dik[0] * dik[0] + dik[1] * dik[1] + dik[2] * dik[2];
+            const double v = djk[0] * djk[0] + djk[1] * djk[1] + djk[2] * djk[2];
+            if (w > gfn2::kD4AtmThr || v > gfn2::kD4AtmThr) continue;
+            const ElemParams& ek = g_tab.e[sel[k]];
+            const double cik = gfn2::kD4A1 * sqrt(3.0 * ei.r4r2 * ek.r4r2) + gfn2::kD4A2;
+            const double cjk = gfn2::kD4A1 * sqrt(3.0 * ej.r4r2 * ek.r4r2) + gfn2::kD4A2;
+            const double uvw = u * v * w, rijk = sqrt(uvw), cijk = cij * cjk * cik;
+            const double aa = u + v - w, bb = u + w - v, cc = w + v - u;
+            const double atm = (0.375 * aa * bb * cc / uvw + 1.0) / (rijk * rijk * rijk);
+            const double x = pow(cijk / rijk, gfn2::kD4Alp / 3.0);
+            const double fd = 1.0 / (1.0 + 6.0 * x);
+            const double c9 = gfn2::kD4S9 * sqrt(c60[i * nat + j] * c60[j * nat + k] * c60[i * nat + k]);
+            eatm += c9 * atm * fd;
+        }
+    }
+    eatm = block_sum(eatm, red);
+    if (tid == 0) {
+        B.eparts[10 * sys + 6] = eatm;
+        B.eparts[10 * sys + 7] = erep;
+    }
+    if (!with_chol) return;
+    // 9. Cholesky S = L L^T and X = L^-T
+    double* A = (ld <= QX_LDS_MAXM) ? dyn : (B.L + si.mat_off);
+    double* Lg = B.L + si.mat_off;
+    double* Xg = B.X + si.mat_off;
+    __syncthreads();
+    for (size_t i = tid; i < ms; i += QX_BLOCK) A[i] = S[i];
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        if (tid == 0) {
+            const double d = sqrt(A[(size_t)j * ld + j]);
+            A[(size_t)j * ld + j] = d;
+            bcast = d;
+        }
+        __syncthreads();
+        const double d = bcast;
+        for (int i = j + 1 + tid; i < n; i += QX_BLOCK) A[(size_t)i * ld + j] /= d;
+        __syncthreads();
+        const int m = n - j - 1;
+        for (int p = tid; p < m * m; p += QX_BLOCK) {
+            const int i = j + 1 + p / m, k = j + 1 + p % m;
+            if (k <= i) A[(size_t)i * ld + k] -= A[(size_t)i * ld + j] * A[(size_t)k * ld + j];
+        }
+        __syncthreads();
+    }
+    // zero the upper triangle, store L
+    for (size_t p = tid; p < ms; p += QX_BLOCK) {
+        const int i = (int)(p / ld), k = (int)(p % ld);
+        const double v = (k <= i && i < n) ? A[p] : 0.0;
+        A[p] = v;
+        Lg[p] = v;
+    }
+    __syncthreads();
+    // X = (L^-1)^T: column c of L^-1 by forward substitution, written as row c of X
+    for (size_t p = tid; p < ms; p += QX_BLOCK) Xg[p] = 0.0;
+    __syncthreads();
+    for (int c = tid; c < n; c += QX_BLOCK) {
+        double* xr = Xg + (size_t)c * ld;  // X[c][i] = Linv[i][c]
+        for (int i = c; i < n; ++i) {
+            double v = (i == c) ? 1.0 : 0.0;
+            for (int k = c; k < i; ++k) v -= A[(size_t)i * ld + k] * xr[k];
+            xr[i] = v / A[(size_t)i * ld + i];
+        }
+    }
+    // X as stored: row c holds column c of L^-1, i.e. Xg = (L^-1)^T row-major
+}
+
+}  // namespace qx

@@ -1,0 +1,166 @@
+"""ctypes host layer over libqcxms_gpu.so.
+
+Mirrors the reference's bridge get_xtb_egrad (src/tblite.f90:65-175): same
+argument meaning, same status codes (0 ok, 5 unknown method, -1 failure).
+There is no CPU fallback: a missing library raises LibraryMissing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+GFN1_XTB, GFN2_XTB, IPEA1_XTB = 1, 2, 11  # tblite.f90:34-40
+STAT_OK, STAT_UNKNOWN_METHOD, STAT_FATAL, STAT_INVALID = 0, 5, -1, -2
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "libqcxms_gpu.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def library():
+    """Load libqcxms_gpu.so (built by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise LibraryMissing(f"{library_path} is not built; run __graft_entry__.build()")
+        L = C.CDLL(library_path)
+        L.qcxms_gpu_last_error.restype = C.c_char_p
+        L.qcxms_batch_create.restype = C.c_void_p
+        L.qcxms_batch_create.argtypes = [C.c_int32, _ip, _ip, _ip, C.c_int32]
+        L.qcxms_batch_destroy.argtypes = [C.c_void_p]
+        L.qcxms_batch_set_scc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32,
+                                          C.c_double, C.c_double]
+        L.qcxms_batch_set_warm_start.argtypes = [C.c_void_p, C.c_int32]
+        L.qcxms_batch_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+        L.qcxms_batch_kernel_time.argtypes = [C.c_void_p, C.c_char_p, _dp, _ip]
+        L.qcxms_batch_egrad.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _ip]
+        L.qcxms_batch_info.argtypes = [C.c_void_p, _ip, _dp]
+        L.qcxms_batch_multipoles.argtypes = [C.c_void_p, _dp, _dp]
+        L.qcxms_get_xtb_egrad.argtypes = [C.c_int32, _ip, _dp, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_double, _dp, _dp, _dp, _ip]
+        L.qcxms_egrad_batch.argtypes = [C.c_int32, _ip, _ip, _dp, _ip, _ip, C.c_int32, _dp, _dp,
+                                        _dp, _dp, _ip]
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return library().qcxms_gpu_last_error().decode()
+
+
+def get_xtb_egrad(num, xyz, charge, multiplicity, method=GFN2_XTB, etemp=300.0):
+    """Energy (Eh), gradient (Eh/bohr, [nat,3]), Mulliken charges and status
+    for one structure (xyz in Bohr, [nat,3]) -- the reference's
+    get_xtb_egrad with a fresh wavefunction per call."""
+    L = library()
+    num = np.ascontiguousarray(num, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    nat = len(num)
+    qat = np.zeros(nat)
+    grad = np.zeros((nat, 3))
+    e = C.c_double()
+    stat = C.c_int32()
+    L.qcxms_get_xtb_egrad(nat, _p(num, _ip), _p(xyz), int(charge), int(multiplicity),
+                          int(method), float(etemp), _p(qat), C.byref(e), _p(grad),
+                          C.byref(stat))
+    return qat, e.value, grad, stat.value
+
+
+class Batch:
+    """Device-resident batch of systems evaluated together (one workgroup per
+    system).  systems: list of (num, charge)."""
+
+    def __init__(self, systems, method=GFN2_XTB):
+        L = library()
+        nums = [np.asarray(s[0], dtype=np.int32) for s in systems]
+        self.offsets = np.cumsum([0] + [len(n) for n in nums]).astype(np.int32)
+        self.num = np.concatenate(nums).astype(np.int32)
+        self.charge = np.array([int(s[1]) for s in systems], dtype=np.int32)
+        self.nsys = len(systems)
+        self.natoms = int(self.offsets[-1])
+        h = L.qcxms_batch_create(self.nsys, _p(self.offsets, _ip), _p(self.num, _ip),
+                                 _p(self.charge, _ip), int(method))
+        if not h:
+            raise RuntimeError("qcxms_batch_create failed: " + last_error())
+        self._h = C.c_void_p(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().qcxms_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_scc(self, econv=1e-9, pconv=1e-8, maxiter=100, damping=0.4, jacobi_tol=1e-11):
+        st = library().qcxms_batch_set_scc(self._h, econv, pconv, maxiter, damping, jacobi_tol)
+        if st:
+            raise RuntimeError(last_error())
+
+    def set_warm_start(self, warm):
+        library().qcxms_batch_set_warm_start(self._h, int(bool(warm)))
+
+    def set_profiling(self, on):
+        library().qcxms_batch_set_profiling(self._h, int(bool(on)))
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        n = C.c_int32()
+        if library().qcxms_batch_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n)):
+            raise RuntimeError(last_error())
+        return ms.value, n.value
+
+    def egrad(self, xyz, etemp=300.0, gradient=True):
+        """xyz: [natoms,3] Bohr (all systems concatenated).  Returns
+        (energy[nsys], grad[natoms,3] or None, qat[natoms], stat[nsys])."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.natoms, 3)
+        et = np.ascontiguousarray(np.broadcast_to(np.asarray(etemp, dtype=np.float64),
+                                                  (self.nsys,)))
+        e = np.zeros(self.nsys)
+        g = np.zeros((self.natoms, 3)) if gradient else None
+        q = np.zeros(self.natoms)
+        st = np.zeros(self.nsys, dtype=np.int32)
+        rc = library().qcxms_batch_egrad(self._h, _p(xyz), _p(et), _p(e), _p(g), _p(q),
+                                         _p(st, _ip))
+        if rc not in (STAT_OK, STAT_FATAL):
+            raise RuntimeError(f"qcxms_batch_egrad failed ({rc}): " + last_error())
+        return e, g, q, st
+
+    def info(self):
+        it = np.zeros(self.nsys, dtype=np.int32)
+        parts = np.zeros((self.nsys, 10))
+        library().qcxms_batch_info(self._h, _p(it, _ip), _p(parts))
+        return it, parts
+
+    def multipoles(self):
+        d = np.zeros((self.natoms, 3))
+        q = np.zeros((self.natoms, 6))
+        library().qcxms_batch_multipoles(self._h, _p(d), _p(q))
+        return d, q
+
+
+def egrad_batch(systems, xyz_list, etemp=300.0, method=GFN2_XTB):
+    """One-shot batched single points: systems [(num, charge)], xyz_list of
+    [nat,3] arrays.  Returns lists of energies, gradients, charges, stats."""
+    b = Batch(systems, method)
+    try:
+        e, g, q, st = b.egrad(np.concatenate(xyz_list), etemp)
+    finally:
+        b.close()
+    off = b.offsets
+    return (e, [g[off[i]:off[i + 1]] for i in range(len(systems))],
+            [q[off[i]:off[i + 1]] for i in range(len(systems))], st)
