@@ -84,6 +84,75 @@ int qcxms_batch_set_profiling(qcxms_batch *b, int32_t enable);
  * "setup", "scc", "grad", "md_pre", "md_post"; counters reset on read. */
 int qcxms_batch_kernel_time(qcxms_batch *b, const char *kernel, double *ms, int32_t *launches);
 
+
+/* ---- molecular dynamics: the EI production loop of src/md.f90:285-682 ----
+ * All trajectories of an ensemble advance together; each step is one
+ * leap-frog update, one GFN2 energy+gradient (warm-started SCC), heating by
+ * impactscale, setetemp, fragment connectivity and the exit tests.  A
+ * trajectory that meets an exit condition stops while the others continue. */
+typedef struct qcxms_md qcxms_md;
+
+typedef struct {
+    double tstep;       /* time step in atomic units (0.5 fs = 0.5*41.3413733365614) */
+    double etempin;     /* electronic temperature (K); < 0: setetemp (utility.f90:69-86) */
+    double ax;          /* common1 ax (0 for GFN2 runs, input.f90) */
+    double ieetemp;     /* IEE-dependent etemp scaling (input.f90:199 default 0) */
+    int32_t nfragexit;  /* exit when more fragments than this appear (input.f90:216: 3) */
+    int32_t max_steps;  /* nmax = tmax / tstep */
+    int32_t use_graph;  /* != 0: replay the five kernels of a step as one hipGraph */
+    int32_t pad;
+    double econv, pconv;  /* SCC thresholds (defaults 1e-9, 1e-8 when 0) */
+} qcxms_md_params;
+
+/* Final or current state of one trajectory (md.f90:684-700 outputs). */
+typedef struct {
+    int32_t nstep, nfrag, fragstate, mdok, exit_reason, running, scc_iterations, pad;
+    double tav, epav, ekav;  /* averages over the steps run */
+    double ttime, dtime, atlast, epot, ekin, eerror, etemp;
+    double fragm[10];        /* fragment masses (amu), compacted as fragmass */
+    double fragT[10];        /* fragment temperatures (K) */
+} qcxms_md_result;
+
+/* exit_reason codes: 0 running, 1 SCF/checkqc failure (E = 0), 2 energy drift
+ * > 0.1 Eh, 3 more than 6 fragments, 4 nfrag > nfragexit, 5 nothing happens
+ * (fconst > 1000), 6 nfrag = nfragexit for 250 more steps, 7 max_steps
+ * reached, 8 impactscale found no scaling within 20000 iterations. */
+
+/* ntraj trajectories (offsets/num/charge as qcxms_batch_create); per atom:
+ * mass (m_e), imass (isotope mass number or -1), xyz (Bohr), velo (au),
+ * velof (heating factors, main.F90:843-853); per trajectory: eimp (Eh) and
+ * tadd (au).  Runs the initial egrad (md.f90:182). */
+qcxms_md *qcxms_md_create(int32_t ntraj, const int32_t *offsets, const int32_t *num,
+                          const int32_t *charge, const double *mass, const int32_t *imass,
+                          const double *xyz, const double *velo, const double *velof,
+                          const double *eimp, const double *tadd, const qcxms_md_params *params);
+void qcxms_md_destroy(qcxms_md *md);
+/* Advance every running trajectory by up to nsteps loop iterations;
+ * *running receives the number still running afterwards (may be NULL). */
+int qcxms_md_run(qcxms_md *md, int32_t nsteps, int32_t *running);
+/* Copy state back: any pointer may be NULL.  list = fragment labels. */
+int qcxms_md_read(qcxms_md *md, double *xyz, double *velo, double *grad, double *qat,
+                  int32_t *list, qcxms_md_result *results);
+/* Timing hooks of the ensemble's underlying batch (see qcxms_batch_kernel_time). */
+qcxms_batch *qcxms_md_batch(qcxms_md *md);
+
+/* ---- batched drop-ins for single reference routines ---- */
+/* fragment_structure(nat,iat,xyz,3.0,1,0,list) + fragmass masses
+ * (src/fragments.f90:10-182, as called at src/md.f90:450-451); systems with
+ * up to 64 atoms.  nfrag[nsys], fragm[10*nsys] (amu). */
+int qcxms_fragments_batch(int32_t nsys, const int32_t *offsets, const int32_t *num,
+                          const double *xyz, const double *mass, int32_t *list, int32_t *nfrag,
+                          double *fragm);
+/* leapfrog (src/md.f90:749-773); ekin[nsys]. */
+int qcxms_leapfrog_batch(int32_t nsys, const int32_t *offsets, const double *grad,
+                         const double *mass, double tstep, double *xyz, double *velo,
+                         double *ekin);
+/* impactscale (src/impact.f90:12-55); per system eimp, ff, e0; iterations[nsys]
+ * (>= 20000: the reference stops with 'error in impactscale'). */
+int qcxms_impactscale_batch(int32_t nsys, const int32_t *offsets, double *velo,
+                            const double *mass, const double *velof, const double *eimp,
+                            const double *ff, const double *e0, int32_t *iterations);
+
 #ifdef __cplusplus
 }
 #endif

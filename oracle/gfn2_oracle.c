@@ -1771,3 +1771,63 @@ int orc_gfn2_egrad_batch(int nsys, const int32_t *offsets, const int32_t *num, c
     free(th);
     return 0;
 }
+
+/* ------------------------------------------------------------ MD with GFN2 */
+/* egrad callback for orc_md_run: ctx points to orc_scc_opts (iniqm.f90:641-655
+ * without checkqc, which orc_md_run applies) */
+int orc_gfn2_md_egrad(void *ctx, int nat, const int32_t *iat, const double *xyz, int chrg,
+                      double etemp, double *e, double *g, double *q)
+{
+    return orc_gfn2_egrad(nat, iat, xyz, chrg, etemp, (const orc_scc_opts *)ctx, e, g, q, NULL,
+                          NULL, NULL);
+}
+
+typedef struct {
+    int ntraj;
+    const int32_t *offsets, *num, *imass;
+    const double *mass, *velof;
+    double *xyz, *velo, *grad, *qat;
+    int32_t *list;
+    orc_md_state *states;
+    const orc_scc_opts *opts;
+    int steps_limit, next;
+    pthread_mutex_t lock;
+} mdbatch_t;
+
+static void *md_worker(void *arg)
+{
+    mdbatch_t *b = (mdbatch_t *)arg;
+    for (;;) {
+        pthread_mutex_lock(&b->lock);
+        int s = b->next++;
+        pthread_mutex_unlock(&b->lock);
+        if (s >= b->ntraj) break;
+        int o = b->offsets[s];
+        orc_md_run(&b->states[s], b->num + o, b->imass + o, b->mass + o, b->velof + o,
+                   b->xyz + 3 * o, b->velo + 3 * o, b->grad + 3 * o, b->qat + o, b->list + o,
+                   orc_gfn2_md_egrad, (void *)b->opts, b->steps_limit);
+    }
+    return NULL;
+}
+
+/* CPU baseline: whole trajectories per worker thread, as bin/pqcxms runs one
+ * single-threaded qcxms process per core (bin/pqcxms:27-28,91,98). */
+int orc_md_run_batch(int ntraj, const int32_t *offsets, const int32_t *num, const int32_t *imass,
+                     const double *mass, const double *velof, double *xyz, double *velo,
+                     double *grad, double *qat, int32_t *list, orc_md_state *states,
+                     const orc_scc_opts *opts, int steps_limit, int nthreads)
+{
+    mdbatch_t b;
+    b.ntraj = ntraj; b.offsets = offsets; b.num = num; b.imass = imass; b.mass = mass;
+    b.velof = velof; b.xyz = xyz; b.velo = velo; b.grad = grad; b.qat = qat; b.list = list;
+    b.states = states; b.opts = opts; b.steps_limit = steps_limit; b.next = 0;
+    pthread_mutex_init(&b.lock, NULL);
+    d4_setup();
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, md_worker, &b);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    pthread_mutex_destroy(&b.lock);
+    return 0;
+}

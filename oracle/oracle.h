@@ -118,6 +118,14 @@ int orc_gfn2_egrad_batch(int nsys, const int32_t *offsets, const int32_t *num,
                          const orc_scc_opts *opts, double *energy, double *grad,
                          double *qat, int32_t *stat, int nthreads);
 
+/* MD driven by the GFN2 oracle (ctx: orc_scc_opts*), and its threaded batch */
+int orc_gfn2_md_egrad(void *ctx, int nat, const int32_t *iat, const double *xyz, int chrg,
+                      double etemp, double *e, double *g, double *q);
+int orc_md_run_batch(int ntraj, const int32_t *offsets, const int32_t *num, const int32_t *imass,
+                     const double *mass, const double *velof, double *xyz, double *velo,
+                     double *grad, double *qat, int32_t *list, orc_md_state *states,
+                     const orc_scc_opts *opts, int steps_limit, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,8 +4,10 @@ The compute runs in libqcxms_gpu.so (HIP, gfx950) through the C ABI declared
 in include/qcxms_gpu.h; this package is its Python host layer, mirroring the
 reference's Fortran interface (src/tblite.f90 get_xtb_egrad, src/md.f90).
 """
-from .gpu import (GFN1_XTB, GFN2_XTB, IPEA1_XTB, Batch, LibraryMissing, egrad_batch,
-                  get_xtb_egrad, library, library_path)
+from .gpu import (EXIT_REASONS, GFN1_XTB, GFN2_XTB, IPEA1_XTB, Batch, LibraryMissing,
+                  MdEnsemble, egrad_batch, fragments_batch, get_xtb_egrad, impactscale_batch,
+                  leapfrog_batch, library, library_path)
 
-__all__ = ["GFN1_XTB", "GFN2_XTB", "IPEA1_XTB", "Batch", "LibraryMissing", "egrad_batch",
-           "get_xtb_egrad", "library", "library_path"]
+__all__ = ["EXIT_REASONS", "GFN1_XTB", "GFN2_XTB", "IPEA1_XTB", "Batch", "LibraryMissing",
+           "MdEnsemble", "egrad_batch", "fragments_batch", "get_xtb_egrad", "impactscale_batch",
+           "leapfrog_batch", "library", "library_path"]

@@ -13,9 +13,11 @@
 #include "../../include/qcxms_gpu.h"
 #include "common.h"
 #include "d4_data.h"
+#include "radii.h"
 #include "sto_ng_table.h"
 
 __constant__ DevTables g_tab;
+__constant__ double g_rad[118];
 
 #include "k_grad.h"
 #include "k_md.h"
@@ -172,11 +174,15 @@ int ensure_init(int device)
     HIPCHK(hipSetDevice(device));
     build_tables(g_host_tab);
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_tab), &g_host_tab, sizeof(DevTables)));
+    double rad[118];
+    for (int z = 0; z < 118; ++z) rad[z] = gfn2::kAatoAu * kQcxmsRadAa[z];
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_rad), rad, sizeof(rad)));
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_scc, hipFuncAttributeMaxDynamicSharedMemorySize,
                                2 * QX_LDS_MAXM * QX_LDS_MAXM * (int)sizeof(double)));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_setup, hipFuncAttributeMaxDynamicSharedMemorySize,
                                QX_LDS_MAXM * QX_LDS_MAXM * (int)sizeof(double)));
+    HIPCHK(hipFuncSetAttribute((const void*)qx::k_grad, hipFuncAttributeMaxDynamicSharedMemorySize, 0));
     g_device = device;
     g_ready = true;
     return QCXMS_OK;
@@ -360,53 +366,85 @@ struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
 };
 
-int launch_egrad(qcxms_batch* b, bool with_grad)
+// enqueue one kernel, optionally bracketed by events whose elapsed time is
+// added to timer t after the next synchronisation point
+struct PendingTime {
+    hipEvent_t a, b;
+    Timer* t;
+};
+thread_local std::vector<PendingTime> g_pending;
+
+template <class F>
+int timed(qcxms_batch* b, Timer& t, F&& launch)
 {
-    // systems whose matrices fit in LDS use dynamic LDS of their own order;
-    // size it for the largest such order present (at most QX_LDS_MAXM)
-    const size_t ls = std::min(b->max_ld, QX_LDS_MAXM), lm = std::min(b->max_m, QX_LDS_MAXM);
-    const size_t lds_setup = sizeof(double) * ls * ls;
-    const size_t lds_scc = 2 * sizeof(double) * lm * lm;
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     if (b->profiling) {
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventCreate(&e2));
-        HIPCHK(hipEventCreate(&e3));
         HIPCHK(hipEventRecord(e0, g_stream));
     }
-    if (!b->warm_start) HIPCHK(hipMemsetAsync(b->d.warm, 0, sizeof(int) * b->nsys, g_stream));
-    hipLaunchKernelGGL(qx::k_setup, dim3(b->nsys), dim3(QX_BLOCK), lds_setup, g_stream, b->d, 1);
+    launch();
     HIPCHK(hipGetLastError());
-    if (b->profiling) HIPCHK(hipEventRecord(e1, g_stream));
-    hipLaunchKernelGGL(qx::k_scc, dim3(b->nsys), dim3(QX_BLOCK), lds_scc, g_stream, b->d, b->opts);
-    HIPCHK(hipGetLastError());
-    if (b->profiling) HIPCHK(hipEventRecord(e2, g_stream));
-    if (with_grad) {
-        hipLaunchKernelGGL(qx::k_grad, dim3(b->nsys), dim3(QX_BLOCK), 0, g_stream, b->d);
-        HIPCHK(hipGetLastError());
-    }
     if (b->profiling) {
-        HIPCHK(hipEventRecord(e3, g_stream));
-        HIPCHK(hipEventSynchronize(e3));
-        float t1, t2, t3;
-        HIPCHK(hipEventElapsedTime(&t1, e0, e1));
-        HIPCHK(hipEventElapsedTime(&t2, e1, e2));
-        HIPCHK(hipEventElapsedTime(&t3, e2, e3));
-        b->t_setup.ms += t1;
-        b->t_setup.launches += 1;
-        b->t_scc.ms += t2;
-        b->t_scc.launches += 1;
-        if (with_grad) {
-            b->t_grad.ms += t3;
-            b->t_grad.launches += 1;
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipEventDestroy(e2);
-        (void)hipEventDestroy(e3);
+        HIPCHK(hipEventRecord(e1, g_stream));
+        g_pending.push_back({e0, e1, &t});
     }
     return QCXMS_OK;
+}
+
+int collect_times()
+{
+    for (auto& p : g_pending) {
+        HIPCHK(hipEventSynchronize(p.b));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+        p.t->ms += ms;
+        p.t->launches += 1;
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    g_pending.clear();
+    return QCXMS_OK;
+}
+
+size_t lds_setup_bytes(const qcxms_batch* b)
+{
+    const size_t ls = std::min(b->max_ld, QX_LDS_MAXM);
+    return sizeof(double) * ls * ls;
+}
+size_t lds_scc_bytes(const qcxms_batch* b)
+{
+    const size_t lm = std::min(b->max_m, QX_LDS_MAXM);
+    return 2 * sizeof(double) * lm * lm;
+}
+
+// setup -> SCC -> gradient for every system of the batch (masked by d.active)
+int enqueue_egrad(qcxms_batch* b, bool with_grad)
+{
+    int st;
+    const size_t l1 = lds_setup_bytes(b), l2 = lds_scc_bytes(b);
+    if ((st = timed(b, b->t_setup, [&] {
+             hipLaunchKernelGGL(qx::k_setup, dim3(b->nsys), dim3(QX_BLOCK), l1, g_stream, b->d, 1);
+         })))
+        return st;
+    if ((st = timed(b, b->t_scc, [&] {
+             hipLaunchKernelGGL(qx::k_scc, dim3(b->nsys), dim3(QX_BLOCK), l2, g_stream, b->d, b->opts);
+         })))
+        return st;
+    if (with_grad &&
+        (st = timed(b, b->t_grad, [&] {
+             hipLaunchKernelGGL(qx::k_grad, dim3(b->nsys), dim3(QX_BLOCK), 0, g_stream, b->d);
+         })))
+        return st;
+    return QCXMS_OK;
+}
+
+int launch_egrad(qcxms_batch* b, bool with_grad)
+{
+    if (!b->warm_start) HIPCHK(hipMemsetAsync(b->d.warm, 0, sizeof(int) * b->nsys, g_stream));
+    int st = enqueue_egrad(b, with_grad);
+    if (st) return st;
+    return b->profiling ? collect_times() : QCXMS_OK;
 }
 
 int set_etemp(qcxms_batch* b, const double* etemp)
@@ -423,6 +461,88 @@ int set_etemp(qcxms_batch* b, const double* etemp)
     if (changed)
         HIPCHK(hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * b->nsys, hipMemcpyHostToDevice,
                               g_stream));
+    return QCXMS_OK;
+}
+
+}  // namespace
+
+
+struct qcxms_md {
+    qcxms_batch* b = nullptr;
+    qx::MdParams p{};
+    qx::MdPtrs m{};
+    int ntraj = 0;
+    bool use_graph = false, graph_ready = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void*> allocs;
+    ~qcxms_md()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : allocs) (void)hipFree(p);
+        delete b;
+    }
+};
+
+namespace {
+
+template <class T>
+int mdalloc(qcxms_md* md, T** p, size_t n, const T* host = nullptr)
+{
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
+    md->allocs.push_back((void*)*p);
+    if (host) HIPCHK(hipMemcpyAsync(*p, host, n * sizeof(T), hipMemcpyHostToDevice, g_stream));
+    else HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), g_stream));
+    return QCXMS_OK;
+}
+
+// one MD loop iteration for all running trajectories
+int enqueue_md_step(qcxms_md* md)
+{
+    qcxms_batch* b = md->b;
+    int st;
+    if ((st = timed(b, b->t_md_pre, [&] {
+             hipLaunchKernelGGL(qx::k_md_pre, dim3(md->ntraj), dim3(64), 0, g_stream, b->d, md->m, md->p);
+         })))
+        return st;
+    if ((st = enqueue_egrad(b, true))) return st;
+    return timed(b, b->t_md_post, [&] {
+        hipLaunchKernelGGL(qx::k_md_post, dim3(md->ntraj), dim3(64), 0, g_stream, b->d, md->m, md->p);
+    });
+}
+
+double host_setetemp(int nfrag, double eimp, double ax, double ieetemp)
+{
+    double etemp = (double)5000.0f + (double)20000.0f * ax;
+    if (eimp > 0 && nfrag <= 1) etemp = etemp + (eimp > 0.0 ? eimp : 0.0) * ieetemp;
+    return etemp;
+}
+
+struct TmpBuf {
+    std::vector<void*> v;
+    ~TmpBuf()
+    {
+        for (void* p : v) (void)hipFree(p);
+    }
+    template <class T>
+    int up(T** d, const T* h, size_t n)
+    {
+        HIPCHK(hipMalloc((void**)d, std::max<size_t>(n, 1) * sizeof(T)));
+        v.push_back((void*)*d);
+        if (h) HIPCHK(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, g_stream));
+        return QCXMS_OK;
+    }
+};
+
+int check_offsets(int nsys, const int32_t* offsets, int maxnat)
+{
+    if (nsys < 1 || !offsets) return fail("invalid system count or offsets");
+    for (int s = 0; s < nsys; ++s) {
+        const int n = offsets[s + 1] - offsets[s];
+        if (n < 1 || n > maxnat) return fail("system size outside 1..64 atoms");
+    }
     return QCXMS_OK;
 }
 
@@ -596,6 +716,261 @@ int qcxms_get_xtb_egrad(int32_t nat, const int32_t* num, const double* xyz, int3
     const int out = (st == QCXMS_UNKNOWN_METHOD) ? QCXMS_UNKNOWN_METHOD : (st == QCXMS_OK ? s : QCXMS_FATAL);
     if (stat) *stat = out;
     return out;
+}
+
+qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* num, const int32_t* charge,
+                          const double* mass, const int32_t* imass, const double* xyz, const double* velo,
+                          const double* velof, const double* eimp, const double* tadd,
+                          const qcxms_md_params* params)
+{
+    g_err.clear();
+    if (!params || !mass || !xyz || !velo || !velof || !eimp || !tadd) {
+        g_err = "null argument";
+        return nullptr;
+    }
+    if (check_offsets(ntraj, offsets, QX_MAX_NAT) != QCXMS_OK) return nullptr;
+    if (!(params->tstep > 0.0) || params->max_steps < 1) {
+        g_err = "tstep and max_steps must be positive";
+        return nullptr;
+    }
+    qcxms_batch* b = qcxms_batch_create(ntraj, offsets, num, charge, QCXMS_GFN2_XTB);
+    if (!b) return nullptr;
+    qcxms_md* md = new qcxms_md();
+    md->b = b;
+    md->ntraj = ntraj;
+    md->use_graph = params->use_graph != 0;
+    md->p.tstep = params->tstep;
+    md->p.etempin = params->etempin;
+    md->p.ax = params->ax;
+    md->p.ieetemp = params->ieetemp;
+    md->p.nfragexit = params->nfragexit;
+    md->p.max_steps = params->max_steps;
+    b->warm_start = 1;
+    if (params->econv > 0.0) b->opts.econv = params->econv;
+    if (params->pconv > 0.0) b->opts.pconv = params->pconv;
+    const int na = b->natoms;
+    std::vector<qx::MdTraj> tr(ntraj);
+    std::vector<int> act(ntraj, 1);
+    for (int s = 0; s < ntraj; ++s) {
+        std::memset(&tr[s], 0, sizeof(qx::MdTraj));
+        tr[s].eimp = eimp[s];
+        tr[s].tadd = tadd[s];
+        tr[s].nfrag = 1;
+        const double et = (params->etempin < 0.0)
+                              ? host_setetemp(1, eimp[s], params->ax, params->ieetemp)
+                              : params->etempin;
+        tr[s].etemp = et;
+        b->info[s].kt = et * gfn2::kKtoAu;
+    }
+    std::vector<int> im(na, -1), nm(na);
+    for (int i = 0; i < na; ++i) {
+        if (imass) im[i] = imass[i];
+        nm[i] = num[i];
+    }
+    std::vector<int> lst(na, 1);
+    int st = QCXMS_OK;
+    double* d_velof;
+    double* d_mass;
+    int* d_imass;
+    int* d_num;
+    if ((st = mdalloc(md, &md->m.traj, ntraj, tr.data())) || (st = mdalloc(md, &md->m.velo, 3 * (size_t)na, velo)) ||
+        (st = mdalloc(md, &d_velof, (size_t)na, velof)) || (st = mdalloc(md, &d_mass, (size_t)na, mass)) ||
+        (st = mdalloc(md, &d_imass, (size_t)na, im.data())) || (st = mdalloc(md, &d_num, (size_t)na, nm.data())) ||
+        (st = mdalloc(md, &md->m.list, (size_t)na, lst.data())) || (st = mdalloc(md, &md->m.avchrg, (size_t)na)) ||
+        (st = mdalloc(md, &md->m.active, (size_t)ntraj, act.data()))) {
+        delete md;
+        return nullptr;
+    }
+    md->m.velof = d_velof;
+    md->m.mass = d_mass;
+    md->m.imass = d_imass;
+    md->m.num = d_num;
+    b->d.active = md->m.active;
+    if (hipMemcpyAsync(b->d.xyz, xyz, sizeof(double) * 3 * na, hipMemcpyHostToDevice, g_stream) != hipSuccess ||
+        hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * ntraj, hipMemcpyHostToDevice, g_stream) !=
+            hipSuccess) {
+        g_err = "upload failed";
+        delete md;
+        return nullptr;
+    }
+    if (enqueue_egrad(b, true) != QCXMS_OK) {
+        delete md;
+        return nullptr;
+    }
+    hipLaunchKernelGGL(qx::k_md_init, dim3(ntraj), dim3(64), 0, g_stream, b->d, md->m, md->p);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g_stream) != hipSuccess) {
+        g_err = "initial energy/gradient failed on the device";
+        delete md;
+        return nullptr;
+    }
+    if (b->profiling) collect_times();
+    return md;
+}
+
+void qcxms_md_destroy(qcxms_md* md)
+{
+    if (!md) return;
+    (void)hipStreamSynchronize(g_stream);
+    delete md;
+}
+
+qcxms_batch* qcxms_md_batch(qcxms_md* md) { return md ? md->b : nullptr; }
+
+int qcxms_md_run(qcxms_md* md, int32_t nsteps, int32_t* running)
+{
+    if (!md) return fail("null ensemble");
+    qcxms_batch* b = md->b;
+    int st;
+    if (md->use_graph && !b->profiling) {
+        if (!md->graph_ready) {
+            HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+            st = enqueue_md_step(md);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(g_stream, &g);
+            if (st) return st;
+            if (ec != hipSuccess) return fail(std::string("graph capture failed: ") + hipGetErrorString(ec));
+            md->graph = g;
+            HIPCHK(hipGraphInstantiate(&md->exec, md->graph, nullptr, nullptr, 0));
+            md->graph_ready = true;
+        }
+        for (int k = 0; k < nsteps; ++k) HIPCHK(hipGraphLaunch(md->exec, g_stream));
+    } else {
+        for (int k = 0; k < nsteps; ++k)
+            if ((st = enqueue_md_step(md))) return st;
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (b->profiling) collect_times();
+    if (running) {
+        std::vector<int> act(md->ntraj);
+        HIPCHK(hipMemcpy(act.data(), md->m.active, sizeof(int) * md->ntraj, hipMemcpyDeviceToHost));
+        int n = 0;
+        for (int v : act) n += v;
+        *running = n;
+    }
+    return QCXMS_OK;
+}
+
+int qcxms_md_read(qcxms_md* md, double* xyz, double* velo, double* grad, double* qat, int32_t* list,
+                  qcxms_md_result* results)
+{
+    if (!md) return fail("null ensemble");
+    qcxms_batch* b = md->b;
+    const size_t na = b->natoms;
+    if (xyz) HIPCHK(hipMemcpyAsync(xyz, b->d.xyz, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
+    if (velo) HIPCHK(hipMemcpyAsync(velo, md->m.velo, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
+    if (grad) HIPCHK(hipMemcpyAsync(grad, b->d.grad, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
+    if (qat) HIPCHK(hipMemcpyAsync(qat, b->d.qat, sizeof(double) * na, hipMemcpyDeviceToHost, g_stream));
+    if (list) HIPCHK(hipMemcpyAsync(list, md->m.list, sizeof(int) * na, hipMemcpyDeviceToHost, g_stream));
+    std::vector<qx::MdTraj> tr(md->ntraj);
+    std::vector<int> it(md->ntraj), act(md->ntraj);
+    HIPCHK(hipMemcpyAsync(tr.data(), md->m.traj, sizeof(qx::MdTraj) * md->ntraj, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(it.data(), b->d.iters, sizeof(int) * md->ntraj, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(act.data(), md->m.active, sizeof(int) * md->ntraj, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (results)
+        for (int s = 0; s < md->ntraj; ++s) {
+            const qx::MdTraj& t = tr[s];
+            qcxms_md_result& r = results[s];
+            std::memset(&r, 0, sizeof(r));
+            r.nstep = t.nstep;
+            r.nfrag = t.nfrag;
+            r.fragstate = t.fragstate;
+            r.mdok = t.mdok;
+            r.exit_reason = t.exit_reason;
+            r.running = act[s];
+            r.scc_iterations = it[s];
+            r.tav = t.nstep ? t.tav / t.nstep : 0.0;
+            r.epav = t.nstep ? t.epav / t.nstep : 0.0;
+            r.ekav = t.nstep ? t.ekav / t.nstep : 0.0;
+            r.ttime = t.ttime;
+            r.dtime = t.dtime;
+            r.atlast = t.kdump ? t.atlast / t.kdump : 0.0;
+            r.epot = t.epot;
+            r.ekin = t.ekin;
+            r.eerror = t.eerror;
+            r.etemp = t.etemp;
+            for (int f = 0; f < 10; ++f) {
+                r.fragm[f] = t.fragm[f];
+                r.fragT[f] = t.fragT[f];
+            }
+        }
+    return QCXMS_OK;
+}
+
+int qcxms_fragments_batch(int32_t nsys, const int32_t* offsets, const int32_t* num, const double* xyz,
+                          const double* mass, int32_t* list, int32_t* nfrag, double* fragm)
+{
+    g_err.clear();
+    int st = ensure_init(0);
+    if (st) return st;
+    if ((st = check_offsets(nsys, offsets, QX_MAX_NAT))) return st;
+    const size_t na = offsets[nsys];
+    for (size_t i = 0; i < na; ++i)
+        if (num[i] < 1 || num[i] > 118) return fail("atomic number outside 1..118");
+    TmpBuf t;
+    int *d_off, *d_num, *d_list, *d_nf;
+    double *d_xyz, *d_mass, *d_fm;
+    if ((st = t.up(&d_off, offsets, nsys + 1)) || (st = t.up(&d_num, num, na)) || (st = t.up(&d_xyz, xyz, 3 * na)) ||
+        (st = t.up(&d_mass, mass, na)) || (st = t.up(&d_list, (const int*)nullptr, na)) ||
+        (st = t.up(&d_nf, (const int*)nullptr, nsys)) || (st = t.up(&d_fm, (const double*)nullptr, 10 * (size_t)nsys)))
+        return st;
+    hipLaunchKernelGGL(qx::k_fragments_batch, dim3(nsys), dim3(64), 0, g_stream, d_off, d_num, d_xyz, d_mass, d_list,
+                       d_nf, d_fm);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(list, d_list, sizeof(int) * na, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(nfrag, d_nf, sizeof(int) * nsys, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(fragm, d_fm, sizeof(double) * 10 * nsys, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return QCXMS_OK;
+}
+
+int qcxms_leapfrog_batch(int32_t nsys, const int32_t* offsets, const double* grad, const double* mass, double tstep,
+                         double* xyz, double* velo, double* ekin)
+{
+    g_err.clear();
+    int st = ensure_init(0);
+    if (st) return st;
+    if ((st = check_offsets(nsys, offsets, QX_MAX_NAT))) return st;
+    const size_t na = offsets[nsys];
+    TmpBuf t;
+    int* d_off;
+    double *d_g, *d_m, *d_x, *d_v, *d_e;
+    if ((st = t.up(&d_off, offsets, nsys + 1)) || (st = t.up(&d_g, grad, 3 * na)) || (st = t.up(&d_m, mass, na)) ||
+        (st = t.up(&d_x, xyz, 3 * na)) || (st = t.up(&d_v, velo, 3 * na)) ||
+        (st = t.up(&d_e, (const double*)nullptr, nsys)))
+        return st;
+    hipLaunchKernelGGL(qx::k_leapfrog_batch, dim3(nsys), dim3(64), 0, g_stream, d_off, d_g, d_m, tstep, d_x, d_v, d_e);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(xyz, d_x, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(velo, d_v, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(ekin, d_e, sizeof(double) * nsys, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return QCXMS_OK;
+}
+
+int qcxms_impactscale_batch(int32_t nsys, const int32_t* offsets, double* velo, const double* mass,
+                            const double* velof, const double* eimp, const double* ff, const double* e0,
+                            int32_t* iterations)
+{
+    g_err.clear();
+    int st = ensure_init(0);
+    if (st) return st;
+    if ((st = check_offsets(nsys, offsets, QX_MAX_NAT))) return st;
+    const size_t na = offsets[nsys];
+    TmpBuf t;
+    int *d_off, *d_it;
+    double *d_v, *d_m, *d_f, *d_ei, *d_ff, *d_e0;
+    if ((st = t.up(&d_off, offsets, nsys + 1)) || (st = t.up(&d_v, velo, 3 * na)) || (st = t.up(&d_m, mass, na)) ||
+        (st = t.up(&d_f, velof, na)) || (st = t.up(&d_ei, eimp, nsys)) || (st = t.up(&d_ff, ff, nsys)) ||
+        (st = t.up(&d_e0, e0, nsys)) || (st = t.up(&d_it, (const int*)nullptr, nsys)))
+        return st;
+    hipLaunchKernelGGL(qx::k_impactscale_batch, dim3(nsys), dim3(64), 0, g_stream, d_off, d_v, d_m, d_f, d_ei, d_ff,
+                       d_e0, d_it);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(velo, d_v, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(iterations, d_it, sizeof(int) * nsys, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return QCXMS_OK;
 }
 
 }  // extern "C"

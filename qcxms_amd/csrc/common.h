@@ -36,6 +36,7 @@ struct DevTables {
 };
 
 extern __constant__ DevTables g_tab;
+extern __constant__ double g_rad[118];  // QCxMS bond radii (covalent_radii.f90:89-114), Bohr
 
 struct SysInfo {
     int nat, nsh, nao, ld, ndim, nx;

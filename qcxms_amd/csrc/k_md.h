@@ -1,3 +1,455 @@
-// MD kernels (leap-frog, heating, connectivity, bookkeeping): see below.
+// MD step kernels of the EI production loop (src/md.f90:285-682, it > 0,
+// method < 3, icoll = 0), one 64-lane workgroup per trajectory.
+//
+//   k_md_pre : step counters, temperature and energy averages, drift/SCF
+//              error exits, charge averaging, leap-frog (md.f90:287-394)
+//   (egrad   : k_setup -> k_scc -> k_grad on the trajectories still running)
+//   k_md_post: checkqc, heating by impactscale, setetemp, fragment
+//              connectivity, fragment masses/temperatures, fragmentation
+//              exits (md.f90:396-679)
+//
+// Floating-point bookkeeping reproduces the Fortran operation order exactly:
+// FMA contraction is off from here on, sums that the reference accumulates
+// sequentially are accumulated sequentially by lane 0, and the reference's
+// default-real (REAL(4)) literals are float constants widened to double.
 #pragma once
 #include "k_grad.h"
+
+#pragma clang fp contract(off)
+
+namespace qx {
+
+constexpr double kKB = 3.166808578545117e-06;   // xtb_mctc_constants.f90:35
+constexpr double kFsToAu = 41.3413733365614;    // xtb_mctc_convert.f90:57
+
+enum MdExit { kRunning = 0, kErrScf = 1, kErrDrift = 2, kNfrag6 = 3, kNfragExit = 4,
+              kFconst = 5, kMorestep = 6, kMaxStep = 7, kHeatFail = 8 };
+
+struct MdTraj {
+    int nstep, nfrag, fconst, morestep, kdump, nadd, done, mdok, fragstate, exit_reason;
+    double ekin, epot, tav, epav, ekav, edum, eerror, ttime, dtime, atlast, ekinstart, etemp,
+        eimp, fadd, tadd;
+    double fragm[10], fragT[10];
+};
+
+struct MdParams {
+    double tstep, etempin, ax, ieetemp;
+    int nfragexit, max_steps;
+};
+
+struct MdPtrs {
+    MdTraj* traj;
+    double* velo;    // 3 per atom
+    const double* velof;
+    const double* mass;
+    const int* imass;
+    const int* num;  // atomic numbers
+    int* list;       // fragment labels per atom
+    double* avchrg;  // running charge sums (md.f90:355-361)
+    int* active;     // per trajectory: 1 while running
+};
+
+// utility.f90:69-86
+QX_DEV double setetemp(int nfrag, double eimp, double ax, double ieetemp)
+{
+    double etemp = (double)5000.0f + (double)20000.0f * ax;
+    if (eimp > 0 && nfrag <= 1) {
+        const double tmp = eimp > 0.0 ? eimp : 0.0;
+        etemp = etemp + tmp * ieetemp;
+    }
+    return etemp;
+}
+
+// iniqm.f90:684-732 (gnorm counts grad(2,i) twice and never z)
+QX_DEV int checkqc(int nat, double& e, const double* g, const double* q, int mchrg)
+{
+    if (fabs(e) < 1.0e-8) return 0;
+    double gn = 0.0;
+    for (int i = 0; i < nat; ++i)
+        gn = gn + g[3 * i] * g[3 * i] + g[3 * i + 1] * g[3 * i + 1] + g[3 * i + 1] * g[3 * i + 1];
+    gn = sqrt(gn);
+    if (gn < 1.0e-8 || gn > (double)20.0f) {
+        e = 0.0;
+        return 0;
+    }
+    if (mchrg > 0) {
+        double mx = -1.0e300;
+        for (int i = 0; i < nat; ++i) mx = fmax(mx, q[i]);
+        if (fabs(mx) < 1.0e-5) {
+            e = 0.0;
+            return 0;
+        }
+    }
+    return 1;
+}
+
+// sequential sum of per-atom kinetic terms m (vx^2 + vy^2 + vz^2), ekinet order
+QX_DEV double ekin_terms_sum(const double* t, int nat)
+{
+    double e = 0.0;
+    for (int i = 0; i < nat; ++i) e = e + t[i];
+    return e;
+}
+
+
+// leap-frog for one system by a 64-lane block; returns the kinetic energy of
+// the averaged velocities, accumulated atom-major as md.f90:760-768
+QX_DEV double dev_leapfrog(int nat, const double* mass, double dt, const double* g, double* x,
+                           double* v, double* term)
+{
+    const int lane = threadIdx.x;
+    for (int i = lane; i < nat; i += 64) {
+        const double m = mass[i];
+        for (int j = 0; j < 3; ++j) {
+            const double vold = v[3 * i + j];
+            v[3 * i + j] = v[3 * i + j] - (dt * g[3 * i + j] / m);
+            const double vavg = 0.5 * (vold + v[3 * i + j]);
+            x[3 * i + j] = x[3 * i + j] + (dt * v[3 * i + j]);
+            term[3 * i + j] = 0.5 * (m * vavg * vavg);
+        }
+    }
+    __syncthreads();
+    __shared__ double s_ke;
+    if (lane == 0) {
+        double ke = 0.0;
+        for (int k = 0; k < 3 * nat; ++k) ke = ke + term[k];
+        s_ke = ke;
+    }
+    __syncthreads();
+    return s_ke;
+}
+
+// impactscale (impact.f90:12-55) by a 64-lane block; returns the iteration
+// count k (k >= 20000 is the reference's stop)
+QX_DEV int dev_impactscale(int nat, double* velo, const double* mass, const double* velof,
+                           double eimp, double ff, double e0, double* term)
+{
+    const int lane = threadIdx.x;
+    __shared__ int s_go, s_k;
+    __shared__ double s_scal;
+    const double esoll = eimp * ff + e0;
+    double scal = 0.0;
+    int k = 0;  // advanced by lane 0 only
+    for (;;) {
+        for (int i = lane; i < nat; i += 64) {
+            const double* v = velo + 3 * i;
+            const double f = 1.0 + velof[i] * scal;
+            const double v0 = v[0] * f, v1 = v[1] * f, v2 = v[2] * f;
+            term[i] = mass[i] * (v0 * v0 + v1 * v1 + v2 * v2);
+        }
+        __syncthreads();
+        if (lane == 0) {
+            k = k + 1;
+            const double e = ekin_terms_sum(term, nat) * 0.5;
+            s_scal = scal + (double)0.0002f;
+            s_k = k;
+            s_go = (esoll - e > (double)0.001f && k < 20000);
+        }
+        __syncthreads();
+        scal = s_scal;
+        if (!s_go) break;
+    }
+    for (int i = lane; i < nat; i += 64) {
+        double* v = velo + 3 * i;
+        const double f = 1.0 + velof[i] * scal;
+        v[0] = v[0] * f;
+        v[1] = v[1] * f;
+        v[2] = v[2] * f;
+    }
+    __syncthreads();
+    return s_k;
+}
+
+// fragment_structure(rcut = 3, at1 = 1, at2 = 0) + fragmass masses by a
+// 64-lane block (nat <= 64).  Labels are the connected components numbered by
+// their lowest atom index, which is what the reference's flood fill yields.
+// Returns nfrag; fragm[0..9] receives the masses (amu) of the non-empty labels.
+QX_DEV int dev_fragments(int nat, const int* num, const double* x, const double* mass, int* list,
+                         double* fragm, unsigned long long* adj)
+{
+    const int lane = threadIdx.x;
+    for (int i = lane; i < nat; i += 64) {
+        unsigned long long row = 0ull;
+        const double ri = g_rad[num[i] - 1];
+        for (int j = 0; j < nat; ++j) {
+            if (j == i) continue;
+            const int a = i < j ? i : j, b = i < j ? j : i;
+            const double dx = x[3 * a] - x[3 * b], dy = x[3 * a + 1] - x[3 * b + 1],
+                         dz = x[3 * a + 2] - x[3 * b + 2];
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double rcov = 3.0 * 0.5 * (ri + g_rad[num[j] - 1]);
+            if (r < rcov) row |= (1ull << j);
+        }
+        adj[i] = row;
+    }
+    __syncthreads();
+    __shared__ int s_nfrag;
+    if (lane == 0) {
+        unsigned long long unassigned = (nat == 64) ? ~0ull : ((1ull << nat) - 1ull);
+        int label = 0;
+        while (unassigned) {
+            ++label;
+            unsigned long long comp = unassigned & (~unassigned + 1ull);
+            for (;;) {
+                unsigned long long grow = comp, c = comp;
+                while (c) {
+                    const int i = __ffsll((long long)c) - 1;
+                    c &= c - 1ull;
+                    grow |= adj[i];
+                }
+                grow &= unassigned;
+                if (grow == comp) break;
+                comp = grow;
+            }
+            unsigned long long c = comp;
+            while (c) {
+                const int i = __ffsll((long long)c) - 1;
+                c &= c - 1ull;
+                list[i] = label;
+            }
+            unassigned &= ~comp;
+        }
+        double fm[10];
+        for (int f = 0; f < 10; ++f) fm[f] = 0.0;
+        for (int i = 0; i < nat; ++i) {
+            const int l = list[i];
+            if (l >= 1 && l <= 10) fm[l - 1] = fm[l - 1] + mass[i];
+        }
+        const double autoamu = (1.0 / 1.660539040e-27) * 9.10938356e-31;
+        int nf = 0;
+        for (int f = 0; f < 10; ++f) fragm[f] = 0.0;
+        for (int f = 0; f < 10; ++f)
+            if (fm[f] > 0) {
+                fragm[nf] = fm[f] * autoamu;
+                ++nf;
+            }
+        s_nfrag = nf;
+    }
+    __syncthreads();
+    return s_nfrag;
+}
+
+__global__ __launch_bounds__(64) void k_md_init(BatchPtrs B, MdPtrs M, MdParams p)
+{
+    const int sys = blockIdx.x;
+    const SysInfo si = B.info[sys];
+    const int lane = threadIdx.x, nat = si.nat, off = si.atom_off;
+    __shared__ double term[QX_MAX_NAT];
+    for (int i = lane; i < nat; i += 64) {
+        const double* v = M.velo + 3 * (size_t)(off + i);
+        term[i] = M.mass[off + i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    MdTraj& t = M.traj[sys];
+    const double ek = ekin_terms_sum(term, nat) * 0.5;  // mdinit.f90:57-76
+    t.ekin = ek;
+    t.ekinstart = ek;
+    t.nadd = (int)((t.tadd + p.tstep) / p.tstep - 1);  // md.f90:265
+    t.fadd = p.tstep / (t.tadd + p.tstep);
+    t.nfrag = 1;
+    t.kdump = 50;
+    double e = B.energy[sys];
+    checkqc(nat, e, B.grad + 3 * (size_t)off, B.qat + off, si.charge);
+    t.epot = e;
+    if (e == 0.0) {
+        t.done = 1;
+        t.exit_reason = kErrScf;
+        M.active[sys] = 0;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_md_pre(BatchPtrs B, MdPtrs M, MdParams p)
+{
+    const int sys = blockIdx.x;
+    if (!M.active[sys]) return;
+    const SysInfo si = B.info[sys];
+    const int lane = threadIdx.x, nat = si.nat, off = si.atom_off;
+    MdTraj& t = M.traj[sys];
+    __shared__ double term[3 * QX_MAX_NAT];
+    __shared__ int s_exit;
+    __shared__ double s_T;
+    if (lane == 0) {
+        int nstep = t.nstep + 1;
+        t.nstep = nstep;
+        const double T = t.ekin / ((double)(1.5f * (float)nat) * kKB);
+        t.tav = t.tav + T;
+        t.epav = t.epav + t.epot;
+        t.ekav = t.ekav + t.ekin;
+        double eav;
+        if (nstep > t.nadd) {
+            t.edum = t.edum + t.epot + t.ekin;
+            eav = t.edum / (double)(float)(nstep - t.nadd);
+        } else {
+            eav = t.epot + t.ekin;
+        }
+        t.eerror = eav - t.epot - t.ekin;
+        if (t.nfrag == 1) t.fragT[0] = t.tav / nstep;
+        const int err1 = (t.epot == 0.0), err2 = (fabs(t.eerror) > (double)0.1f);
+        s_exit = 0;
+        if (err1 || err2) {
+            t.mdok = (t.nfrag > 1 && t.nfrag <= 4);
+            t.exit_reason = err1 ? kErrScf : kErrDrift;
+            t.done = 1;
+            M.active[sys] = 0;
+            s_exit = 1;
+        } else if (t.kdump > 49) {
+            t.kdump = 0;
+            t.atlast = 0.0;
+        }
+        s_T = T;
+    }
+    __syncthreads();
+    if (s_exit) return;
+    const bool reset = (t.kdump == 0);
+    for (int i = lane; i < nat; i += 64) {
+        const double prev = reset ? 0.0 : M.avchrg[off + i];
+        M.avchrg[off + i] = prev + B.qat[off + i];
+    }
+    // leap-frog (md.f90:749-773)
+    const double ke = dev_leapfrog(nat, M.mass + off, p.tstep, B.grad + 3 * (size_t)off,
+                                   B.xyz + 3 * (size_t)off, M.velo + 3 * (size_t)off, term);
+    if (lane == 0) {
+        t.atlast = t.atlast + s_T;
+        t.ekin = ke;
+        t.ttime = t.ttime + p.tstep / kFsToAu;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_md_post(BatchPtrs B, MdPtrs M, MdParams p)
+{
+    const int sys = blockIdx.x;
+    if (!M.active[sys]) return;
+    const SysInfo si = B.info[sys];
+    const int lane = threadIdx.x, nat = si.nat, off = si.atom_off;
+    MdTraj& t = M.traj[sys];
+    __shared__ double term[QX_MAX_NAT];
+    __shared__ unsigned long long adj[QX_MAX_NAT];
+    __shared__ int s_more;
+    if (lane == 0) {
+        double e = B.energy[sys];
+        checkqc(nat, e, B.grad + 3 * (size_t)off, B.qat + off, si.charge);
+        t.epot = e;
+        t.kdump = t.kdump + 1;
+        if (t.nfrag == 1) t.morestep = 0;
+        if (t.nfrag > 1 && t.dtime < 1.0e-6) t.dtime = t.ttime / (double)1000.0f;
+        s_more = (t.nstep <= t.nadd && t.nfrag == 1);
+    }
+    __syncthreads();
+    // heating: impactscale (impact.f90:12-55) with ff = fadd * nstep
+    if (s_more) {
+        const int k = dev_impactscale(nat, M.velo + 3 * (size_t)off, M.mass + off, M.velof + off,
+                                      t.eimp, t.fadd * t.nstep, t.ekinstart, term);
+        if (k >= 20000) {
+            if (lane == 0) {
+                t.done = 1;
+                t.exit_reason = kHeatFail;
+                M.active[sys] = 0;
+            }
+            return;
+        }
+    }
+    // connectivity (fragments.f90:93-182) and fragment masses (fragments.f90:10-84)
+    double fragm[10];
+    const int nfrag = dev_fragments(nat, M.num + off, B.xyz + 3 * (size_t)off, M.mass + off,
+                                    M.list + off, fragm, adj);
+    if (lane != 0) return;
+    for (int f = 0; f < 10; ++f) t.fragm[f] = fragm[f];
+    t.nfrag = nfrag;
+    // electronic temperature for the next egrad (md.f90:443-445)
+    const double dum = t.eimp - t.eimp * (double)(float)t.nstep / (double)(float)t.nadd;
+    t.etemp = setetemp(t.nfrag, dum, p.ax, p.ieetemp);
+    ((SysInfo*)B.info)[sys].kt = t.etemp * gfn2::kKtoAu;
+    // fragment temperatures (md.f90:715-741)
+    if (nfrag > 1) {
+        double eint[10];
+        int cnt[10];
+        for (int f = 0; f < 10; ++f) {
+            eint[f] = 0.0;
+            cnt[f] = 0;
+        }
+        for (int i = 0; i < nat; ++i) {
+            const int j = M.list[off + i];
+            if (j < 1 || j > 10) continue;
+            const double* v = M.velo + 3 * (size_t)(off + i);
+            eint[j - 1] = eint[j - 1] + 0.5 * M.mass[off + i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            cnt[j - 1] = cnt[j - 1] + 1;
+        }
+        for (int f = 0; f < nfrag && f < 10; ++f)
+            t.fragT[f] = eint[f] / ((double)(1.5f * (float)cnt[f]) * kKB);
+    }
+    // exits (md.f90:458-462, 630-679)
+    int reason = kRunning;
+    if (nfrag > 6) {
+        reason = kNfrag6;
+    } else if (nfrag > p.nfragexit) {
+        t.fragstate = 1;
+        t.mdok = 1;
+        reason = kNfragExit;
+    } else {
+        if (nfrag >= 2) t.fconst = t.fconst + 1;
+        else t.fconst = 0;
+        if (t.fconst > 1000) {
+            t.fragstate = 2;
+            t.mdok = 1;
+            reason = kFconst;
+        } else if (nfrag >= p.nfragexit && (t.morestep = t.morestep + 1) > 250) {
+            t.fragstate = 1;
+            t.mdok = 1;
+            reason = kMorestep;
+        } else if (t.nstep >= p.max_steps) {
+            t.fragstate = 1;
+            t.mdok = 1;
+            reason = kMaxStep;
+        }
+    }
+    if (reason != kRunning) {
+        t.exit_reason = reason;
+        t.done = 1;
+        M.active[sys] = 0;
+    }
+}
+
+}  // namespace qx
+
+namespace qx {
+
+// ---- standalone batched drop-ins (one 64-lane block per system) ----
+__global__ __launch_bounds__(64) void k_fragments_batch(const int* offsets, const int* num,
+                                                        const double* xyz, const double* mass,
+                                                        int* list, int* nfrag, double* fragm)
+{
+    const int sys = blockIdx.x, off = offsets[sys], nat = offsets[sys + 1] - off;
+    __shared__ unsigned long long adj[QX_MAX_NAT];
+    double fm[10];
+    const int nf = dev_fragments(nat, num + off, xyz + 3 * (size_t)off, mass + off, list + off, fm, adj);
+    if (threadIdx.x == 0) {
+        nfrag[sys] = nf;
+        for (int f = 0; f < 10; ++f) fragm[10 * sys + f] = fm[f];
+    }
+}
+
+__global__ __launch_bounds__(64) void k_leapfrog_batch(const int* offsets, const double* grad,
+                                                       const double* mass, double tstep, double* xyz,
+                                                       double* velo, double* ekin)
+{
+    const int sys = blockIdx.x, off = offsets[sys], nat = offsets[sys + 1] - off;
+    __shared__ double term[3 * QX_MAX_NAT];
+    const double ke = dev_leapfrog(nat, mass + off, tstep, grad + 3 * (size_t)off, xyz + 3 * (size_t)off,
+                                   velo + 3 * (size_t)off, term);
+    if (threadIdx.x == 0) ekin[sys] = ke;
+}
+
+__global__ __launch_bounds__(64) void k_impactscale_batch(const int* offsets, double* velo,
+                                                          const double* mass, const double* velof,
+                                                          const double* eimp, const double* ff,
+                                                          const double* e0, int* iters)
+{
+    const int sys = blockIdx.x, off = offsets[sys], nat = offsets[sys + 1] - off;
+    __shared__ double term[QX_MAX_NAT];
+    const int k = dev_impactscale(nat, velo + 3 * (size_t)off, mass + off, velof + off, eimp[sys], ff[sys],
+                                  e0[sys], term);
+    if (threadIdx.x == 0) iters[sys] = k;
+}
+
+}  // namespace qx

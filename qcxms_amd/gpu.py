@@ -164,3 +164,167 @@ def egrad_batch(systems, xyz_list, etemp=300.0, method=GFN2_XTB):
     off = b.offsets
     return (e, [g[off[i]:off[i + 1]] for i in range(len(systems))],
             [q[off[i]:off[i + 1]] for i in range(len(systems))], st)
+
+
+# ---------------------------------------------------------------- MD ensemble
+class MdParams(C.Structure):
+    _fields_ = [("tstep", C.c_double), ("etempin", C.c_double), ("ax", C.c_double),
+                ("ieetemp", C.c_double), ("nfragexit", C.c_int32), ("max_steps", C.c_int32),
+                ("use_graph", C.c_int32), ("pad", C.c_int32), ("econv", C.c_double),
+                ("pconv", C.c_double)]
+
+
+class MdResult(C.Structure):
+    _fields_ = ([(n, C.c_int32) for n in ("nstep", "nfrag", "fragstate", "mdok", "exit_reason",
+                                          "running", "scc_iterations", "pad")]
+                + [(n, C.c_double) for n in ("tav", "epav", "ekav", "ttime", "dtime", "atlast",
+                                             "epot", "ekin", "eerror", "etemp")]
+                + [("fragm", C.c_double * 10), ("fragT", C.c_double * 10)])
+
+
+EXIT_REASONS = {0: "running", 1: "scf/checkqc failure", 2: "energy drift", 3: "more than 6 fragments",
+                4: "nfrag > nfragexit", 5: "nothing happens", 6: "nfrag = nfragexit, more steps done",
+                7: "max steps", 8: "impactscale failed"}
+
+
+def _md_sigs(L):
+    if getattr(L, "_md_sigs", False):
+        return
+    L.qcxms_md_create.restype = C.c_void_p
+    L.qcxms_md_create.argtypes = [C.c_int32, _ip, _ip, _ip, _dp, _ip, _dp, _dp, _dp, _dp, _dp,
+                                  C.POINTER(MdParams)]
+    L.qcxms_md_destroy.argtypes = [C.c_void_p]
+    L.qcxms_md_run.argtypes = [C.c_void_p, C.c_int32, _ip]
+    L.qcxms_md_read.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _ip, C.POINTER(MdResult)]
+    L.qcxms_md_batch.restype = C.c_void_p
+    L.qcxms_md_batch.argtypes = [C.c_void_p]
+    L.qcxms_fragments_batch.argtypes = [C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
+    L.qcxms_leapfrog_batch.argtypes = [C.c_int32, _ip, _dp, _dp, C.c_double, _dp, _dp, _dp]
+    L.qcxms_impactscale_batch.argtypes = [C.c_int32, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+    L._md_sigs = True
+
+
+class MdEnsemble:
+    """Batched EI production MD (src/md.f90:285-682) on the GPU.
+
+    trajectories: list of dicts with num, xyz [nat,3] (Bohr), velo [nat,3]
+    (au), mass [nat] (m_e), eimp (Eh), tadd (au); optional charge, imass,
+    velof."""
+
+    def __init__(self, trajectories, tstep=0.5 * 41.3413733365614, max_steps=2000,
+                 nfragexit=3, etempin=-1.0, ax=0.0, ieetemp=0.0, use_graph=True,
+                 econv=0.0, pconv=0.0):
+        L = library()
+        _md_sigs(L)
+        nums = [np.asarray(t["num"], dtype=np.int32) for t in trajectories]
+        self.offsets = np.cumsum([0] + [len(n) for n in nums]).astype(np.int32)
+        self.ntraj = len(trajectories)
+        self.natoms = int(self.offsets[-1])
+        cat = lambda key, dt, default=None: np.ascontiguousarray(np.concatenate(
+            [np.asarray(t.get(key, default(t) if default else None), dtype=dt).reshape(-1)
+             for t in trajectories]))
+        self.num = np.concatenate(nums).astype(np.int32)
+        charge = np.array([int(t.get("charge", 1)) for t in trajectories], dtype=np.int32)
+        mass = cat("mass", np.float64)
+        imass = cat("imass", np.int32, lambda t: np.full(len(t["num"]), -1))
+        xyz = cat("xyz", np.float64)
+        velo = cat("velo", np.float64)
+        velof = cat("velof", np.float64, lambda t: np.ones(len(t["num"])))
+        eimp = np.array([float(t["eimp"]) for t in trajectories])
+        tadd = np.array([float(t["tadd"]) for t in trajectories])
+        p = MdParams(tstep, etempin, ax, ieetemp, nfragexit, max_steps, int(bool(use_graph)), 0,
+                     econv, pconv)
+        h = L.qcxms_md_create(self.ntraj, _p(self.offsets, _ip), _p(self.num, _ip), _p(charge, _ip),
+                              _p(mass), _p(imass, _ip), _p(xyz), _p(velo), _p(velof), _p(eimp),
+                              _p(tadd), C.byref(p))
+        if not h:
+            raise RuntimeError("qcxms_md_create failed: " + last_error())
+        self._h = C.c_void_p(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().qcxms_md_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps):
+        running = C.c_int32()
+        if library().qcxms_md_run(self._h, int(nsteps), C.byref(running)):
+            raise RuntimeError("qcxms_md_run failed: " + last_error())
+        return running.value
+
+    def read(self):
+        xyz = np.zeros((self.natoms, 3)); velo = np.zeros((self.natoms, 3))
+        grad = np.zeros((self.natoms, 3)); qat = np.zeros(self.natoms)
+        lst = np.zeros(self.natoms, dtype=np.int32)
+        res = (MdResult * self.ntraj)()
+        if library().qcxms_md_read(self._h, _p(xyz), _p(velo), _p(grad), _p(qat), _p(lst, _ip), res):
+            raise RuntimeError("qcxms_md_read failed: " + last_error())
+        return dict(xyz=xyz, velo=velo, grad=grad, qat=qat, list=lst, results=list(res))
+
+    def batch_handle(self):
+        return library().qcxms_md_batch(self._h)
+
+    def set_profiling(self, on):
+        library().qcxms_batch_set_profiling(C.c_void_p(self.batch_handle()), int(bool(on)))
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        n = C.c_int32()
+        library().qcxms_batch_kernel_time(C.c_void_p(self.batch_handle()), name.encode(),
+                                          C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+
+def fragments_batch(nums, xyzs, masses):
+    """fragment_structure + fragmass masses for each system (<= 64 atoms)."""
+    L = library()
+    _md_sigs(L)
+    off = np.cumsum([0] + [len(n) for n in nums]).astype(np.int32)
+    num = np.ascontiguousarray(np.concatenate(nums), dtype=np.int32)
+    xyz = np.ascontiguousarray(np.concatenate(xyzs), dtype=np.float64)
+    mass = np.ascontiguousarray(np.concatenate(masses), dtype=np.float64)
+    lst = np.zeros(len(num), dtype=np.int32)
+    nf = np.zeros(len(nums), dtype=np.int32)
+    fm = np.zeros((len(nums), 10))
+    if L.qcxms_fragments_batch(len(nums), _p(off, _ip), _p(num, _ip), _p(xyz), _p(mass),
+                               _p(lst, _ip), _p(nf, _ip), _p(fm)):
+        raise RuntimeError(last_error())
+    return [lst[off[i]:off[i + 1]] for i in range(len(nums))], nf, fm
+
+
+def leapfrog_batch(grads, masses, tstep, xyzs, velos):
+    L = library()
+    _md_sigs(L)
+    off = np.cumsum([0] + [len(m) for m in masses]).astype(np.int32)
+    g = np.ascontiguousarray(np.concatenate(grads), dtype=np.float64)
+    m = np.ascontiguousarray(np.concatenate(masses), dtype=np.float64)
+    x = np.ascontiguousarray(np.concatenate(xyzs), dtype=np.float64)
+    v = np.ascontiguousarray(np.concatenate(velos), dtype=np.float64)
+    ek = np.zeros(len(masses))
+    if L.qcxms_leapfrog_batch(len(masses), _p(off, _ip), _p(g), _p(m), float(tstep), _p(x), _p(v),
+                              _p(ek)):
+        raise RuntimeError(last_error())
+    return x, v, ek
+
+
+def impactscale_batch(velos, masses, velofs, eimp, ff, e0):
+    L = library()
+    _md_sigs(L)
+    off = np.cumsum([0] + [len(m) for m in masses]).astype(np.int32)
+    v = np.ascontiguousarray(np.concatenate(velos), dtype=np.float64)
+    m = np.ascontiguousarray(np.concatenate(masses), dtype=np.float64)
+    f = np.ascontiguousarray(np.concatenate(velofs), dtype=np.float64)
+    ei = np.ascontiguousarray(eimp, dtype=np.float64)
+    ffa = np.ascontiguousarray(ff, dtype=np.float64)
+    e0a = np.ascontiguousarray(e0, dtype=np.float64)
+    it = np.zeros(len(masses), dtype=np.int32)
+    if L.qcxms_impactscale_batch(len(masses), _p(off, _ip), _p(v), _p(m), _p(f), _p(ei), _p(ffa),
+                                 _p(e0a), _p(it, _ip)):
+        raise RuntimeError(last_error())
+    return v, it

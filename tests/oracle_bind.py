@@ -170,3 +170,30 @@ def impactscale(lib, prefix, velo, mass, velof, eimp, ff, e0):
     k = fn(len(mass), ptr(v), ptr(mass), ptr(velof), C.c_double(eimp), C.c_double(ff),
            C.c_double(e0))
     return v, k
+
+
+def md_run(traj, steps_limit, tstep=0.5 * 41.3413733365614, max_steps=2000, nfragexit=3,
+           etempin=-1.0, opts=None, charge=1):
+    """Oracle EI production MD (md.f90:285-682) with GFN2 forces for one
+    trajectory dict (num, xyz, velo, mass, eimp, tadd[, velof, imass])."""
+    L = oracle()
+    num = np.ascontiguousarray(traj["num"], dtype=np.int32)
+    nat = len(num)
+    imass = np.ascontiguousarray(traj.get("imass", np.full(nat, -1)), dtype=np.int32)
+    mass = np.ascontiguousarray(traj["mass"], dtype=np.float64)
+    velof = np.ascontiguousarray(traj.get("velof", np.ones(nat)), dtype=np.float64)
+    xyz = np.array(traj["xyz"], dtype=np.float64, order="C")
+    velo = np.array(traj["velo"], dtype=np.float64, order="C")
+    grad = np.zeros((nat, 3))
+    qat = np.zeros(nat)
+    lst = np.zeros(nat, dtype=np.int32)
+    st = MdState()
+    st.nat, st.chrg, st.nfragexit, st.max_steps = nat, int(charge), nfragexit, max_steps
+    st.tstep, st.tadd, st.eimp = tstep, float(traj["tadd"]), float(traj["eimp"])
+    st.ax, st.ieetemp, st.etempin = 0.0, 0.0, etempin
+    o = opts if opts is not None else default_opts()
+    fn = C.cast(L.orc_gfn2_md_egrad, EGRAD_FN)
+    L.orc_md_run(C.byref(st), ptr(num, ip), ptr(imass, ip), ptr(mass), ptr(velof), ptr(xyz),
+                 ptr(velo), ptr(grad), ptr(qat), ptr(lst, ip), fn, C.cast(C.pointer(o), C.c_void_p),
+                 int(steps_limit))
+    return dict(state=st, xyz=xyz, velo=velo, grad=grad, qat=qat, list=lst)

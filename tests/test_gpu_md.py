@@ -1,0 +1,126 @@
+"""GPU MD bookkeeping: bit-exact against the reference Fortran (golden
+vectors) and the oracle; batched MD trajectories against the oracle MD."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from qcxms_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+TSTEP = 0.5 * W.FSTOAU
+
+
+def q():
+    import qcxms_amd
+    return qcxms_amd
+
+
+def test_fragments_bitexact_against_reference(golden):
+    off = golden["frag_off"]
+    nums, xyzs, masses, idx = [], [], [], []
+    for k in range(len(off) - 1):
+        oz = golden["frag_oz"][off[k]:off[k + 1]].astype(np.int32)
+        if len(oz) > 64:
+            continue
+        nums.append(oz)
+        xyzs.append(golden["frag_xyz"][off[k]:off[k + 1]])
+        masses.append(golden["mass"][oz - 1])
+        idx.append(k)
+    lists, nf, fm = q().fragments_batch(nums, xyzs, masses)
+    for j, k in enumerate(idx):
+        want = golden["frag_out"][off[k]:off[k + 1]]
+        assert np.array_equal(lists[j], want), k
+        if golden["fm_nfrag"][k] >= 0:
+            assert nf[j] == golden["fm_nfrag"][k]
+            assert np.array_equal(fm[j], golden["fm_fragm"][k]), k
+
+
+def test_impactscale_bitexact_against_reference(golden):
+    off = golden["is_off"]
+    n = len(off) - 1
+    velos = [golden["is_velo"][off[k]:off[k + 1]] for k in range(n)]
+    masses = [golden["is_mass"][off[k]:off[k + 1]] for k in range(n)]
+    velofs = [golden["is_velof"][off[k]:off[k + 1]] for k in range(n)]
+    par = golden["is_par"]
+    v, it = q().impactscale_batch(velos, masses, velofs, par[:, 0], par[:, 1], par[:, 2])
+    assert np.all(it < 20000)
+    assert np.array_equal(v, golden["is_out"])
+
+
+def test_leapfrog_bitexact_against_oracle():
+    import ctypes as C
+    L = ob.oracle()
+    rng = np.random.default_rng(42)
+    grads, masses, xyzs, velos, want_x, want_v, want_e = [], [], [], [], [], [], []
+    for s in range(30):
+        nat = int(rng.integers(1, 65))
+        m = W.atomic_mass(6) * rng.uniform(0.08, 1.4, nat)
+        g = rng.normal(0, 0.05, (nat, 3)); x = rng.normal(0, 5, (nat, 3)); v = rng.normal(0, 1e-3, (nat, 3))
+        grads.append(g); masses.append(m); xyzs.append(x); velos.append(v)
+        x2, v2, ke = x.copy(), v.copy(), C.c_double()
+        L.orc_leapfrog(nat, ob.ptr(g), ob.ptr(m), C.c_double(TSTEP), ob.ptr(x2), ob.ptr(v2), C.byref(ke))
+        want_x.append(x2); want_v.append(v2); want_e.append(ke.value)
+    x, v, ek = q().leapfrog_batch(grads, masses, TSTEP, xyzs, velos)
+    assert np.array_equal(x, np.concatenate(want_x))
+    assert np.array_equal(v, np.concatenate(want_v))
+    assert np.array_equal(ek, np.array(want_e))
+
+
+def ensemble(n, seed0=1, tadd_fs=5.0):
+    return W.md_ensemble(n, base_seed=seed0, tadd_fs=tadd_fs)
+
+
+def test_md_ensemble_matches_oracle_md():
+    trajs = ensemble(3)
+    nsteps = 24
+    md = q().MdEnsemble(trajs, max_steps=nsteps, use_graph=False, econv=1e-11, pconv=1e-9)
+    running = md.run(nsteps + 5)
+    out = md.read()
+    assert running == 0
+    opts = ob.default_opts(econv=1e-12, pconv=1e-10)
+    for t, tr in enumerate(trajs):
+        ref = ob.md_run(tr, steps_limit=0, max_steps=nsteps, opts=opts)
+        st, res = ref["state"], out["results"][t]
+        sl = slice(md.offsets[t], md.offsets[t + 1])
+        print(f"traj {t}: nstep {res.nstep}/{st.nstep} exit {res.exit_reason}/{st.exit_reason} "
+              f"dEpot {abs(res.epot - st.epot):.2e} dEkin {abs(res.ekin - st.ekin):.2e} "
+              f"dx {np.abs(out['xyz'][sl] - ref['xyz']).max():.2e}", flush=True)
+        assert res.nstep == st.nstep == nsteps
+        assert res.exit_reason == st.exit_reason == 7
+        assert res.nfrag == st.nfrag
+        assert np.array_equal(out["list"][sl], ref["list"])
+        assert abs(res.epot - st.epot) < 1e-8
+        assert abs(res.ekin - st.ekin) < 1e-9
+        assert abs(res.tav - st.tav) < 1e-4
+        assert np.abs(out["xyz"][sl] - ref["xyz"]).max() < 1e-8
+        assert np.abs(out["velo"][sl] - ref["velo"]).max() < 1e-10
+
+
+def test_md_graph_replay_equals_eager():
+    trajs = ensemble(2, seed0=11)
+    a = q().MdEnsemble(trajs, max_steps=100, use_graph=False)
+    b = q().MdEnsemble(trajs, max_steps=100, use_graph=True)
+    for _ in range(3):
+        a.run(4)
+        b.run(4)
+    ra, rb = a.read(), b.read()
+    assert np.array_equal(ra["xyz"], rb["xyz"])
+    assert np.array_equal(ra["velo"], rb["velo"])
+    assert [r.nstep for r in ra["results"]] == [r.nstep for r in rb["results"]] == [12, 12]
+
+
+def test_md_fragment_exit_matches_oracle():
+    """Two methyl fragments 8 A apart: nfrag = 2 > nfragexit = 1 ends the run
+    after the first step with fragstate 1 and mdok."""
+    n1, x1 = W.build_chain([6], noise=0.0)
+    num = np.concatenate([n1[:4], n1[:4]])
+    xyz = np.concatenate([x1[:4], x1[:4] + np.array([8.0 * W.AATOAU, 0, 0])])
+    mass = np.array([W.atomic_mass(int(z)) for z in num])
+    tr = dict(num=num, xyz=xyz, velo=W.maxwell_boltzmann(mass, 300.0, 5), mass=mass,
+              eimp=0.05, tadd=0.0)
+    md = q().MdEnsemble([tr], max_steps=50, nfragexit=1, use_graph=False)
+    md.run(5)
+    res = md.read()["results"][0]
+    ref = ob.md_run(tr, steps_limit=0, max_steps=50, nfragexit=1)["state"]
+    assert (res.nstep, res.nfrag, res.exit_reason, res.mdok, res.fragstate) == \
+        (ref.nstep, ref.nfrag, ref.exit_reason, ref.mdok, ref.fragstate) == (1, 2, 4, 1, 1)
