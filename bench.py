@@ -1,0 +1,212 @@
+"""Throughput of the batched GFN2-xTB EI production MD on one MI355X.
+
+Workload (the batched configuration the metric is quoted on): 256 independent
+trajectories of the n-octane radical cation C8H18+ (26 atoms, 50 basis
+functions), each with its own seeded geometry noise (0.02 A), Maxwell-Boltzmann
+velocities at 500 K and an internal excess energy drawn from a Poisson-shaped
+distribution (mean 0.6 eV/atom, truncated at 1.5 eV/atom) added by impactscale
+over tadd = 100 fs; FP64 GFN2-xTB forces at etemp 5000 K (setetemp), 0.5 fs
+leap-frog steps, trajectory length 5000 steps.  A step is one pass of the MD
+loop (src/md.f90:285-682) over every running trajectory.
+
+value = trajectory-steps executed / wall time of the timed steps, with all
+state resident in HBM.  The roofline object describes the dominant kernel
+(the SCC kernel) with the analytic FP64 flop count per SCC iteration
+15 n^3 + 30 n^2 (n = basis functions), timed by HIP events on the library's
+stream.  cpu_baseline: the C oracle's restatement of the same MD loop with the
+same forces (not QCxMS+tblite, which cannot be built here), one trajectory per
+host thread as bin/pqcxms runs it, on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "trajectory-MD-steps/sec on 1 GPU (batched vs 1-per-launch) at |dE|≤1e-8 Eh"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def scc_flops(n):
+    """Analytic FP64 flops of one SCC iteration of an n-function system."""
+    return 15.0 * n ** 3 + 30.0 * n ** 2
+
+
+def workload(ntraj):
+    from qcxms_amd import workloads as W
+    return W.md_ensemble(ntraj, base_seed=1, mean_ev=0.6, max_ev=1.5, temp=500.0, noise=0.02,
+                         tadd_fs=100.0)
+
+
+def traffic_from_profiles():
+    """HBM bytes per SCC launch measured with rocprofv3 counters (if recorded)."""
+    p = os.path.join(ROOT, "profiles", "scc_traffic.json")
+    if os.path.exists(p):
+        try:
+            return float(json.load(open(p))["bytes_per_launch"])
+        except Exception:
+            return None
+    return None
+
+
+def run_gpu(ntraj, steps, warmup, use_graph_check=True):
+    import qcxms_amd
+    trajs = workload(ntraj)
+    nbf = 50
+    md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=False)
+    md.run(warmup)
+    md.set_profiling(True)
+    for k in ("setup", "scc", "grad", "md_pre", "md_post"):
+        md.kernel_time(k)
+    it0 = md.scc_iterations_total().astype(np.float64).sum()
+    n0 = sum(r.nstep for r in md.read()["results"])
+    t0 = time.perf_counter()
+    md.run(steps)
+    t1 = time.perf_counter()
+    res = md.read()["results"]
+    n1 = sum(r.nstep for r in res)
+    it1 = md.scc_iterations_total().astype(np.float64).sum()
+    kt = {k: md.kernel_time(k) for k in ("setup", "scc", "grad", "md_pre", "md_post")}
+    md.set_profiling(False)
+    running = sum(r.running for r in res)
+    md.close()
+    wall = t1 - t0
+    traj_steps = n1 - n0
+    scc_ms, scc_launches = kt["scc"]
+    flops = (it1 - it0) * scc_flops(nbf)
+    achieved = flops / (scc_ms * 1e-3) / 1e12 if scc_ms > 0 else 0.0
+    out = dict(wall=wall, traj_steps=traj_steps, value=traj_steps / wall,
+               ms_per_step=1e3 * wall / steps, running=running,
+               scc_iters_per_traj_step=(it1 - it0) / max(traj_steps, 1),
+               kernels={k: dict(ms_total=v[0], launches=v[1],
+                                ms_avg=(v[0] / v[1] if v[1] else 0.0)) for k, v in kt.items()},
+               achieved_tflops=achieved, scc_avg_ms=scc_ms / max(scc_launches, 1),
+               scc_flops_per_launch=flops / max(scc_launches, 1))
+    # the same steps replayed as one hipGraph per step (no per-kernel events)
+    if use_graph_check:
+        md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True)
+        md.run(warmup)
+        n0 = sum(r.nstep for r in md.read()["results"])
+        t0 = time.perf_counter()
+        md.run(steps)
+        t1 = time.perf_counter()
+        n1 = sum(r.nstep for r in md.read()["results"])
+        md.close()
+        out["graph_value"] = (n1 - n0) / (t1 - t0)
+    return out
+
+
+def run_one_per_launch(steps, warmup):
+    import qcxms_amd
+    trajs = workload(1)
+    md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True)
+    md.run(warmup)
+    n0 = md.read()["results"][0].nstep
+    t0 = time.perf_counter()
+    md.run(steps)
+    t1 = time.perf_counter()
+    n1 = md.read()["results"][0].nstep
+    md.close()
+    return (n1 - n0) / (t1 - t0)
+
+
+def run_cpu_baseline(steps):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes as C
+
+    import oracle_bind as ob
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    ntraj = threads
+    trajs = workload(ntraj)
+    L = ob.oracle()
+    L.orc_md_run_batch.argtypes = [C.c_int, ob.ip, ob.ip, ob.ip, ob.dp, ob.dp, ob.dp, ob.dp,
+                                   ob.dp, ob.dp, ob.ip, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+    off = np.cumsum([0] + [len(t["num"]) for t in trajs]).astype(np.int32)
+    num = np.concatenate([t["num"] for t in trajs]).astype(np.int32)
+    imass = np.full(len(num), -1, dtype=np.int32)
+    mass = np.concatenate([t["mass"] for t in trajs])
+    velof = np.ones(len(num))
+    xyz = np.ascontiguousarray(np.concatenate([t["xyz"] for t in trajs]))
+    velo = np.ascontiguousarray(np.concatenate([t["velo"] for t in trajs]))
+    grad = np.zeros_like(xyz)
+    qat = np.zeros(len(num))
+    lst = np.zeros(len(num), dtype=np.int32)
+    states = (ob.MdState * ntraj)()
+    for s, t in enumerate(trajs):
+        st = states[s]
+        st.nat, st.chrg, st.nfragexit, st.max_steps = len(t["num"]), 1, 3, 5000
+        st.tstep, st.tadd, st.eimp = 0.5 * 41.3413733365614, t["tadd"], t["eimp"]
+        st.ax, st.ieetemp, st.etempin = 0.0, 0.0, -1.0
+    opts = ob.default_opts()
+    t0 = time.perf_counter()
+    L.orc_md_run_batch(ntraj, ob.ptr(off, ob.ip), ob.ptr(num, ob.ip), ob.ptr(imass, ob.ip),
+                       ob.ptr(mass), ob.ptr(velof), ob.ptr(xyz), ob.ptr(velo), ob.ptr(grad),
+                       ob.ptr(qat), ob.ptr(lst, ob.ip), C.cast(states, C.c_void_p),
+                       C.cast(C.pointer(opts), C.c_void_p), steps, threads)
+    wall = time.perf_counter() - t0
+    done = sum(states[s].nstep for s in range(ntraj))
+    return dict(value=done / wall, unit="trajectory-steps/s", cores=threads, kind="port",
+                sample=f"{ntraj} trajectories x {steps} MD steps of the same C8H18+ workload "
+                       f"(oracle C restatement, SCC cold-started every step as the reference, "
+                       f"one trajectory per thread; not QCxMS+tblite), wall {wall:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--ntraj", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=25)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-one-per-launch", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs every trajectory on a single "
+              "MI355X and has no multi-GPU path; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    g = run_gpu(args.ntraj, args.steps, args.warmup)
+    line = {
+        "metric": METRIC,
+        "value": g["value"],
+        "unit": "trajectory-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": g["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched EI MD: 256 x C8H18+ (26 atoms, 50 basis functions), "
+                               "GFN2-xTB FP64, etemp 5000 K, 0.5 fs leap-frog, IEE Poisson "
+                               "0.6 eV/atom (<= 1.5), tadd 100 fs, 500 K Maxwell-Boltzmann",
+                   "ntraj": args.ntraj, "trajectory_steps": g["traj_steps"],
+                   "running_after": g["running"],
+                   "scc_iterations_per_trajectory_step": g["scc_iters_per_traj_step"]},
+        "roofline": {"bound": "mfma", "kernel": "k_scc",
+                     "achieved": g["achieved_tflops"], "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": g["achieved_tflops"] / FP64_PEAK_TFLOPS,
+                     "traffic": traffic_from_profiles(),
+                     "flops_per_launch": g["scc_flops_per_launch"],
+                     "avg_launch_ms": g["scc_avg_ms"]},
+        "kernels_ms_avg": {k: v["ms_avg"] for k, v in g["kernels"].items()},
+        "graph_replay_value": g.get("graph_value"),
+    }
+    if not args.no_one_per_launch:
+        line["one_per_launch"] = {"value": run_one_per_launch(100, 10),
+                                  "unit": "trajectory-steps/s"}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = run_cpu_baseline(args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -74,6 +74,8 @@ int qcxms_batch_egrad(qcxms_batch *b, const double *xyz, const double *etemp, do
 /* Per-system SCC iteration counts and energy parts
  * (band, es2, es3, aes, axc, disp2, atm, rep, -TS, electronic) [nsys*10]. */
 int qcxms_batch_info(qcxms_batch *b, int32_t *iterations, double *eparts);
+/* SCC iterations per system summed over all evaluations so far [nsys]. */
+int qcxms_batch_scc_iterations_total(qcxms_batch *b, uint64_t *total);
 /* Atomic dipoles [3*natoms] and traceless quadrupoles [6*natoms, xx xy yy xz yz zz]. */
 int qcxms_batch_multipoles(qcxms_batch *b, double *dipoles, double *quadrupoles);
 

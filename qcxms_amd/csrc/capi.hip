@@ -354,7 +354,10 @@ int allocate(qcxms_batch* b)
     ALLOC(iters, b->nsys);
     ALLOC(status, b->nsys);
     ALLOC(warm, b->nsys);
+    unsigned long long* itot;
+    ALLOC(itot, b->nsys);
     d.iters = iters;
+    d.iters_total = itot;
     d.status = status;
     d.warm = warm;
     d.active = nullptr;
@@ -660,6 +663,14 @@ int qcxms_batch_info(qcxms_batch* b, int32_t* iterations, double* eparts)
         HIPCHK(hipMemcpyAsync(iterations, b->d.iters, sizeof(int) * b->nsys, hipMemcpyDeviceToHost, g_stream));
     if (eparts)
         HIPCHK(hipMemcpyAsync(eparts, b->d.eparts, sizeof(double) * 10 * b->nsys, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return QCXMS_OK;
+}
+
+int qcxms_batch_scc_iterations_total(qcxms_batch* b, uint64_t* total)
+{
+    if (!b || !total) return fail("null argument");
+    HIPCHK(hipMemcpyAsync(total, b->d.iters_total, sizeof(uint64_t) * b->nsys, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return QCXMS_OK;
 }

@@ -95,6 +95,7 @@ struct BatchPtrs {
     double* energy;   // per system: total
     double* eparts;   // per system: 10 parts (band, es2, es3, aes, axc, d4, atm, rep, ts, scc)
     int* iters;       // per system
+    unsigned long long* iters_total;  // per system: SCC iterations summed over calls
     int* status;      // per system: 0 ok, -1 not converged
     int* warm;        // per system: 1 if U/x hold a previous solution
     const int* active;  // per system: 0 skips the system (finished trajectories), may be null

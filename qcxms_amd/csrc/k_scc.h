@@ -621,6 +621,7 @@ __global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
         ep[9] = etot;
         B.energy[sys] = etot + ep[6] + ep[7];
         B.iters[sys] = it > opt.maxiter ? opt.maxiter : it;
+        B.iters_total[sys] += (unsigned long long)(it > opt.maxiter ? opt.maxiter : it);
         B.status[sys] = conv ? 0 : -1;
         B.warm[sys] = 1;
     }

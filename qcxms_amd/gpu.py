@@ -47,6 +47,7 @@ def library():
         L.qcxms_batch_egrad.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _ip]
         L.qcxms_batch_info.argtypes = [C.c_void_p, _ip, _dp]
         L.qcxms_batch_multipoles.argtypes = [C.c_void_p, _dp, _dp]
+        L.qcxms_batch_scc_iterations_total.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         L.qcxms_get_xtb_egrad.argtypes = [C.c_int32, _ip, _dp, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_double, _dp, _dp, _dp, _ip]
         L.qcxms_egrad_batch.argtypes = [C.c_int32, _ip, _ip, _dp, _ip, _ip, C.c_int32, _dp, _dp,
@@ -279,6 +280,12 @@ class MdEnsemble:
         library().qcxms_batch_kernel_time(C.c_void_p(self.batch_handle()), name.encode(),
                                           C.byref(ms), C.byref(n))
         return ms.value, n.value
+
+    def scc_iterations_total(self):
+        out = np.zeros(self.ntraj, dtype=np.uint64)
+        library().qcxms_batch_scc_iterations_total(C.c_void_p(self.batch_handle()),
+                                                   out.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return out
 
 
 def fragments_batch(nums, xyzs, masses):
