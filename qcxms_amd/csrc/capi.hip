@@ -179,7 +179,7 @@ int ensure_init(int device)
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_rad), rad, sizeof(rad)));
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_scc, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               2 * QX_LDS_MAXM * QX_LDS_MAXM * (int)sizeof(double)));
+                               5 * qx::kFastM * qx::kFastM * (int)sizeof(double)));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_setup, hipFuncAttributeMaxDynamicSharedMemorySize,
                                QX_LDS_MAXM * QX_LDS_MAXM * (int)sizeof(double)));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_grad, hipFuncAttributeMaxDynamicSharedMemorySize, 0));
@@ -202,7 +202,7 @@ struct qcxms_batch {
     int natoms = 0, nshells = 0, nrefs = 0, nxtot = 0;
     size_t mat_total = 0, gam_total = 0, d4_total = 0, hist_total = 0, pair_total = 0;
     int max_ld = 0, max_m = 0;
-    SccOpts opts{1e-9, 1e-8, 0.4, 1e-11, 100, 100};
+    SccOpts opts{1e-9, 1e-8, 0.4, 1e-11, 100, 100, 0, 0};
     int warm_start = 0;
     int profiling = 0;
     Timer t_setup, t_scc, t_grad, t_md_pre, t_md_post;
@@ -346,6 +346,8 @@ int allocate(qcxms_batch* b)
     ALLOC(d.focc, (size_t)b->nsys * QX_MAX_NAO);
     ALLOC(d.hist, b->hist_total);
     ALLOC(d.pairbuf, b->pair_total);
+    ALLOC(d.aesg, b->pair_total / 2);
+    ALLOC(d.xprev, b->nxtot);
     ALLOC(d.energy, b->nsys);
     ALLOC(d.eparts, 10 * (size_t)b->nsys);
     int* iters;
@@ -415,10 +417,15 @@ size_t lds_setup_bytes(const qcxms_batch* b)
     const size_t ls = std::min(b->max_ld, QX_LDS_MAXM);
     return sizeof(double) * ls * ls;
 }
+// five m x m working matrices for the systems that keep them in LDS
 size_t lds_scc_bytes(const qcxms_batch* b)
 {
-    const size_t lm = std::min(b->max_m, QX_LDS_MAXM);
-    return 2 * sizeof(double) * lm * lm;
+    size_t lm = 0;
+    for (const SysInfo& si : b->info) {
+        const size_t m = (size_t)((si.nao + 1) & ~1);
+        if (m <= (size_t)qx::kFastM) lm = std::max(lm, m);
+    }
+    return 5 * sizeof(double) * lm * lm;
 }
 
 // setup -> SCC -> gradient for every system of the batch (masked by d.active)
@@ -757,6 +764,7 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
     md->p.nfragexit = params->nfragexit;
     md->p.max_steps = params->max_steps;
     b->warm_start = 1;
+    b->opts.extrapolate = 1;
     if (params->econv > 0.0) b->opts.econv = params->econv;
     if (params->pconv > 0.0) b->opts.pconv = params->pconv;
     const int na = b->natoms;

@@ -53,6 +53,8 @@ struct SysInfo {
 struct SccOpts {
     double econv, pconv, damp, jacobi_tol;
     int maxiter, mem;
+    int extrapolate;  // 1: start MD steps from 2 x(n-1) - x(n-2)
+    int pad;
 };
 
 // Device-resident batch state.  Every pointer is a flat array over all systems.
@@ -92,6 +94,8 @@ struct BatchPtrs {
     double* focc;     // occupations, ld per system
     double* hist;     // Broyden history
     double* pairbuf;  // gradient scratch
+    double* aesg;     // per ordered atom pair: R_i - R_j (3), h3, h5, r^2 h5 (multipole kernels)
+    double* xprev;    // mixing vector of the step before (extrapolated MD start)
     double* energy;   // per system: total
     double* eparts;   // per system: 10 parts (band, es2, es3, aes, axc, d4, atm, rep, ts, scc)
     int* iters;       // per system

@@ -2,14 +2,17 @@
 // all iterations inside one launch (data-dependent iteration counts never
 // leave the device).
 //
-// Per iteration: potentials from the mixed moments -> Fock matrix ->
+// Per iteration: potentials from the mixed moments -> Fock matrix built from
+// row-local potentials G (F = H0 - (G + G^T)/2, coalesced row reads) ->
 // A = C^T F C in the current orbital basis C (S-orthonormal) -> parallel
-// cyclic Jacobi on A (round-robin pairing, LDS resident for order <= 64) ->
-// C <- C V -> Fermi occupations (one wavefront) -> P = C diag(2f) C^T ->
-// Mulliken shell charges and atomic dipole/quadrupole moments -> energy ->
-// modified Broyden mixing (Johnson, PRB 38, 12807 (1988)).
-// C starts as X U with X = L^-T and U the previous call's orthogonal basis
-// (identity on a cold start), so warm-started Jacobi needs few sweeps.
+// cyclic Jacobi on A (round-robin pairing, rounds without a significant
+// rotation skipped) -> C <- C V -> Fermi occupations (one wavefront) ->
+// P = C diag(2f) C^T -> Mulliken shell charges and atomic dipole/quadrupole
+// moments (wavefront per AO row) -> energy -> modified Broyden mixing
+// (Johnson, PRB 38, 12807 (1988)).
+// Systems with at most kFastM basis functions keep A, V, C, F and P in LDS;
+// larger ones use per-system global scratch.  C starts as X U with X = L^-T
+// and U the previous call's orthogonal basis (identity on a cold start).
 #pragma once
 #include "k_setup.h"
 
@@ -17,6 +20,7 @@ namespace qx {
 
 constexpr int kMaxNx = 128 + 9 * QX_MAX_NAT;
 constexpr int kMaxRef = QX_MAX_REF * QX_MAX_NAT;
+constexpr int kFastM = 52;
 
 // C[i][j] = sum_k opA[i][k] * d[k] * opB[k][j] over i < M, j < N, k < K
 template <bool TA, bool TB>
@@ -57,6 +61,39 @@ QX_DEV void block_gemm(int M, int N, int K, const double* A, int lda, const doub
     }
 }
 
+QX_DEV double block_max(double v, double* smem)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) smem[w] = v;
+    __syncthreads();
+    double s = smem[0];
+    for (int i = 1; i < QX_BLOCK / 64; ++i) s = fmax(s, smem[i]);
+    __syncthreads();
+    return s;
+}
+
+// block-wide sums of K values at once (all threads call)
+template <int K>
+QX_DEV void block_sums(double* v, double* smem)
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0)
+        for (int k = 0; k < K; ++k) smem[w * K + k] = v[k];
+    __syncthreads();
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        for (int i = 0; i < QX_BLOCK / 64; ++i) s += smem[i * K + k];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
 // round-robin pairing: round r, slot k of m players
 QX_DEV void rr_pair(int m, int r, int k, int& p, int& q)
 {
@@ -70,32 +107,43 @@ QX_DEV void rr_pair(int m, int r, int k, int& p, int& q)
 }
 
 // Parallel cyclic Jacobi on symmetric A (m x m, m even, leading dim lda),
-// accumulating rotations into V.  Returns the number of sweeps.
+// accumulating rotations into V (V must hold the identity on entry).
+// Stops after a sweep in which no off-diagonal element exceeded tol.
 QX_DEV int jacobi(double* A, double* V, int m, int lda, double tol, int maxsweep, double* rot,
-                  int* rpq, double* red)
+                  int* rpq, double* red, int* s_any)
 {
     const int half = m / 2;
+    double mx = 0.0;
+    for (int p = threadIdx.x; p < m * m; p += QX_BLOCK) {
+        const int i = p / m, j = p % m;
+        if (i != j) mx = fmax(mx, fabs(A[(size_t)i * lda + j]));
+    }
+    mx = block_max(mx, red);
+    if (mx < tol) return 0;
+    const double skip = 0.01 * tol;
+    if (threadIdx.x == 0) {
+        s_any[0] = 0;
+        s_any[1] = 0;
+    }
+    __syncthreads();
     int sweep = 0;
     for (; sweep < maxsweep; ++sweep) {
-        double off = 0.0;
-        for (int p = threadIdx.x; p < m * m; p += QX_BLOCK) {
-            const int i = p / m, j = p % m;
-            if (i != j) off += A[(size_t)i * lda + j] * A[(size_t)i * lda + j];
-        }
-        off = block_sum(off, red);
-        if (off < tol * tol) break;
+        double smx = 0.0;
         for (int r = 0; r < m - 1; ++r) {
+            const int f = r & 1;
             for (int k = threadIdx.x; k < half; k += QX_BLOCK) {
                 int p, q;
                 rr_pair(m, r, k, p, q);
                 const double apq = A[(size_t)p * lda + q];
+                smx = fmax(smx, fabs(apq));
                 double c = 1.0, s = 0.0;
-                if (apq != 0.0) {
+                if (fabs(apq) > skip) {
                     const double app = A[(size_t)p * lda + p], aqq = A[(size_t)q * lda + q];
                     const double tau = (aqq - app) / (2.0 * apq);
                     const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                     c = 1.0 / sqrt(1.0 + t * t);
                     s = t * c;
+                    s_any[f] = 1;
                 }
                 rot[2 * k] = c;
                 rot[2 * k + 1] = s;
@@ -103,44 +151,55 @@ QX_DEV int jacobi(double* A, double* V, int m, int lda, double tol, int maxsweep
                 rpq[2 * k + 1] = q;
             }
             __syncthreads();
-            const int nblk = half * (half + 1) / 2;
-            for (int b = threadIdx.x; b < nblk; b += QX_BLOCK) {
-                int P = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
-                while (P * (P + 1) / 2 > b) --P;
-                while ((P + 1) * (P + 2) / 2 <= b) ++P;
-                const int R = b - P * (P + 1) / 2;  // R <= P
-                const int p = rpq[2 * P], q = rpq[2 * P + 1], r = rpq[2 * R], s = rpq[2 * R + 1];
-                const double cp = rot[2 * P], sp = rot[2 * P + 1], cr = rot[2 * R], sr = rot[2 * R + 1];
-                const double b00 = A[(size_t)p * lda + r], b01 = A[(size_t)p * lda + s];
-                const double b10 = A[(size_t)q * lda + r], b11 = A[(size_t)q * lda + s];
-                const double m00 = b00 * cr - b01 * sr, m01 = b00 * sr + b01 * cr;
-                const double m10 = b10 * cr - b11 * sr, m11 = b10 * sr + b11 * cr;
-                double n00 = cp * m00 - sp * m10, n01 = cp * m01 - sp * m11;
-                double n10 = sp * m00 + cp * m10, n11 = sp * m01 + cp * m11;
-                if (P == R) {
-                    n01 = 0.0;
-                    n10 = 0.0;
+            if (s_any[f]) {
+                const int nblk = half * (half + 1) / 2;
+                for (int b = threadIdx.x; b < nblk; b += QX_BLOCK) {
+                    int P = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+                    while (P * (P + 1) / 2 > b) --P;
+                    while ((P + 1) * (P + 2) / 2 <= b) ++P;
+                    const int R = b - P * (P + 1) / 2;  // R <= P
+                    const double cp = rot[2 * P], sp = rot[2 * P + 1], cr = rot[2 * R], sr = rot[2 * R + 1];
+                    if (sp == 0.0 && sr == 0.0) continue;
+                    const int p = rpq[2 * P], q = rpq[2 * P + 1], r2 = rpq[2 * R], s2 = rpq[2 * R + 1];
+                    const double b00 = A[(size_t)p * lda + r2], b01 = A[(size_t)p * lda + s2];
+                    const double b10 = A[(size_t)q * lda + r2], b11 = A[(size_t)q * lda + s2];
+                    const double m00 = b00 * cr - b01 * sr, m01 = b00 * sr + b01 * cr;
+                    const double m10 = b10 * cr - b11 * sr, m11 = b10 * sr + b11 * cr;
+                    double n00 = cp * m00 - sp * m10, n01 = cp * m01 - sp * m11;
+                    double n10 = sp * m00 + cp * m10, n11 = sp * m01 + cp * m11;
+                    if (P == R) {
+                        n01 = 0.0;
+                        n10 = 0.0;
+                    }
+                    A[(size_t)p * lda + r2] = n00;
+                    A[(size_t)p * lda + s2] = n01;
+                    A[(size_t)q * lda + r2] = n10;
+                    A[(size_t)q * lda + s2] = n11;
+                    if (P != R) {
+                        A[(size_t)r2 * lda + p] = n00;
+                        A[(size_t)s2 * lda + p] = n01;
+                        A[(size_t)r2 * lda + q] = n10;
+                        A[(size_t)s2 * lda + q] = n11;
+                    }
                 }
-                A[(size_t)p * lda + r] = n00;
-                A[(size_t)p * lda + s] = n01;
-                A[(size_t)q * lda + r] = n10;
-                A[(size_t)q * lda + s] = n11;
-                if (P != R) {
-                    A[(size_t)r * lda + p] = n00;
-                    A[(size_t)s * lda + p] = n01;
-                    A[(size_t)r * lda + q] = n10;
-                    A[(size_t)s * lda + q] = n11;
+                for (int it = threadIdx.x; it < m * half; it += QX_BLOCK) {
+                    const int i = it / half, k = it % half;
+                    const double s = rot[2 * k + 1];
+                    if (s == 0.0) continue;
+                    const double c = rot[2 * k];
+                    const int p = rpq[2 * k], q = rpq[2 * k + 1];
+                    const double vp = V[(size_t)i * lda + p], vq = V[(size_t)i * lda + q];
+                    V[(size_t)i * lda + p] = c * vp - s * vq;
+                    V[(size_t)i * lda + q] = s * vp + c * vq;
                 }
             }
-            for (int it = threadIdx.x; it < m * half; it += QX_BLOCK) {
-                const int i = it / half, k = it % half;
-                const int p = rpq[2 * k], q = rpq[2 * k + 1];
-                const double c = rot[2 * k], s = rot[2 * k + 1];
-                const double vp = V[(size_t)i * lda + p], vq = V[(size_t)i * lda + q];
-                V[(size_t)i * lda + p] = c * vp - s * vq;
-                V[(size_t)i * lda + q] = s * vp + c * vq;
-            }
+            if (threadIdx.x == 0) s_any[f ^ 1] = 0;
             __syncthreads();
+        }
+        smx = block_max(smx, red);
+        if (smx < tol) {
+            ++sweep;
+            break;
         }
     }
     return sweep;
@@ -149,7 +208,6 @@ QX_DEV int jacobi(double* A, double* V, int m, int lda, double tol, int maxsweep
 // Fermi occupations per spin on wavefront 0 for nalpha electrons; returns -TS.
 QX_DEV double fermi_wave(int n, const double* e, double kt, double nalpha, double* f)
 {
-    // lanes hold up to 4 eigenvalues each (n <= 256)
     const int l = threadIdx.x & 63;
     double lo = 1e300, hi = -1e300;
     for (int i = l; i < n; i += 64) {
@@ -202,7 +260,7 @@ struct SysShared {
     double mrad[QX_MAX_NAT];
     double qat[QX_MAX_NAT], vat[QX_MAX_NAT];
     double zr[kMaxRef], dzr[kMaxRef], yr[kMaxRef];
-    double red[QX_BLOCK / 64];
+    double red[8 * (QX_BLOCK / 64)];
     double eparts[6];  // es2, es3, aes, axc, d4, unused
 };
 
@@ -211,7 +269,8 @@ struct SysShared {
 QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, const double* x,
                        double* vs, double* vd, double* vq)
 {
-    const int tid = threadIdx.x, nat = si.nat, nsh = si.nsh;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nat = si.nat, nsh = si.nsh;
     const double* qsh = x;
     const double* dp = x + nsh;
     const double* qp = x + nsh + 3 * nat;
@@ -221,7 +280,6 @@ QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, con
         ss.qat[a] = q;
     }
     __syncthreads();
-    // D4: z_k, dz_k per reference
     const int nd = si.ndim;
     for (int k = tid; k < nd; k += QX_BLOCK) {
         int a = 0;
@@ -233,94 +291,97 @@ QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, con
         ss.dzr[k] = d4_dzeta(gfn2::kD4Ga, c, e.refq[r] + e.zeff, ss.qat[a] + e.zeff);
     }
     __syncthreads();
+    // D4 response y = W z: wavefront per row, lanes over columns (coalesced)
     const double* WD = B.wd4 + si.d4_off;
     double ed4 = 0.0;
-    for (int k = tid; k < nd; k += QX_BLOCK) {
+    for (int k = wave; k < nd; k += QX_BLOCK / 64) {
         double y = 0.0;
-        for (int l = 0; l < nd; ++l) y += WD[(size_t)k * nd + l] * ss.zr[l];
-        ss.yr[k] = y;
-        ed4 += -0.5 * ss.zr[k] * y;
+        for (int l = lane; l < nd; l += 64) y += WD[(size_t)k * nd + l] * ss.zr[l];
+        y = wave_sum(y);
+        if (lane == 0) {
+            ss.yr[k] = y;
+            ed4 += -0.5 * ss.zr[k] * y;
+        }
     }
     __syncthreads();
-    // atom-level: AES, AXC, D4 potential
+    // AES and AXC: wavefront per atom i, lanes over partners j (nat <= 64)
+    const double* ag = B.aesg + si.pair_off / 2;
     double eaes = 0.0, eaxc = 0.0;
-    for (int i = tid; i < nat; i += QX_BLOCK) {
-        const ElemParams& ei = g_tab.e[ss.sel[i]];
+    for (int i = wave; i < nat; i += QX_BLOCK / 64) {
         const double* mi = dp + 3 * i;
         const double* ti = qp + 6 * i;
-        double va = 0.0, vdi[3] = {0, 0, 0}, vqi[6] = {0, 0, 0, 0, 0, 0};
-        for (int j = 0; j < nat; ++j) {
-            if (j == i) continue;
-            const double rv[3] = {ss.sx[3 * i] - ss.sx[3 * j], ss.sx[3 * i + 1] - ss.sx[3 * j + 1],
-                                  ss.sx[3 * i + 2] - ss.sx[3 * j + 2]};  // R_i - R_j
-            const double r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
-            const double r = sqrt(r2);
-            const double r0 = 0.5 * (ss.mrad[i] + ss.mrad[j]);
-            const double r03 = r0 * r0 * r0;
-            const double h3 = 1.0 / (r2 * r + 6.0 * r03);
-            const double h5 = 1.0 / (r * (r2 * r2 + 6.0 * r03 * r0));
-            const double h5b = r2 * h5;
+        double acc[11];
+#pragma unroll
+        for (int k = 0; k < 11; ++k) acc[k] = 0.0;
+        const int j = lane;
+        if (j < nat && j != i) {
+            const double* g = ag + (size_t)(i * nat + j) * 8;
+            const double rv0 = g[0], rv1 = g[1], rv2 = g[2], h3 = g[3], h5 = g[4], h5b = g[5];
             const double* mj = dp + 3 * j;
             const double* tj = qp + 6 * j;
-            const double mjr = mj[0] * rv[0] + mj[1] * rv[1] + mj[2] * rv[2];
-            const double mir = mi[0] * rv[0] + mi[1] * rv[1] + mi[2] * rv[2];
-            const double tjrr = tj[0] * rv[0] * rv[0] + 2 * tj[1] * rv[0] * rv[1] + tj[2] * rv[1] * rv[1] +
-                                2 * tj[3] * rv[0] * rv[2] + 2 * tj[4] * rv[1] * rv[2] + tj[5] * rv[2] * rv[2];
+            const double mjr = mj[0] * rv0 + mj[1] * rv1 + mj[2] * rv2;
+            const double tjrr = tj[0] * rv0 * rv0 + 2 * tj[1] * rv0 * rv1 + tj[2] * rv1 * rv1 +
+                                2 * tj[3] * rv0 * rv2 + 2 * tj[4] * rv1 * rv2 + tj[5] * rv2 * rv2;
             const double qj = ss.qat[j];
-            va += mjr * h3 + tjrr * h5;
-            for (int c = 0; c < 3; ++c) vdi[c] += -qj * rv[c] * h3 + mj[c] * h5b - 3.0 * rv[c] * mjr * h5;
-            vqi[0] += qj * rv[0] * rv[0] * h5;
-            vqi[1] += qj * 2 * rv[0] * rv[1] * h5;
-            vqi[2] += qj * rv[1] * rv[1] * h5;
-            vqi[3] += qj * 2 * rv[0] * rv[2] * h5;
-            vqi[4] += qj * 2 * rv[1] * rv[2] * h5;
-            vqi[5] += qj * rv[2] * rv[2] * h5;
+            acc[0] = mjr * h3 + tjrr * h5;
+            acc[1] = -qj * rv0 * h3 + mj[0] * h5b - 3.0 * rv0 * mjr * h5;
+            acc[2] = -qj * rv1 * h3 + mj[1] * h5b - 3.0 * rv1 * mjr * h5;
+            acc[3] = -qj * rv2 * h3 + mj[2] * h5b - 3.0 * rv2 * mjr * h5;
+            acc[4] = qj * rv0 * rv0 * h5;
+            acc[5] = qj * 2 * rv0 * rv1 * h5;
+            acc[6] = qj * rv1 * rv1 * h5;
+            acc[7] = qj * 2 * rv0 * rv2 * h5;
+            acc[8] = qj * 2 * rv1 * rv2 * h5;
+            acc[9] = qj * rv2 * rv2 * h5;
             if (j < i) {
+                const double mir = mi[0] * rv0 + mi[1] * rv1 + mi[2] * rv2;
                 const double mimj = mi[0] * mj[0] + mi[1] * mj[1] + mi[2] * mj[2];
-                const double tirr = ti[0] * rv[0] * rv[0] + 2 * ti[1] * rv[0] * rv[1] + ti[2] * rv[1] * rv[1] +
-                                    2 * ti[3] * rv[0] * rv[2] + 2 * ti[4] * rv[1] * rv[2] + ti[5] * rv[2] * rv[2];
+                const double tirr = ti[0] * rv0 * rv0 + 2 * ti[1] * rv0 * rv1 + ti[2] * rv1 * rv1 +
+                                    2 * ti[3] * rv0 * rv2 + 2 * ti[4] * rv1 * rv2 + ti[5] * rv2 * rv2;
                 const double qi = ss.qat[i];
-                eaes += qi * mjr * h3 - qj * mir * h3 + mimj * h5b - 3.0 * mir * mjr * h5 +
-                        qi * tjrr * h5 + qj * tirr * h5;
+                acc[10] = qi * mjr * h3 - qj * mir * h3 + mimj * h5b - 3.0 * mir * mjr * h5 +
+                          qi * tjrr * h5 + qj * tirr * h5;
             }
         }
-        eaxc += ei.dkernel * (mi[0] * mi[0] + mi[1] * mi[1] + mi[2] * mi[2]) +
-                ei.qkernel * (ti[0] * ti[0] + ti[2] * ti[2] + ti[5] * ti[5] +
-                              2.0 * (ti[1] * ti[1] + ti[3] * ti[3] + ti[4] * ti[4]));
-        for (int c = 0; c < 3; ++c) vd[3 * i + c] = vdi[c] + 2.0 * ei.dkernel * mi[c];
-        vq[6 * i + 0] = vqi[0] + 2.0 * ei.qkernel * ti[0];
-        vq[6 * i + 1] = vqi[1] + 4.0 * ei.qkernel * ti[1];
-        vq[6 * i + 2] = vqi[2] + 2.0 * ei.qkernel * ti[2];
-        vq[6 * i + 3] = vqi[3] + 4.0 * ei.qkernel * ti[3];
-        vq[6 * i + 4] = vqi[4] + 4.0 * ei.qkernel * ti[4];
-        vq[6 * i + 5] = vqi[5] + 2.0 * ei.qkernel * ti[5];
-        double vdisp = 0.0;
-        for (int k = ss.refoff[i]; k < ss.refoff[i + 1]; ++k) vdisp += -ss.dzr[k] * ss.yr[k];
-        ss.vat[i] = va + vdisp;
+#pragma unroll
+        for (int k = 0; k < 11; ++k) acc[k] = wave_sum(acc[k]);
+        if (lane == 0) {
+            const ElemParams& ei = g_tab.e[ss.sel[i]];
+            eaes += acc[10];
+            eaxc += ei.dkernel * (mi[0] * mi[0] + mi[1] * mi[1] + mi[2] * mi[2]) +
+                    ei.qkernel * (ti[0] * ti[0] + ti[2] * ti[2] + ti[5] * ti[5] +
+                                  2.0 * (ti[1] * ti[1] + ti[3] * ti[3] + ti[4] * ti[4]));
+            for (int c = 0; c < 3; ++c) vd[3 * i + c] = acc[1 + c] + 2.0 * ei.dkernel * mi[c];
+            vq[6 * i + 0] = acc[4] + 2.0 * ei.qkernel * ti[0];
+            vq[6 * i + 1] = acc[5] + 4.0 * ei.qkernel * ti[1];
+            vq[6 * i + 2] = acc[6] + 2.0 * ei.qkernel * ti[2];
+            vq[6 * i + 3] = acc[7] + 4.0 * ei.qkernel * ti[3];
+            vq[6 * i + 4] = acc[8] + 4.0 * ei.qkernel * ti[4];
+            vq[6 * i + 5] = acc[9] + 2.0 * ei.qkernel * ti[5];
+            double vdisp = 0.0;
+            for (int k = ss.refoff[i]; k < ss.refoff[i + 1]; ++k) vdisp += -ss.dzr[k] * ss.yr[k];
+            ss.vat[i] = acc[0] + vdisp;
+        }
     }
     __syncthreads();
+    // ES2 + third order: wavefront per shell row of gamma
     const double* G = B.gam + si.gam_off;
     double es2 = 0.0, es3 = 0.0;
-    for (int a = tid; a < nsh; a += QX_BLOCK) {
+    for (int a = wave; a < nsh; a += QX_BLOCK / 64) {
         double v = 0.0;
-        for (int b = 0; b < nsh; ++b) v += G[(size_t)a * nsh + b] * qsh[b];
-        es2 += 0.5 * qsh[a] * v;
-        const double g3 = g_tab.e[ss.sh[a].el].third[ss.sh[a].k];
-        es3 += g3 * qsh[a] * qsh[a] * qsh[a] / 3.0;
-        vs[a] = v + g3 * qsh[a] * qsh[a] + ss.vat[ss.sh[a].at];
+        for (int b = lane; b < nsh; b += 64) v += G[(size_t)a * nsh + b] * qsh[b];
+        v = wave_sum(v);
+        if (lane == 0) {
+            es2 += 0.5 * qsh[a] * v;
+            const double g3 = g_tab.e[ss.sh[a].el].third[ss.sh[a].k];
+            es3 += g3 * qsh[a] * qsh[a] * qsh[a] / 3.0;
+            vs[a] = v + g3 * qsh[a] * qsh[a] + ss.vat[ss.sh[a].at];
+        }
     }
-    es2 = block_sum(es2, ss.red);
-    es3 = block_sum(es3, ss.red);
-    eaes = block_sum(eaes, ss.red);
-    eaxc = block_sum(eaxc, ss.red);
-    ed4 = block_sum(ed4, ss.red);
-    if (tid == 0) {
-        ss.eparts[0] = es2;
-        ss.eparts[1] = es3;
-        ss.eparts[2] = eaes;
-        ss.eparts[3] = eaxc;
-        ss.eparts[4] = ed4;
-    }
+    double e5[5] = {es2, es3, eaes, eaxc, ed4};
+    block_sums<5>(e5, ss.red);
+    if (tid == 0)
+        for (int k = 0; k < 5; ++k) ss.eparts[k] = e5[k];
     __syncthreads();
 }
 
@@ -351,51 +412,52 @@ QX_DEV void load_system(const SysInfo& si, const BatchPtrs& B, SysShared& ss)
     __syncthreads();
 }
 
-__global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
+struct SccBuffers {
+    double *A, *V, *C, *F, *P;  // LDS or global
+    int ld;                     // leading dimension of the five buffers
+};
+
+template <bool FAST>
+QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, SysShared& ss,
+                     SccBuffers bf, double* xin, double* xout, double* vs, double* vd, double* vq,
+                     double* eps, double* focc, double* f2, double* rot, int* rpq, double* s_e,
+                     int* s_any)
 {
     const int sys = blockIdx.x;
-    if (B.active && !B.active[sys]) return;
-    const SysInfo si = B.info[sys];
-    const int tid = threadIdx.x, nat = si.nat, nsh = si.nsh, n = si.nao, ld = si.ld, nx = si.nx;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nat = si.nat, nsh = si.nsh, n = si.nao, ld = si.ld, nx = si.nx;
     const int m = (n + 1) & ~1;
-    __shared__ SysShared ss;
-    __shared__ double xin[kMaxNx], xout[kMaxNx], vs[QX_MAX_NSH], vd[3 * QX_MAX_NAT],
-        vq[6 * QX_MAX_NAT];
-    __shared__ double eps[QX_MAX_NAO], focc[QX_MAX_NAO], f2[QX_MAX_NAO];
-    __shared__ double rot[QX_MAX_NAO];
-    __shared__ int rpq[QX_MAX_NAO];
-    __shared__ double s_e[4];
-    __shared__ int s_flag[2];
-    extern __shared__ double dyn[];
-
-    load_system(si, B, ss);
+    const int lb = bf.ld;
     const size_t ms = (size_t)ld * ld;
     const double* S = B.S + si.mat_off;
     const double* H = B.H0 + si.mat_off;
     const double* Dm = B.D + 3 * si.mat_off;
     const double* Qm = B.Q + 6 * si.mat_off;
-    double* Cg = B.C + si.mat_off;
-    double* F = B.T + si.mat_off;
-    double* T2 = B.P + si.mat_off;  // scratch until the final density is formed
-    double* Pg = B.P + si.mat_off;
-    const bool in_lds = (m <= QX_LDS_MAXM);
-    const int lda = in_lds ? m : ld;
-    double* A = in_lds ? dyn : (B.W + si.mat_off);
-    double* V = in_lds ? dyn + (size_t)m * m : (B.U + si.mat_off);
     double* xg = B.x + si.x_off;
-    const bool warm = B.warm[sys] != 0;
+    double* xp = B.xprev + si.x_off;
+    const int warm = B.warm[sys];
+    double* A = bf.A;
+    double* V = bf.V;
+    double* Cc = bf.C;
+    double* Ft = bf.F;
+    double* P = bf.P;
 
-    // initial moments and basis: C = X U (U = 1 on a cold start)
-    for (int i = tid; i < nx; i += QX_BLOCK) xin[i] = warm ? xg[i] : 0.0;
+    // initial moments (extrapolated in MD) and basis C = X U
+    for (int i = tid; i < nx; i += QX_BLOCK) {
+        double v = 0.0;
+        if (warm >= 2 && opt.extrapolate) v = 2.0 * xg[i] - xp[i];
+        else if (warm >= 1) v = xg[i];
+        xin[i] = v;
+        if (warm >= 1) xp[i] = xg[i];
+    }
     if (warm) {
-        block_gemm<false, false>(n, n, n, B.X + si.mat_off, ld, B.U + si.mat_off, ld, Cg, ld, nullptr);
+        block_gemm<false, false>(n, n, n, B.X + si.mat_off, ld, B.U + si.mat_off, ld, Cc, lb, nullptr);
     } else {
         const double* X = B.X + si.mat_off;
-        for (size_t p = tid; p < ms; p += QX_BLOCK) Cg[p] = X[p];
+        for (int p = tid; p < n * n; p += QX_BLOCK) Cc[(size_t)(p / n) * lb + p % n] = X[(size_t)(p / n) * ld + p % n];
     }
     __syncthreads();
 
-    // Broyden history
     double* hist = B.hist + si.hist_off;
     const int mem = opt.mem;
     double* hdf = hist;
@@ -404,7 +466,7 @@ __global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
     double* hom = ha + (size_t)mem * mem;
     double* hql = hom + mem;
     double* hdql = hql + nx;
-    double* hbeta = hdql + nx;  // mem*mem scratch
+    double* hbeta = hdql + nx;
     double* hgam = hbeta + (size_t)mem * mem;
 
     const double nalpha = 0.5 * si.nel;
@@ -412,82 +474,110 @@ __global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
     int it = 1, conv = 0;
     for (; it <= opt.maxiter; ++it) {
         potentials(si, B, ss, xin, vs, vd, vq);
-        // Fock matrix
+        // G = row-local potential part of the Fock matrix
+        for (int mu = wave; mu < n; mu += QX_BLOCK / 64) {
+            const int am = ss.ao_at[mu];
+            const double v0 = vs[ss.ao_sh[mu]];
+            const double d0 = vd[3 * am], d1 = vd[3 * am + 1], d2 = vd[3 * am + 2];
+            const double* q = vq + 6 * am;
+            for (int nu = lane; nu < n; nu += 64) {
+                const size_t o = (size_t)mu * ld + nu;
+                const double g = S[o] * v0 + Dm[o] * d0 + Dm[ms + o] * d1 + Dm[2 * ms + o] * d2 +
+                                 Qm[o] * q[0] + Qm[ms + o] * q[1] + Qm[2 * ms + o] * q[2] +
+                                 Qm[3 * ms + o] * q[3] + Qm[4 * ms + o] * q[4] + Qm[5 * ms + o] * q[5];
+                Ft[(size_t)mu * lb + nu] = g;
+            }
+        }
+        __syncthreads();
         for (int p = tid; p < n * n; p += QX_BLOCK) {
             const int mu = p / n, nu = p % n;
-            const int am = ss.ao_at[mu], an = ss.ao_at[nu];
-            const size_t o = (size_t)mu * ld + nu, ot = (size_t)nu * ld + mu;
-            double v = S[o] * (vs[ss.ao_sh[mu]] + vs[ss.ao_sh[nu]]);
-            for (int c = 0; c < 3; ++c) v += Dm[c * ms + o] * vd[3 * am + c] + Dm[c * ms + ot] * vd[3 * an + c];
-            for (int k = 0; k < 6; ++k) v += Qm[k * ms + o] * vq[6 * am + k] + Qm[k * ms + ot] * vq[6 * an + k];
-            F[o] = H[o] - 0.5 * v;
+            if (nu < mu) continue;
+            const double gmn = Ft[(size_t)mu * lb + nu], gnm = Ft[(size_t)nu * lb + mu];
+            const double f = H[(size_t)mu * ld + nu] - 0.5 * (gmn + gnm);
+            Ft[(size_t)mu * lb + nu] = f;
+            Ft[(size_t)nu * lb + mu] = f;
         }
         __syncthreads();
         // A = C^T F C
-        block_gemm<false, false>(n, n, n, F, ld, Cg, ld, T2, ld, nullptr);
+        block_gemm<false, false>(n, n, n, Ft, lb, Cc, lb, P, lb, nullptr);
         __syncthreads();
-        block_gemm<true, false>(n, n, n, Cg, ld, T2, ld, A, lda, nullptr);
+        block_gemm<true, false>(n, n, n, Cc, lb, P, lb, A, lb, nullptr);
         for (int p = tid; p < m * m; p += QX_BLOCK) {
             const int i = p / m, j = p % m;
-            V[(size_t)i * lda + j] = (i == j) ? 1.0 : 0.0;
-            if (i >= n || j >= n) A[(size_t)i * lda + j] = 0.0;
+            V[(size_t)i * lb + j] = (i == j) ? 1.0 : 0.0;
+            if (i >= n || j >= n) A[(size_t)i * lb + j] = 0.0;
         }
         __syncthreads();
-        jacobi(A, V, m, lda, opt.jacobi_tol, 50, rot, rpq, ss.red);
-        for (int i = tid; i < n; i += QX_BLOCK) eps[i] = A[(size_t)i * lda + i];
+        jacobi(A, V, m, lb, opt.jacobi_tol, 50, rot, rpq, ss.red, s_any);
+        for (int i = tid; i < n; i += QX_BLOCK) eps[i] = A[(size_t)i * lb + i];
         __syncthreads();
         if (tid < 64) {
             const double t = fermi_wave(n, eps, si.kt, nalpha, focc);
             if (tid == 0) s_e[0] = t;
         }
+        // C <- C V (into the Fock buffer, then swap)
+        block_gemm<false, false>(n, n, n, Cc, lb, V, lb, Ft, lb, nullptr);
         __syncthreads();
         ts = s_e[0];
-        // C <- C V ; P = C diag(2f) C^T
-        block_gemm<false, false>(n, n, n, Cg, ld, V, lda, T2, ld, nullptr);
-        __syncthreads();
-        for (int p = tid; p < n * n; p += QX_BLOCK) {
-            const int i = p / n, j = p % n;
-            Cg[(size_t)i * ld + j] = T2[(size_t)i * ld + j];
+        {
+            double* t = Cc;
+            Cc = Ft;
+            Ft = t;
         }
         for (int i = tid; i < n; i += QX_BLOCK) f2[i] = 2.0 * focc[i];
         __syncthreads();
-        block_gemm<false, true>(n, n, n, Cg, ld, Cg, ld, Pg, ld, f2);
+        block_gemm<false, true>(n, n, n, Cc, lb, Cc, lb, P, lb, f2);
         __syncthreads();
-        // output moments
+        // output moments: 10 row sums per AO row (wavefront per row), into A
+        double* rs = A;
+        for (int mu = wave; mu < n; mu += QX_BLOCK / 64) {
+            double acc[10];
+#pragma unroll
+            for (int k = 0; k < 10; ++k) acc[k] = 0.0;
+            for (int nu = lane; nu < n; nu += 64) {
+                const size_t o = (size_t)mu * ld + nu;
+                const double pm = P[(size_t)mu * lb + nu];
+                acc[0] += pm * S[o];
+                acc[1] += pm * Dm[o];
+                acc[2] += pm * Dm[ms + o];
+                acc[3] += pm * Dm[2 * ms + o];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc[4 + k] += pm * Qm[k * ms + o];
+            }
+#pragma unroll
+            for (int k = 0; k < 10; ++k) acc[k] = wave_sum(acc[k]);
+            if (lane == 0)
+                for (int k = 0; k < 10; ++k) rs[(size_t)mu * 10 + k] = acc[k];
+        }
+        __syncthreads();
         for (int s = tid; s < nsh; s += QX_BLOCK) {
             const ShellDesc d = ss.sh[s];
             double pop = 0.0;
-            for (int a = 0; a < d.nao; ++a) {
-                const int mu = d.ao0 + a;
-                for (int nu = 0; nu < n; ++nu) pop += Pg[(size_t)mu * ld + nu] * S[(size_t)mu * ld + nu];
-            }
+            for (int a = 0; a < d.nao; ++a) pop += rs[(size_t)(d.ao0 + a) * 10];
             xout[s] = g_tab.e[d.el].refocc[d.k] - pop;
         }
         for (int t = tid; t < 9 * nat; t += QX_BLOCK) {
             const int a = t / 9, c = t % 9;
-            const double* M = (c < 3) ? (Dm + c * ms) : (Qm + (c - 3) * ms);
             double v = 0.0;
-            for (int mu = ss.aooff[a]; mu < ss.aooff[a + 1]; ++mu)
-                for (int nu = 0; nu < n; ++nu) v += Pg[(size_t)mu * ld + nu] * M[(size_t)mu * ld + nu];
+            for (int mu = ss.aooff[a]; mu < ss.aooff[a + 1]; ++mu) v += rs[(size_t)mu * 10 + 1 + c];
             if (c < 3) xout[nsh + 3 * a + c] = -v;
             else xout[nsh + 3 * nat + 6 * a + (c - 3)] = -v;
         }
         __syncthreads();
         potentials(si, B, ss, xout, vs, vd, vq);
-        double eb = 0.0, err = 0.0;
+        double red2[2] = {0.0, 0.0};
         for (int p = tid; p < n * n; p += QX_BLOCK) {
             const int i = p / n, j = p % n;
-            eb += Pg[(size_t)i * ld + j] * H[(size_t)i * ld + j];
+            red2[0] += P[(size_t)i * lb + j] * H[(size_t)i * ld + j];
         }
         for (int i = tid; i < nx; i += QX_BLOCK) {
             const double dd = xout[i] - xin[i];
-            err += dd * dd;
+            red2[1] += dd * dd;
         }
-        eb = block_sum(eb, ss.red);
-        err = block_sum(err, ss.red);
-        eband = eb;
-        etot = eb + ss.eparts[0] + ss.eparts[1] + ss.eparts[2] + ss.eparts[3] + ss.eparts[4] + ts;
-        err = sqrt(err / nx);
+        block_sums<2>(red2, ss.red);
+        eband = red2[0];
+        etot = eband + ss.eparts[0] + ss.eparts[1] + ss.eparts[2] + ss.eparts[3] + ss.eparts[4] + ts;
+        const double err = sqrt(red2[1] / nx);
         const double de = etot - eold;
         eold = etot;
         if (it > 1 && fabs(de) < opt.econv && err < opt.pconv) {
@@ -495,9 +585,8 @@ __global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
             break;
         }
         if (it == opt.maxiter) break;
-        // modified Broyden update of xin with dq = xout - xin
-        const int bit = it;  // mixer iteration (1-based)
-        if (bit == 1) {
+        // modified Broyden update of xin with dq = xout - xin (stored into xout)
+        if (it == 1) {
             for (int i = tid; i < nx; i += QX_BLOCK) {
                 const double dq = xout[i] - xin[i];
                 hdql[i] = dq;
@@ -507,50 +596,50 @@ __global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
             __syncthreads();
             continue;
         }
-        int it1 = bit - 1;
+        int it1 = it - 1;
         if (it1 > mem) it1 = mem;
         const int k1 = it1 - 1;
-        double nrm = 0.0;
+        double nr[2] = {0.0, 0.0};
         for (int i = tid; i < nx; i += QX_BLOCK) {
-            const double d = (xout[i] - xin[i]) - hdql[i];
+            const double dq = xout[i] - xin[i];
+            xout[i] = dq;
+            const double d = dq - hdql[i];
             hdf[(size_t)k1 * nx + i] = d;
-            nrm += d * d;
+            nr[0] += d * d;
+            nr[1] += dq * dq;
         }
-        nrm = block_sum(nrm, ss.red);
-        const double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
-        double wn = 0.0;
+        block_sums<2>(nr, ss.red);
+        const double inv = nr[0] > 0.0 ? 1.0 / sqrt(nr[0]) : 0.0;
         for (int i = tid; i < nx; i += QX_BLOCK) {
             const double d = hdf[(size_t)k1 * nx + i] * inv;
             hdf[(size_t)k1 * nx + i] = d;
             hu[(size_t)k1 * nx + i] = opt.damp * d + inv * (xin[i] - hql[i]);
-            const double dq = xout[i] - xin[i];
-            wn += dq * dq;
         }
-        wn = block_sum(wn, ss.red);
         if (tid == 0) {
-            double w = sqrt(wn);
+            double w = sqrt(nr[1]);
             w = (w > 0.01 / 100000.0) ? 0.01 / w : 100000.0;
             if (w < 1.0) w = 1.0;
             hom[k1] = w;
         }
         __syncthreads();
-        for (int i = 0; i < it1; ++i) {
+        // dot products df_i . df_k1 and df_i . dq: wavefront per i
+        for (int i = wave; i < it1; i += QX_BLOCK / 64) {
             double s = 0.0, c = 0.0;
-            for (int j = tid; j < nx; j += QX_BLOCK) {
-                s += hdf[(size_t)i * nx + j] * hdf[(size_t)k1 * nx + j];
-                c += hdf[(size_t)i * nx + j] * (xout[j] - xin[j]);
+            for (int j = lane; j < nx; j += 64) {
+                const double di = hdf[(size_t)i * nx + j];
+                s += di * hdf[(size_t)k1 * nx + j];
+                c += di * xout[j];
             }
-            s = block_sum(s, ss.red);
-            c = block_sum(c, ss.red);
-            if (tid == 0) {
+            s = wave_sum(s);
+            c = wave_sum(c);
+            if (lane == 0) {
                 ha[(size_t)i * mem + k1] = s;
                 ha[(size_t)k1 * mem + i] = s;
-                hgam[i] = c;  // df_i . dq (times omega below)
+                hgam[i] = c;
             }
         }
         __syncthreads();
         if (tid == 0) {
-            // beta = omega0^2 1 + omega_i omega_j a_ij ; solve beta g = c by Cholesky
             const double w0 = 0.01;
             for (int i = 0; i < it1; ++i)
                 for (int j = 0; j < it1; ++j)
@@ -581,7 +670,7 @@ __global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
         }
         __syncthreads();
         for (int j = tid; j < nx; j += QX_BLOCK) {
-            const double dq = xout[j] - xin[j];
+            const double dq = xout[j];
             double q = xin[j];
             hdql[j] = dq;
             hql[j] = q;
@@ -605,10 +694,13 @@ __global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
         B.focc[(size_t)sys * QX_MAX_NAO + i] = focc[i];
         f2[i] = 2.0 * focc[i] * eps[i];
     }
+    double* Pg = B.P + si.mat_off;
+    if (P != Pg)
+        for (int p = tid; p < n * n; p += QX_BLOCK) Pg[(size_t)(p / n) * ld + p % n] = P[(size_t)(p / n) * lb + p % n];
     __syncthreads();
     // W = C diag(2 f e) C^T ; U = L^T C for the next warm start
-    block_gemm<false, true>(n, n, n, Cg, ld, Cg, ld, B.W + si.mat_off, ld, f2);
-    block_gemm<true, false>(n, n, n, B.L + si.mat_off, ld, Cg, ld, B.U + si.mat_off, ld, nullptr);
+    block_gemm<false, true>(n, n, n, Cc, lb, Cc, lb, B.W + si.mat_off, ld, f2);
+    block_gemm<true, false>(n, n, n, B.L + si.mat_off, ld, Cc, lb, B.U + si.mat_off, ld, nullptr);
     if (tid == 0) {
         double* ep = B.eparts + 10 * sys;
         ep[0] = eband;
@@ -620,10 +712,39 @@ __global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
         ep[8] = ts;
         ep[9] = etot;
         B.energy[sys] = etot + ep[6] + ep[7];
-        B.iters[sys] = it > opt.maxiter ? opt.maxiter : it;
-        B.iters_total[sys] += (unsigned long long)(it > opt.maxiter ? opt.maxiter : it);
+        const int itn = it > opt.maxiter ? opt.maxiter : it;
+        B.iters[sys] = itn;
+        B.iters_total[sys] += (unsigned long long)itn;
         B.status[sys] = conv ? 0 : -1;
-        B.warm[sys] = 1;
+        B.warm[sys] = warm >= 1 ? 2 : 1;
+    }
+}
+
+__global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
+{
+    const int sys = blockIdx.x;
+    if (B.active && !B.active[sys]) return;
+    const SysInfo si = B.info[sys];
+    __shared__ SysShared ss;
+    __shared__ double xin[kMaxNx], xout[kMaxNx], vs[QX_MAX_NSH], vd[3 * QX_MAX_NAT],
+        vq[6 * QX_MAX_NAT];
+    __shared__ double eps[QX_MAX_NAO], focc[QX_MAX_NAO], f2[QX_MAX_NAO];
+    __shared__ double rot[QX_MAX_NAO];
+    __shared__ int rpq[QX_MAX_NAO];
+    __shared__ double s_e[4];
+    __shared__ int s_any[2];
+    extern __shared__ double dyn[];
+    load_system(si, B, ss);
+    const int m = (si.nao + 1) & ~1;
+    if (m <= kFastM) {
+        const size_t mm = (size_t)m * m;
+        SccBuffers bf{dyn, dyn + mm, dyn + 2 * mm, dyn + 3 * mm, dyn + 4 * mm, m};
+        scc_body<true>(si, B, opt, ss, bf, xin, xout, vs, vd, vq, eps, focc, f2, rot, rpq, s_e, s_any);
+    } else {
+        SccBuffers bf{B.W + si.mat_off, B.U + si.mat_off, B.C + si.mat_off, B.T + si.mat_off,
+                      B.P + si.mat_off, si.ld};
+        scc_body<false>(si, B, opt, ss, bf, xin, xout, vs, vd, vq, eps, focc, f2, rot, rpq, s_e,
+                        s_any);
     }
 }
 

@@ -277,6 +277,31 @@ __global__ __launch_bounds__(QX_BLOCK) void k_setup(BatchPtrs B, int with_chol)
                     disp * gw[k] * gw[l] * g_tab.c6ref[sel[i]][ii][sel[j]][jj];
             }
     }
+    // 6b. geometric factors of the damped multipole kernels (AES2), ordered pairs
+    double* ag = B.aesg + si.pair_off / 2;
+    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
+        const int i = p / nat, j = p % nat;
+        double* o = ag + (size_t)p * 8;
+        if (i == j) {
+            for (int k = 0; k < 8; ++k) o[k] = 0.0;
+            continue;
+        }
+        const double rv0 = sx[3 * i] - sx[3 * j], rv1 = sx[3 * i + 1] - sx[3 * j + 1],
+                     rv2 = sx[3 * i + 2] - sx[3 * j + 2];
+        const double r2 = rv0 * rv0 + rv1 * rv1 + rv2 * rv2;
+        const double r = sqrt(r2);
+        const double r0 = 0.5 * (B.mrad[si.atom_off + i] + B.mrad[si.atom_off + j]);
+        const double r03 = r0 * r0 * r0;
+        const double h5 = 1.0 / (r * (r2 * r2 + 6.0 * r03 * r0));
+        o[0] = rv0;
+        o[1] = rv1;
+        o[2] = rv2;
+        o[3] = 1.0 / (r2 * r + 6.0 * r03);
+        o[4] = h5;
+        o[5] = r2 * h5;
+        o[6] = 0.0;
+        o[7] = 0.0;
+    }
     // 7. repulsion energy
     double erep = 0.0;
     for (int p = tid; p < nat * nat; p += QX_BLOCK) {
