@@ -17,6 +17,19 @@
  *     derivatives + explicit pair terms + CN chain rules.
  * The generalized eigenproblem uses Cholesky + Householder tridiagonalization
  * + implicit QL (Wilkinson & Reinsch, Handbook for Automatic Computation II).
+ *
+ * Pinning.  Every piece is pinned by a check that a plausible mistake fails:
+ * the analytic gradient against central finite differences of this oracle's
+ * energy, translation/rotation invariance, sum(q) = charge, size consistency,
+ * closed-form s-s and normalised STO-nG overlaps, the eigensolver against
+ * LAPACK (scipy), Stewart's published STO-nG expansions, and the D4 pieces
+ * (zeta, CN weights, reference C6, dispersion energy and gradient) bit-exact
+ * against the reference's own src/dftd4.f90 compiled from source.
+ * Parity unpinned: the GFN2 parameter values (recalled from the publication,
+ * gfn2_params_oracle.h) and therefore absolute energies, gradients and charges
+ * relative to tblite v0.2.1, tblite's exact Broyden variant and thresholds,
+ * and its D4 reference-charge set.  tblite cannot be built or imported here,
+ * and the reference holds no golden vectors for it.
  */
 #include "oracle.h"
 #include "gfn2_params_oracle.h"

@@ -179,7 +179,7 @@ int ensure_init(int device)
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_rad), rad, sizeof(rad)));
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_scc, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               5 * qx::kFastM * qx::kFastM * (int)sizeof(double)));
+                               (4 * qx::kFastM * (qx::kFastM + 1) + qx::kRs * qx::kFastM) * (int)sizeof(double)));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_setup, hipFuncAttributeMaxDynamicSharedMemorySize,
                                QX_LDS_MAXM * QX_LDS_MAXM * (int)sizeof(double)));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_grad, hipFuncAttributeMaxDynamicSharedMemorySize, 0));
@@ -346,7 +346,7 @@ int allocate(qcxms_batch* b)
     ALLOC(d.focc, (size_t)b->nsys * QX_MAX_NAO);
     ALLOC(d.hist, b->hist_total);
     ALLOC(d.pairbuf, b->pair_total);
-    ALLOC(d.aesg, b->pair_total / 2);
+    ALLOC(d.aesg, b->pair_total / QX_PAIR_STRIDE * QX_AES_COMP);
     ALLOC(d.xprev, b->nxtot);
     ALLOC(d.energy, b->nsys);
     ALLOC(d.eparts, 10 * (size_t)b->nsys);
@@ -417,7 +417,8 @@ size_t lds_setup_bytes(const qcxms_batch* b)
     const size_t ls = std::min(b->max_ld, QX_LDS_MAXM);
     return sizeof(double) * ls * ls;
 }
-// five m x m working matrices for the systems that keep them in LDS
+// four m x m working matrices and the AO row sums for the systems that keep
+// them in LDS
 size_t lds_scc_bytes(const qcxms_batch* b)
 {
     size_t lm = 0;
@@ -425,7 +426,7 @@ size_t lds_scc_bytes(const qcxms_batch* b)
         const size_t m = (size_t)((si.nao + 1) & ~1);
         if (m <= (size_t)qx::kFastM) lm = std::max(lm, m);
     }
-    return 5 * sizeof(double) * lm * lm;
+    return sizeof(double) * (4 * lm * (lm + 1) + qx::kRs * lm);
 }
 
 // setup -> SCC -> gradient for every system of the batch (masked by d.active)
@@ -561,6 +562,22 @@ int check_offsets(int nsys, const int32_t* offsets, int maxnat)
 extern "C" {
 
 int qcxms_gpu_init(int device) { return ensure_init(device); }
+
+#ifdef QX_PHASE_TIMING
+// per-phase SCC cycle counters of the first systems (instrumented builds only)
+int qcxms_debug_phase_read(unsigned long long* out, int nsys, int reset)
+{
+    if (!g_ready) return fail("library not initialised");
+    const int n = std::min(nsys, qx::kPhaseSys) * qx::kPhaseSlots;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpyFromSymbol(out, HIP_SYMBOL(qx::g_phase), sizeof(unsigned long long) * n));
+    if (reset) {
+        std::vector<unsigned long long> z(qx::kPhaseSys * qx::kPhaseSlots, 0ull);
+        HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(qx::g_phase), z.data(), sizeof(unsigned long long) * z.size()));
+    }
+    return QCXMS_OK;
+}
+#endif
 
 const char* qcxms_gpu_last_error(void) { return g_err.c_str(); }
 

@@ -16,6 +16,8 @@
 #define QX_MAX_NAO 256
 #define QX_MAX_REF 7
 #define QX_LDS_MAXM 64  // Jacobi matrices up to this order live in LDS
+#define QX_PAIR_STRIDE 16  // doubles per ordered atom pair in the gradient pair buffer
+#define QX_AES_COMP 10     // multipole-kernel factors per ordered atom pair
 
 #define QX_DEV __device__ __forceinline__
 
@@ -94,7 +96,7 @@ struct BatchPtrs {
     double* focc;     // occupations, ld per system
     double* hist;     // Broyden history
     double* pairbuf;  // gradient scratch
-    double* aesg;     // per ordered atom pair: R_i - R_j (3), h3, h5, r^2 h5 (multipole kernels)
+    double* aesg;     // multipole-kernel factors, QX_AES_COMP planes of nat x nat per system
     double* xprev;    // mixing vector of the step before (extrapolated MD start)
     double* energy;   // per system: total
     double* eparts;   // per system: 10 parts (band, es2, es3, aes, axc, d4, atm, rep, ts, scc)

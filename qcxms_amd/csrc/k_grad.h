@@ -14,7 +14,7 @@
 
 namespace qx {
 
-constexpr int kPairStride = 16;
+constexpr int kPairStride = QX_PAIR_STRIDE;
 
 __global__ __launch_bounds__(QX_BLOCK) void k_grad(BatchPtrs B)
 {

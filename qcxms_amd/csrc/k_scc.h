@@ -3,14 +3,18 @@
 // leave the device).
 //
 // Per iteration: potentials from the mixed moments -> Fock matrix built from
-// row-local potentials G (F = H0 - (G + G^T)/2, coalesced row reads) ->
-// A = C^T F C in the current orbital basis C (S-orthonormal) -> parallel
-// cyclic Jacobi on A (round-robin pairing, rounds without a significant
-// rotation skipped) -> C <- C V -> Fermi occupations (one wavefront) ->
-// P = C diag(2f) C^T -> Mulliken shell charges and atomic dipole/quadrupole
-// moments (wavefront per AO row) -> energy -> modified Broyden mixing
-// (Johnson, PRB 38, 12807 (1988)).
-// Systems with at most kFastM basis functions keep A, V, C, F and P in LDS;
+// row-local potentials G (F = H0 - (G + G^T)/2, coalesced reads of the
+// transposed moment integrals) -> A = C^T F C in the current orbital basis C
+// (S-orthonormal) -> parallel cyclic threshold Jacobi on A with the rotations
+// applied straight to the columns of C -> Fermi occupations (one wavefront,
+// safeguarded Newton from the previous chemical potential) ->
+// P = C diag(2f) C^T -> Mulliken shell charges, atomic dipole/quadrupole
+// moments and the band energy as column sweeps -> energy -> modified Broyden
+// mixing (Johnson, PRB 38, 12807 (1988)).
+// The Jacobi tolerance follows the SCC: 1e-3 of the previous iteration's RMS
+// moment change, never looser than 1e-6 and never tighter than the requested
+// floor, so the converged iterate is diagonalised to the floor.
+// Systems with at most kFastM basis functions keep A, C, F and P in LDS;
 // larger ones use per-system global scratch.  C starts as X U with X = L^-T
 // and U the previous call's orthogonal basis (identity on a cold start).
 #pragma once
@@ -20,7 +24,50 @@ namespace qx {
 
 constexpr int kMaxNx = 128 + 9 * QX_MAX_NAT;
 constexpr int kMaxRef = QX_MAX_REF * QX_MAX_NAT;
-constexpr int kFastM = 52;
+constexpr int kFastM = 56;
+constexpr int kRs = 11;  // row sums per AO: S, D (3), Q (6), H0
+
+// Opt-in per-phase cycle counters of the SCC kernel (build with
+// -DQX_PHASE_TIMING); compiled out otherwise.
+#ifdef QX_PHASE_TIMING
+constexpr int kPhaseSys = 1024, kPhaseSlots = 32;
+__device__ unsigned long long g_phase[kPhaseSys * kPhaseSlots];
+#define QX_PH_INIT unsigned long long ph_t = clock64();
+#define QX_PH(k)                                                                   \
+    do {                                                                           \
+        __syncthreads();                                                           \
+        if (threadIdx.x == 0 && blockIdx.x < kPhaseSys) {                          \
+            const unsigned long long t_ = clock64();                               \
+            g_phase[blockIdx.x * kPhaseSlots + (k)] += t_ - ph_t;                  \
+            ph_t = t_;                                                             \
+        }                                                                          \
+    } while (0)
+#define QX_PH_ADD(k, v)                                                            \
+    do {                                                                           \
+        if (threadIdx.x == 0 && blockIdx.x < kPhaseSys) g_phase[blockIdx.x * kPhaseSlots + (k)] += (v); \
+    } while (0)
+#define QX_PT_INIT(tv) unsigned long long tv = clock64();
+#define QX_PT(k, tv)                                                               \
+    do {                                                                           \
+        if (threadIdx.x == 0 && blockIdx.x < kPhaseSys) {                          \
+            const unsigned long long t_ = clock64();                               \
+            g_phase[blockIdx.x * kPhaseSlots + (k)] += t_ - tv;                    \
+            tv = t_;                                                               \
+        }                                                                          \
+    } while (0)
+#else
+#define QX_PT_INIT(tv)
+#define QX_PT(k, tv) \
+    do {             \
+    } while (0)
+#define QX_PH_INIT
+#define QX_PH(k) \
+    do {         \
+    } while (0)
+#define QX_PH_ADD(k, v) \
+    do {                \
+    } while (0)
+#endif
 
 // C[i][j] = sum_k opA[i][k] * d[k] * opB[k][j] over i < M, j < N, k < K
 template <bool TA, bool TB>
@@ -29,7 +76,8 @@ QX_DEV void block_gemm(int M, int N, int K, const double* A, int lda, const doub
 {
     const int tm = (M + 3) >> 2, tn = (N + 3) >> 2;
     for (int t = threadIdx.x; t < tm * tn; t += QX_BLOCK) {
-        const int i0 = (t / tn) * 4, j0 = (t % tn) * 4;
+        // rows i0..i0+3, columns jb + b tn: neighbouring lanes read neighbouring columns
+        const int i0 = (t / tn) * 4, jb = t % tn;
         double acc[4][4];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
@@ -45,7 +93,7 @@ QX_DEV void block_gemm(int M, int N, int K, const double* A, int lda, const doub
             }
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const int j = j0 + b < N ? j0 + b : N - 1;
+                const int j = jb + b * tn < N ? jb + b * tn : N - 1;
                 bv[b] = TB ? Bm[(size_t)j * ldb + k] : Bm[(size_t)k * ldb + j];
             }
 #pragma unroll
@@ -57,7 +105,7 @@ QX_DEV void block_gemm(int M, int N, int K, const double* A, int lda, const doub
         for (int a = 0; a < 4; ++a)
 #pragma unroll
             for (int b = 0; b < 4; ++b)
-                if (i0 + a < M && j0 + b < N) Cm[(size_t)(i0 + a) * ldc + j0 + b] = acc[a][b];
+                if (i0 + a < M && jb + b * tn < N) Cm[(size_t)(i0 + a) * ldc + jb + b * tn] = acc[a][b];
     }
 }
 
@@ -94,119 +142,172 @@ QX_DEV void block_sums(double* v, double* smem)
     __syncthreads();
 }
 
-// round-robin pairing: round r, slot k of m players
+// round-robin pairing: round r, slot k of m players (m even, r < m - 1)
 QX_DEV void rr_pair(int m, int r, int k, int& p, int& q)
 {
     if (k == 0) {
         p = r;
         q = m - 1;
     } else {
-        p = (r + k) % (m - 1);
-        q = (r - k + (m - 1)) % (m - 1);
+        p = r + k;
+        if (p >= m - 1) p -= m - 1;
+        q = r - k;
+        if (q < 0) q += m - 1;
     }
 }
 
-// Parallel cyclic Jacobi on symmetric A (m x m, m even, leading dim lda),
-// accumulating rotations into V (V must hold the identity on entry).
-// Stops after a sweep in which no off-diagonal element exceeded tol.
-QX_DEV int jacobi(double* A, double* V, int m, int lda, double tol, int maxsweep, double* rot,
-                  int* rpq, double* red, int* s_any)
+// largest off-diagonal magnitude of the leading m x m block of A
+QX_DEV double offdiag_max(const double* A, int m, int lda, double* red)
+{
+    double mx = 0.0;
+    for (int i = threadIdx.x >> 6; i < m; i += QX_BLOCK / 64)
+        for (int j = threadIdx.x & 63; j < m; j += 64)
+            if (i != j) mx = fmax(mx, fabs(A[(size_t)i * lda + j]));
+    return block_max(mx, red);
+}
+
+// pair-ordered 2 x 2 block index b -> (P, R) with R <= P
+QX_DEV void tri_index(int b, int& P, int& R)
+{
+    P = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
+    while (P * (P + 1) / 2 > b) --P;
+    while ((P + 1) * (P + 2) / 2 <= b) ++P;
+    R = b - P * (P + 1) / 2;
+}
+
+// Rotate the 2 x 2 blocks (P, R) and (R, P) of A for one round.  All reads of
+// a thread's entries precede its writes; no entry is touched by two blocks.
+QX_DEV void jacobi_block(double* A, int lda, const double2* rot, const int2* rpq, int P, int R)
+{
+    const double2 rp = rot[P], rr = rot[R];
+    const double cp = rp.x, sp = rp.y, cr = rr.x, sr = rr.y;
+    if (sp == 0.0 && sr == 0.0) return;
+    const int2 ip = rpq[P], ir = rpq[R];
+    const int p = ip.x, q = ip.y, r2 = ir.x, s2 = ir.y;
+    const double b00 = A[(size_t)p * lda + r2], b01 = A[(size_t)p * lda + s2];
+    const double b10 = A[(size_t)q * lda + r2], b11 = A[(size_t)q * lda + s2];
+    const double m00 = b00 * cr - b01 * sr, m01 = b00 * sr + b01 * cr;
+    const double m10 = b10 * cr - b11 * sr, m11 = b10 * sr + b11 * cr;
+    double n00 = cp * m00 - sp * m10, n01 = cp * m01 - sp * m11;
+    double n10 = sp * m00 + cp * m10, n11 = sp * m01 + cp * m11;
+    if (P == R) {
+        n01 = 0.0;
+        n10 = 0.0;
+    }
+    A[(size_t)p * lda + r2] = n00;
+    A[(size_t)p * lda + s2] = n01;
+    A[(size_t)q * lda + r2] = n10;
+    A[(size_t)q * lda + s2] = n11;
+    if (P != R) {
+        A[(size_t)r2 * lda + p] = n00;
+        A[(size_t)s2 * lda + p] = n01;
+        A[(size_t)r2 * lda + q] = n10;
+        A[(size_t)s2 * lda + q] = n11;
+    }
+}
+
+// Parallel cyclic threshold Jacobi on symmetric A (m x m, m even, leading
+// dimension lda).  Pairs with |a_pq| < tol are not rotated; a sweep ends the
+// loop when no off-diagonal element of the result reaches tol.  Each rotation
+// is also applied to columns p, q of the nrow x m matrix Cv (leading
+// dimension lda), so Cv ends as Cv V with A = V^T A_in V diagonal.
+// One round = rotation parameters (one thread per pair) | barrier | block and
+// column updates | barrier; rounds without any rotation skip the updates.
+QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, int maxsweep,
+                  double2* rot, int2* rpq, double* red, int* s_any)
 {
     const int half = m / 2;
-    double mx = 0.0;
-    for (int p = threadIdx.x; p < m * m; p += QX_BLOCK) {
-        const int i = p / m, j = p % m;
-        if (i != j) mx = fmax(mx, fabs(A[(size_t)i * lda + j]));
-    }
-    mx = block_max(mx, red);
-    if (mx < tol) return 0;
-    const double skip = 0.01 * tol;
+    if (offdiag_max(A, m, lda, red) < tol) return 0;
     if (threadIdx.x == 0) {
         s_any[0] = 0;
         s_any[1] = 0;
     }
     __syncthreads();
+    const int nblk = half * (half + 1) / 2;
+    // the first two blocks of this thread are the same in every round
+    int bP0 = 0, bR0 = 0, bP1 = 0, bR1 = 0;
+    const bool has0 = threadIdx.x < nblk, has1 = threadIdx.x + QX_BLOCK < nblk;
+    if (has0) tri_index(threadIdx.x, bP0, bR0);
+    if (has1) tri_index(threadIdx.x + QX_BLOCK, bP1, bR1);
+    // columns of Cv: pair slot from the low 5 thread bits, rows ic + 8 j
+    const int kc = threadIdx.x & 31, ic = threadIdx.x >> 5;
     int sweep = 0;
-    for (; sweep < maxsweep; ++sweep) {
-        double smx = 0.0;
+    QX_PT_INIT(jt)
+    while (sweep < maxsweep) {
+        ++sweep;
         for (int r = 0; r < m - 1; ++r) {
             const int f = r & 1;
+            QX_PT(20, jt);
             for (int k = threadIdx.x; k < half; k += QX_BLOCK) {
                 int p, q;
                 rr_pair(m, r, k, p, q);
                 const double apq = A[(size_t)p * lda + q];
-                smx = fmax(smx, fabs(apq));
+                const double app = A[(size_t)p * lda + p], aqq = A[(size_t)q * lda + q];
                 double c = 1.0, s = 0.0;
-                if (fabs(apq) > skip) {
-                    const double app = A[(size_t)p * lda + p], aqq = A[(size_t)q * lda + q];
+                if (fabs(apq) >= tol) {
                     const double tau = (aqq - app) / (2.0 * apq);
                     const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                     c = 1.0 / sqrt(1.0 + t * t);
                     s = t * c;
                     s_any[f] = 1;
                 }
-                rot[2 * k] = c;
-                rot[2 * k + 1] = s;
-                rpq[2 * k] = p;
-                rpq[2 * k + 1] = q;
+                rot[k] = make_double2(c, s);
+                rpq[k] = make_int2(p, q);
             }
             __syncthreads();
+            QX_PT(16, jt);
+            QX_PH_ADD(19, 1);
             if (s_any[f]) {
-                const int nblk = half * (half + 1) / 2;
-                for (int b = threadIdx.x; b < nblk; b += QX_BLOCK) {
-                    int P = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
-                    while (P * (P + 1) / 2 > b) --P;
-                    while ((P + 1) * (P + 2) / 2 <= b) ++P;
-                    const int R = b - P * (P + 1) / 2;  // R <= P
-                    const double cp = rot[2 * P], sp = rot[2 * P + 1], cr = rot[2 * R], sr = rot[2 * R + 1];
-                    if (sp == 0.0 && sr == 0.0) continue;
-                    const int p = rpq[2 * P], q = rpq[2 * P + 1], r2 = rpq[2 * R], s2 = rpq[2 * R + 1];
-                    const double b00 = A[(size_t)p * lda + r2], b01 = A[(size_t)p * lda + s2];
-                    const double b10 = A[(size_t)q * lda + r2], b11 = A[(size_t)q * lda + s2];
-                    const double m00 = b00 * cr - b01 * sr, m01 = b00 * sr + b01 * cr;
-                    const double m10 = b10 * cr - b11 * sr, m11 = b10 * sr + b11 * cr;
-                    double n00 = cp * m00 - sp * m10, n01 = cp * m01 - sp * m11;
-                    double n10 = sp * m00 + cp * m10, n11 = sp * m01 + cp * m11;
-                    if (P == R) {
-                        n01 = 0.0;
-                        n10 = 0.0;
-                    }
-                    A[(size_t)p * lda + r2] = n00;
-                    A[(size_t)p * lda + s2] = n01;
-                    A[(size_t)q * lda + r2] = n10;
-                    A[(size_t)q * lda + s2] = n11;
-                    if (P != R) {
-                        A[(size_t)r2 * lda + p] = n00;
-                        A[(size_t)s2 * lda + p] = n01;
-                        A[(size_t)r2 * lda + q] = n10;
-                        A[(size_t)s2 * lda + q] = n11;
-                    }
+                QX_PH_ADD(18, 1);
+                if (has0) jacobi_block(A, lda, rot, rpq, bP0, bR0);
+                if (has1) jacobi_block(A, lda, rot, rpq, bP1, bR1);
+                for (int b = threadIdx.x + 2 * QX_BLOCK; b < nblk; b += QX_BLOCK) {
+                    int P, R;
+                    tri_index(b, P, R);
+                    jacobi_block(A, lda, rot, rpq, P, R);
                 }
-                for (int it = threadIdx.x; it < m * half; it += QX_BLOCK) {
-                    const int i = it / half, k = it % half;
-                    const double s = rot[2 * k + 1];
+                for (int k = kc; k < half; k += 32) {
+                    const double2 cs = rot[k];
+                    const double c = cs.x, s = cs.y;
                     if (s == 0.0) continue;
-                    const double c = rot[2 * k];
-                    const int p = rpq[2 * k], q = rpq[2 * k + 1];
-                    const double vp = V[(size_t)i * lda + p], vq = V[(size_t)i * lda + q];
-                    V[(size_t)i * lda + p] = c * vp - s * vq;
-                    V[(size_t)i * lda + q] = s * vp + c * vq;
+                    const int2 pq = rpq[k];
+                    const int p = pq.x, q = pq.y;
+                    for (int i0 = ic; i0 < nrow; i0 += 64) {
+                        double vp[8], vq[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const int i = i0 + 8 * u;
+                            vp[u] = i < nrow ? Cv[(size_t)i * lda + p] : 0.0;
+                            vq[u] = i < nrow ? Cv[(size_t)i * lda + q] : 0.0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const int i = i0 + 8 * u;
+                            if (i < nrow) {
+                                Cv[(size_t)i * lda + p] = c * vp[u] - s * vq[u];
+                                Cv[(size_t)i * lda + q] = s * vp[u] + c * vq[u];
+                            }
+                        }
+                    }
                 }
             }
             if (threadIdx.x == 0) s_any[f ^ 1] = 0;
             __syncthreads();
+            QX_PT(17, jt);
         }
-        smx = block_max(smx, red);
-        if (smx < tol) {
-            ++sweep;
-            break;
-        }
+        if (threadIdx.x == 0) s_any[0] = 0;  // set by the sweep's last round
+        if (offdiag_max(A, m, lda, red) < tol) break;
     }
     return sweep;
 }
 
 // Fermi occupations per spin on wavefront 0 for nalpha electrons; returns -TS.
-QX_DEV double fermi_wave(int n, const double* e, double kt, double nalpha, double* f)
+// The chemical potential is found by Newton steps safeguarded by a bracket
+// (bisection whenever a step leaves it), starting from mu_io when it lies
+// inside the bracket, until the electron count is met to 1e-13 relative, and
+// polished by one more Newton step; the root is written back to mu_io.
+QX_DEV double fermi_wave(int n, const double* e, double kt, double nalpha, double* f, double& mu_io,
+                         int& iters)
 {
     const int l = threadIdx.x & 63;
     double lo = 1e300, hi = -1e300;
@@ -220,17 +321,10 @@ QX_DEV double fermi_wave(int n, const double* e, double kt, double nalpha, doubl
     }
     lo = lo - 50.0 * kt - 1.0;
     hi = hi + 50.0 * kt + 1.0;
-    double mu = 0.0;
-    for (int it = 0; it < 200; ++it) {
-        mu = 0.5 * (lo + hi);
-        double s = 0.0;
-        for (int i = l; i < n; i += 64) s += 1.0 / (1.0 + exp((e[i] - mu) / kt));
-        s = wave_sum(s);
-        if (s > nalpha) hi = mu;
-        else lo = mu;
-        if (hi - lo < 1e-15 * (fabs(mu) + 1.0)) break;
-    }
-    for (int it = 0; it < 5; ++it) {
+    double mu = (mu_io > lo && mu_io < hi) ? mu_io : 0.5 * (lo + hi);
+    iters = 0;
+    for (int it = 0; it < 300; ++it) {
+        ++iters;
         double s = 0.0, ds = 0.0;
         for (int i = l; i < n; i += 64) {
             const double fi = 1.0 / (1.0 + exp((e[i] - mu) / kt));
@@ -239,8 +333,17 @@ QX_DEV double fermi_wave(int n, const double* e, double kt, double nalpha, doubl
         }
         s = wave_sum(s);
         ds = wave_sum(ds);
-        if (ds <= 0.0) break;
-        mu += (nalpha - s) / ds;
+        const double diff = nalpha - s;
+        if (fabs(diff) <= 1e-13 * fmax(nalpha, 1.0)) {
+            if (ds > 0.0) mu += diff / ds;  // final Newton polish
+            break;
+        }
+        if (s > nalpha) hi = mu;
+        else lo = mu;
+        double mun = (ds > 0.0) ? mu + diff / ds : 0.5 * (lo + hi);
+        if (!(mun > lo && mun < hi)) mun = 0.5 * (lo + hi);
+        mu = mun;
+        if (hi - lo <= 1e-15 * (fabs(mu) + 1.0)) break;
     }
     double ts = 0.0;
     for (int i = l; i < n; i += 64) {
@@ -248,6 +351,7 @@ QX_DEV double fermi_wave(int n, const double* e, double kt, double nalpha, doubl
         f[i] = fi;
         if (fi > 0.0 && fi < 1.0) ts += fi * log(fi) + (1.0 - fi) * log(1.0 - fi);
     }
+    mu_io = mu;
     return 2.0 * kt * wave_sum(ts);
 }
 
@@ -266,21 +370,26 @@ struct SysShared {
 
 // Potentials vs (per shell), vd (3 per atom), vq (6 per atom) and interaction
 // energies (ES2, ES3, AES, AXC, D4 two-body) at moments x = [qsh, dp, qp].
+// Wavefront 0 evaluates the multipole terms (lane = atom, loop over partners
+// through the precomputed kernel planes); wavefronts 1-3 evaluate the D4
+// response y = W z and the shell Coulomb potential gamma q as column sweeps
+// of the symmetric matrices.  The AES energy is x . v_aes / 2 (the
+// multipole energy is a symmetric bilinear form of the moments).
 QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, const double* x,
                        double* vs, double* vd, double* vq)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nat = si.nat, nsh = si.nsh;
+    const int nat = si.nat, nsh = si.nsh, nd = si.ndim;
     const double* qsh = x;
     const double* dp = x + nsh;
     const double* qp = x + nsh + 3 * nat;
+    QX_PT_INIT(pt)
     for (int a = tid; a < nat; a += QX_BLOCK) {
         double q = 0.0;
         for (int s = ss.shoff[a]; s < ss.shoff[a + 1]; ++s) q += qsh[s];
         ss.qat[a] = q;
     }
     __syncthreads();
-    const int nd = si.ndim;
     for (int k = tid; k < nd; k += QX_BLOCK) {
         int a = 0;
         while (ss.refoff[a + 1] <= k) ++a;
@@ -291,98 +400,96 @@ QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, con
         ss.dzr[k] = d4_dzeta(gfn2::kD4Ga, c, e.refq[r] + e.zeff, ss.qat[a] + e.zeff);
     }
     __syncthreads();
-    // D4 response y = W z: wavefront per row, lanes over columns (coalesced)
-    const double* WD = B.wd4 + si.d4_off;
-    double ed4 = 0.0;
-    for (int k = wave; k < nd; k += QX_BLOCK / 64) {
-        double y = 0.0;
-        for (int l = lane; l < nd; l += 64) y += WD[(size_t)k * nd + l] * ss.zr[l];
-        y = wave_sum(y);
-        if (lane == 0) {
+    QX_PT(21, pt);
+    double eaes = 0.0, eaxc = 0.0, es2 = 0.0, es3 = 0.0, ed4 = 0.0;
+    if (wave == 0) {
+        const int i = lane;
+        if (i < nat) {
+            const int nn = nat * nat;
+            const double* ag = B.aesg + (size_t)si.pair_off / QX_PAIR_STRIDE * QX_AES_COMP + i;
+            double v0 = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
+            double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
+#pragma unroll 4
+            for (int j = 0; j < nat; ++j) {
+                const double* g = ag + (size_t)j * nat;
+                const double g0 = g[0], g1 = g[nn], g2 = g[2 * nn], g3 = g[3 * nn], g4 = g[4 * nn],
+                             g5 = g[5 * nn], g6 = g[6 * nn], g7 = g[7 * nn], g8 = g[8 * nn],
+                             g9 = g[9 * nn];
+                const double qj = ss.qat[j];
+                const double m0 = dp[3 * j], m1 = dp[3 * j + 1], m2 = dp[3 * j + 2];
+                const double* t = qp + 6 * j;
+                v0 += m0 * g0 + m1 * g1 + m2 * g2 + t[0] * g3 + 2.0 * t[1] * g4 + t[2] * g5 +
+                      2.0 * t[3] * g6 + 2.0 * t[4] * g7 + t[5] * g8;
+                d0 += -qj * g0 + g9 * m0 - 3.0 * (g3 * m0 + g4 * m1 + g6 * m2);
+                d1 += -qj * g1 + g9 * m1 - 3.0 * (g4 * m0 + g5 * m1 + g7 * m2);
+                d2 += -qj * g2 + g9 * m2 - 3.0 * (g6 * m0 + g7 * m1 + g8 * m2);
+                w0 += qj * g3;
+                w1 += qj * g4;
+                w2 += qj * g5;
+                w3 += qj * g6;
+                w4 += qj * g7;
+                w5 += qj * g8;
+            }
+            w1 *= 2.0;
+            w3 *= 2.0;
+            w4 *= 2.0;
+            const ElemParams& ei = g_tab.e[ss.sel[i]];
+            const double qi = ss.qat[i];
+            const double mi0 = dp[3 * i], mi1 = dp[3 * i + 1], mi2 = dp[3 * i + 2];
+            const double* ti = qp + 6 * i;
+            eaes = 0.5 * (qi * v0 + mi0 * d0 + mi1 * d1 + mi2 * d2 + ti[0] * w0 + ti[1] * w1 +
+                          ti[2] * w2 + ti[3] * w3 + ti[4] * w4 + ti[5] * w5);
+            const double dk = ei.dkernel, qk = ei.qkernel;
+            eaxc = dk * (mi0 * mi0 + mi1 * mi1 + mi2 * mi2) +
+                   qk * (ti[0] * ti[0] + ti[2] * ti[2] + ti[5] * ti[5] +
+                         2.0 * (ti[1] * ti[1] + ti[3] * ti[3] + ti[4] * ti[4]));
+            vd[3 * i + 0] = d0 + 2.0 * dk * mi0;
+            vd[3 * i + 1] = d1 + 2.0 * dk * mi1;
+            vd[3 * i + 2] = d2 + 2.0 * dk * mi2;
+            vq[6 * i + 0] = w0 + 2.0 * qk * ti[0];
+            vq[6 * i + 1] = w1 + 4.0 * qk * ti[1];
+            vq[6 * i + 2] = w2 + 2.0 * qk * ti[2];
+            vq[6 * i + 3] = w3 + 4.0 * qk * ti[3];
+            vq[6 * i + 4] = w4 + 4.0 * qk * ti[4];
+            vq[6 * i + 5] = w5 + 2.0 * qk * ti[5];
+            ss.vat[i] = v0;
+        }
+    } else {
+        const int t0 = tid - 64, nt = QX_BLOCK - 64;
+        const double* WD = B.wd4 + si.d4_off;
+        for (int k = t0; k < nd; k += nt) {
+            double y = 0.0;
+#pragma unroll 16
+            for (int l = 0; l < nd; ++l) y += WD[(size_t)l * nd + k] * ss.zr[l];
             ss.yr[k] = y;
             ed4 += -0.5 * ss.zr[k] * y;
         }
-    }
-    __syncthreads();
-    // AES and AXC: wavefront per atom i, lanes over partners j (nat <= 64)
-    const double* ag = B.aesg + si.pair_off / 2;
-    double eaes = 0.0, eaxc = 0.0;
-    for (int i = wave; i < nat; i += QX_BLOCK / 64) {
-        const double* mi = dp + 3 * i;
-        const double* ti = qp + 6 * i;
-        double acc[11];
-#pragma unroll
-        for (int k = 0; k < 11; ++k) acc[k] = 0.0;
-        const int j = lane;
-        if (j < nat && j != i) {
-            const double* g = ag + (size_t)(i * nat + j) * 8;
-            const double rv0 = g[0], rv1 = g[1], rv2 = g[2], h3 = g[3], h5 = g[4], h5b = g[5];
-            const double* mj = dp + 3 * j;
-            const double* tj = qp + 6 * j;
-            const double mjr = mj[0] * rv0 + mj[1] * rv1 + mj[2] * rv2;
-            const double tjrr = tj[0] * rv0 * rv0 + 2 * tj[1] * rv0 * rv1 + tj[2] * rv1 * rv1 +
-                                2 * tj[3] * rv0 * rv2 + 2 * tj[4] * rv1 * rv2 + tj[5] * rv2 * rv2;
-            const double qj = ss.qat[j];
-            acc[0] = mjr * h3 + tjrr * h5;
-            acc[1] = -qj * rv0 * h3 + mj[0] * h5b - 3.0 * rv0 * mjr * h5;
-            acc[2] = -qj * rv1 * h3 + mj[1] * h5b - 3.0 * rv1 * mjr * h5;
-            acc[3] = -qj * rv2 * h3 + mj[2] * h5b - 3.0 * rv2 * mjr * h5;
-            acc[4] = qj * rv0 * rv0 * h5;
-            acc[5] = qj * 2 * rv0 * rv1 * h5;
-            acc[6] = qj * rv1 * rv1 * h5;
-            acc[7] = qj * 2 * rv0 * rv2 * h5;
-            acc[8] = qj * 2 * rv1 * rv2 * h5;
-            acc[9] = qj * rv2 * rv2 * h5;
-            if (j < i) {
-                const double mir = mi[0] * rv0 + mi[1] * rv1 + mi[2] * rv2;
-                const double mimj = mi[0] * mj[0] + mi[1] * mj[1] + mi[2] * mj[2];
-                const double tirr = ti[0] * rv0 * rv0 + 2 * ti[1] * rv0 * rv1 + ti[2] * rv1 * rv1 +
-                                    2 * ti[3] * rv0 * rv2 + 2 * ti[4] * rv1 * rv2 + ti[5] * rv2 * rv2;
-                const double qi = ss.qat[i];
-                acc[10] = qi * mjr * h3 - qj * mir * h3 + mimj * h5b - 3.0 * mir * mjr * h5 +
-                          qi * tjrr * h5 + qj * tirr * h5;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 11; ++k) acc[k] = wave_sum(acc[k]);
-        if (lane == 0) {
-            const ElemParams& ei = g_tab.e[ss.sel[i]];
-            eaes += acc[10];
-            eaxc += ei.dkernel * (mi[0] * mi[0] + mi[1] * mi[1] + mi[2] * mi[2]) +
-                    ei.qkernel * (ti[0] * ti[0] + ti[2] * ti[2] + ti[5] * ti[5] +
-                                  2.0 * (ti[1] * ti[1] + ti[3] * ti[3] + ti[4] * ti[4]));
-            for (int c = 0; c < 3; ++c) vd[3 * i + c] = acc[1 + c] + 2.0 * ei.dkernel * mi[c];
-            vq[6 * i + 0] = acc[4] + 2.0 * ei.qkernel * ti[0];
-            vq[6 * i + 1] = acc[5] + 4.0 * ei.qkernel * ti[1];
-            vq[6 * i + 2] = acc[6] + 2.0 * ei.qkernel * ti[2];
-            vq[6 * i + 3] = acc[7] + 4.0 * ei.qkernel * ti[3];
-            vq[6 * i + 4] = acc[8] + 4.0 * ei.qkernel * ti[4];
-            vq[6 * i + 5] = acc[9] + 2.0 * ei.qkernel * ti[5];
-            double vdisp = 0.0;
-            for (int k = ss.refoff[i]; k < ss.refoff[i + 1]; ++k) vdisp += -ss.dzr[k] * ss.yr[k];
-            ss.vat[i] = acc[0] + vdisp;
-        }
-    }
-    __syncthreads();
-    // ES2 + third order: wavefront per shell row of gamma
-    const double* G = B.gam + si.gam_off;
-    double es2 = 0.0, es3 = 0.0;
-    for (int a = wave; a < nsh; a += QX_BLOCK / 64) {
-        double v = 0.0;
-        for (int b = lane; b < nsh; b += 64) v += G[(size_t)a * nsh + b] * qsh[b];
-        v = wave_sum(v);
-        if (lane == 0) {
-            es2 += 0.5 * qsh[a] * v;
+        const double* G = B.gam + si.gam_off;
+        for (int a = t0; a < nsh; a += nt) {
+            double v = 0.0;
+#pragma unroll 8
+            for (int b = 0; b < nsh; ++b) v += G[(size_t)b * nsh + a] * qsh[b];
+            const double q = qsh[a];
             const double g3 = g_tab.e[ss.sh[a].el].third[ss.sh[a].k];
-            es3 += g3 * qsh[a] * qsh[a] * qsh[a] / 3.0;
-            vs[a] = v + g3 * qsh[a] * qsh[a] + ss.vat[ss.sh[a].at];
+            es2 += 0.5 * q * v;
+            es3 += g3 * q * q * q / 3.0;
+            vs[a] = v + g3 * q * q;
         }
+    }
+    __syncthreads();
+    QX_PT(22, pt);
+    for (int a = tid; a < nsh; a += QX_BLOCK) {
+        const int at = ss.sh[a].at;
+        double vdisp = 0.0;
+        for (int k = ss.refoff[at]; k < ss.refoff[at + 1]; ++k) vdisp += -ss.dzr[k] * ss.yr[k];
+        vs[a] += ss.vat[at] + vdisp;
     }
     double e5[5] = {es2, es3, eaes, eaxc, ed4};
     block_sums<5>(e5, ss.red);
     if (tid == 0)
         for (int k = 0; k < 5; ++k) ss.eparts[k] = e5[k];
     __syncthreads();
+    QX_PT(23, pt);
 }
 
 QX_DEV void load_system(const SysInfo& si, const BatchPtrs& B, SysShared& ss)
@@ -413,14 +520,15 @@ QX_DEV void load_system(const SysInfo& si, const BatchPtrs& B, SysShared& ss)
 }
 
 struct SccBuffers {
-    double *A, *V, *C, *F, *P;  // LDS or global
-    int ld;                     // leading dimension of the five buffers
+    double *A, *C, *F, *P;  // m x m working matrices (LDS or global), leading dimension ld
+    double* rs;             // kRs row sums per AO
+    int ld;
 };
 
 template <bool FAST>
 QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, SysShared& ss,
                      SccBuffers bf, double* xin, double* xout, double* vs, double* vd, double* vq,
-                     double* eps, double* focc, double* f2, double* rot, int* rpq, double* s_e,
+                     double* eps, double* focc, double* f2, double2* rot, int2* rpq, double* s_e,
                      int* s_any)
 {
     const int sys = blockIdx.x;
@@ -431,18 +539,19 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
     const size_t ms = (size_t)ld * ld;
     const double* S = B.S + si.mat_off;
     const double* H = B.H0 + si.mat_off;
-    const double* Dm = B.D + 3 * si.mat_off;
+    const double* Dm = B.D + 3 * si.mat_off;  // transposed: (mu, nu) at nu * ld + mu
     const double* Qm = B.Q + 6 * si.mat_off;
     double* xg = B.x + si.x_off;
     double* xp = B.xprev + si.x_off;
     const int warm = B.warm[sys];
     double* A = bf.A;
-    double* V = bf.V;
     double* Cc = bf.C;
     double* Ft = bf.F;
     double* P = bf.P;
+    double* rs = bf.rs;
 
-    // initial moments (extrapolated in MD) and basis C = X U
+    QX_PH_INIT
+    // initial moments (extrapolated in MD) and basis C = X U (padding column zero)
     for (int i = tid; i < nx; i += QX_BLOCK) {
         double v = 0.0;
         if (warm >= 2 && opt.extrapolate) v = 2.0 * xg[i] - xp[i];
@@ -456,7 +565,11 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         const double* X = B.X + si.mat_off;
         for (int p = tid; p < n * n; p += QX_BLOCK) Cc[(size_t)(p / n) * lb + p % n] = X[(size_t)(p / n) * ld + p % n];
     }
+    if (m > n)
+        for (int i = tid; i < n; i += QX_BLOCK) Cc[(size_t)i * lb + n] = 0.0;
+    if (tid == 0) s_e[1] = -1e300;
     __syncthreads();
+    QX_PH(11);
 
     double* hist = B.hist + si.hist_off;
     const int mem = opt.mem;
@@ -470,22 +583,33 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
     double* hgam = hbeta + (size_t)mem * mem;
 
     const double nalpha = 0.5 * si.nel;
-    double eold = 0.0, etot = 0.0, ts = 0.0, eband = 0.0;
+    double eold = 0.0, etot = 0.0, ts = 0.0, eband = 0.0, errp = 1e-3;
     int it = 1, conv = 0;
     for (; it <= opt.maxiter; ++it) {
         potentials(si, B, ss, xin, vs, vd, vq);
-        // G = row-local potential part of the Fock matrix
-        for (int mu = wave; mu < n; mu += QX_BLOCK / 64) {
+        QX_PH(0);
+        // G^T (row nu holds G_{mu nu} over mu), then F = H0 - (G + G^T) / 2
+        for (int mu = lane; mu < n; mu += 64) {
             const int am = ss.ao_at[mu];
-            const double v0 = vs[ss.ao_sh[mu]];
-            const double d0 = vd[3 * am], d1 = vd[3 * am + 1], d2 = vd[3 * am + 2];
+            const double w0 = vs[ss.ao_sh[mu]], w1 = vd[3 * am], w2 = vd[3 * am + 1],
+                         w3 = vd[3 * am + 2];
             const double* q = vq + 6 * am;
-            for (int nu = lane; nu < n; nu += 64) {
-                const size_t o = (size_t)mu * ld + nu;
-                const double g = S[o] * v0 + Dm[o] * d0 + Dm[ms + o] * d1 + Dm[2 * ms + o] * d2 +
-                                 Qm[o] * q[0] + Qm[ms + o] * q[1] + Qm[2 * ms + o] * q[2] +
-                                 Qm[3 * ms + o] * q[3] + Qm[4 * ms + o] * q[4] + Qm[5 * ms + o] * q[5];
-                Ft[(size_t)mu * lb + nu] = g;
+            const double w4 = q[0], w5 = q[1], w6 = q[2], w7 = q[3], w8 = q[4], w9 = q[5];
+            for (int nu0 = wave; nu0 < n; nu0 += 4 * (QX_BLOCK / 64)) {
+                double g[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int nu = nu0 + u * (QX_BLOCK / 64);
+                    const size_t o = (size_t)(nu < n ? nu : 0) * ld + mu;
+                    g[u] = S[o] * w0 + Dm[o] * w1 + Dm[ms + o] * w2 + Dm[2 * ms + o] * w3 +
+                           Qm[o] * w4 + Qm[ms + o] * w5 + Qm[2 * ms + o] * w6 +
+                           Qm[3 * ms + o] * w7 + Qm[4 * ms + o] * w8 + Qm[5 * ms + o] * w9;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int nu = nu0 + u * (QX_BLOCK / 64);
+                    if (nu < n) Ft[(size_t)nu * lb + mu] = g[u];
+                }
             }
         }
         __syncthreads();
@@ -498,78 +622,73 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
             Ft[(size_t)nu * lb + mu] = f;
         }
         __syncthreads();
-        // A = C^T F C
+        QX_PH(1);
+        // A = C^T F C (F C staged in the density buffer)
         block_gemm<false, false>(n, n, n, Ft, lb, Cc, lb, P, lb, nullptr);
         __syncthreads();
         block_gemm<true, false>(n, n, n, Cc, lb, P, lb, A, lb, nullptr);
-        for (int p = tid; p < m * m; p += QX_BLOCK) {
-            const int i = p / m, j = p % m;
-            V[(size_t)i * lb + j] = (i == j) ? 1.0 : 0.0;
-            if (i >= n || j >= n) A[(size_t)i * lb + j] = 0.0;
-        }
+        if (m > n)
+            for (int j = tid; j < m; j += QX_BLOCK) {
+                A[(size_t)n * lb + j] = 0.0;
+                A[(size_t)j * lb + n] = 0.0;
+            }
         __syncthreads();
-        jacobi(A, V, m, lb, opt.jacobi_tol, 50, rot, rpq, ss.red, s_any);
+        QX_PH(2);
+        double jtol = fmin(1e-6, 1e-3 * errp);
+        if (jtol < opt.jacobi_tol) jtol = opt.jacobi_tol;
+        const int nsweep = jacobi(A, Cc, m, n, lb, jtol, 50, rot, rpq, ss.red, s_any);
+        QX_PH(3);
+        QX_PH_ADD(12, nsweep);
+        (void)nsweep;
         for (int i = tid; i < n; i += QX_BLOCK) eps[i] = A[(size_t)i * lb + i];
         __syncthreads();
         if (tid < 64) {
-            const double t = fermi_wave(n, eps, si.kt, nalpha, focc);
-            if (tid == 0) s_e[0] = t;
+            double mu = s_e[1];
+            int fit = 0;
+            const double t = fermi_wave(n, eps, si.kt, nalpha, focc, mu, fit);
+            if (tid == 0) {
+                s_e[0] = t;
+                s_e[1] = mu;
+            }
+            QX_PH_ADD(14, fit);
+            (void)fit;
+            for (int i = tid; i < n; i += 64) f2[i] = 2.0 * focc[i];
         }
-        // C <- C V (into the Fock buffer, then swap)
-        block_gemm<false, false>(n, n, n, Cc, lb, V, lb, Ft, lb, nullptr);
         __syncthreads();
         ts = s_e[0];
-        {
-            double* t = Cc;
-            Cc = Ft;
-            Ft = t;
-        }
-        for (int i = tid; i < n; i += QX_BLOCK) f2[i] = 2.0 * focc[i];
-        __syncthreads();
+        QX_PH(4);
         block_gemm<false, true>(n, n, n, Cc, lb, Cc, lb, P, lb, f2);
         __syncthreads();
-        // output moments: 10 row sums per AO row (wavefront per row), into A
-        double* rs = A;
-        for (int mu = wave; mu < n; mu += QX_BLOCK / 64) {
-            double acc[10];
-#pragma unroll
-            for (int k = 0; k < 10; ++k) acc[k] = 0.0;
-            for (int nu = lane; nu < n; nu += 64) {
-                const size_t o = (size_t)mu * ld + nu;
-                const double pm = P[(size_t)mu * lb + nu];
-                acc[0] += pm * S[o];
-                acc[1] += pm * Dm[o];
-                acc[2] += pm * Dm[ms + o];
-                acc[3] += pm * Dm[2 * ms + o];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) acc[4 + k] += pm * Qm[k * ms + o];
-            }
-#pragma unroll
-            for (int k = 0; k < 10; ++k) acc[k] = wave_sum(acc[k]);
-            if (lane == 0)
-                for (int k = 0; k < 10; ++k) rs[(size_t)mu * 10 + k] = acc[k];
+        QX_PH(5);
+        // row sums of P o M for M = S, D (3), Q (6), H0 as coalesced column sweeps
+        for (int t = tid; t < kRs * n; t += QX_BLOCK) {
+            const int c = t / n, mu = t - c * n;
+            const double* M = c == 0 ? S : c < 4 ? Dm + (size_t)(c - 1) * ms : c < 10 ? Qm + (size_t)(c - 4) * ms : H;
+            double acc = 0.0;
+#pragma unroll 8
+            for (int nu = 0; nu < n; ++nu) acc += P[(size_t)nu * lb + mu] * M[(size_t)nu * ld + mu];
+            rs[(size_t)mu * kRs + c] = acc;
         }
         __syncthreads();
         for (int s = tid; s < nsh; s += QX_BLOCK) {
             const ShellDesc d = ss.sh[s];
             double pop = 0.0;
-            for (int a = 0; a < d.nao; ++a) pop += rs[(size_t)(d.ao0 + a) * 10];
+            for (int a = 0; a < d.nao; ++a) pop += rs[(size_t)(d.ao0 + a) * kRs];
             xout[s] = g_tab.e[d.el].refocc[d.k] - pop;
         }
         for (int t = tid; t < 9 * nat; t += QX_BLOCK) {
             const int a = t / 9, c = t % 9;
             double v = 0.0;
-            for (int mu = ss.aooff[a]; mu < ss.aooff[a + 1]; ++mu) v += rs[(size_t)mu * 10 + 1 + c];
+            for (int mu = ss.aooff[a]; mu < ss.aooff[a + 1]; ++mu) v += rs[(size_t)mu * kRs + 1 + c];
             if (c < 3) xout[nsh + 3 * a + c] = -v;
             else xout[nsh + 3 * nat + 6 * a + (c - 3)] = -v;
         }
-        __syncthreads();
-        potentials(si, B, ss, xout, vs, vd, vq);
         double red2[2] = {0.0, 0.0};
-        for (int p = tid; p < n * n; p += QX_BLOCK) {
-            const int i = p / n, j = p % n;
-            red2[0] += P[(size_t)i * lb + j] * H[(size_t)i * ld + j];
-        }
+        for (int mu = tid; mu < n; mu += QX_BLOCK) red2[0] += rs[(size_t)mu * kRs + 10];
+        __syncthreads();
+        QX_PH(6);
+        potentials(si, B, ss, xout, vs, vd, vq);
+        QX_PH(7);
         for (int i = tid; i < nx; i += QX_BLOCK) {
             const double dd = xout[i] - xin[i];
             red2[1] += dd * dd;
@@ -578,6 +697,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         eband = red2[0];
         etot = eband + ss.eparts[0] + ss.eparts[1] + ss.eparts[2] + ss.eparts[3] + ss.eparts[4] + ts;
         const double err = sqrt(red2[1] / nx);
+        errp = err;
         const double de = etot - eold;
         eold = etot;
         if (it > 1 && fabs(de) < opt.econv && err < opt.pconv) {
@@ -585,6 +705,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
             break;
         }
         if (it == opt.maxiter) break;
+        QX_PH(8);
         // modified Broyden update of xin with dq = xout - xin (stored into xout)
         if (it == 1) {
             for (int i = tid; i < nx; i += QX_BLOCK) {
@@ -594,6 +715,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
                 xin[i] = xin[i] + opt.damp * dq;
             }
             __syncthreads();
+            QX_PH(9);
             continue;
         }
         int it1 = it - 1;
@@ -639,34 +761,49 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
             }
         }
         __syncthreads();
+        // (w0^2 I + diag(om) a diag(om)) gamma = diag(om) c, Cholesky in LDS
+        // (the Fock buffer is free here) when it fits, else in global memory
+        const bool lds_solve = it1 * (it1 + 2) <= m * m;
+        double* bt = lds_solve ? Ft : hbeta;
+        const int lbt = lds_solve ? it1 : mem;
+        double* gm = lds_solve ? Ft + (size_t)it1 * it1 : hgam;
+        double* om = lds_solve ? gm + it1 : hom;
+        if (lds_solve) {
+            for (int i = tid; i < it1; i += QX_BLOCK) {
+                om[i] = hom[i];
+                gm[i] = hgam[i];
+            }
+            __syncthreads();
+        }
+        for (int p = tid; p < it1 * it1; p += QX_BLOCK) {
+            const int i = p / it1, j = p % it1;
+            bt[(size_t)i * lbt + j] = om[i] * om[j] * ha[(size_t)i * mem + j] + (i == j ? 1e-4 : 0.0);
+        }
+        __syncthreads();
         if (tid == 0) {
-            const double w0 = 0.01;
-            for (int i = 0; i < it1; ++i)
-                for (int j = 0; j < it1; ++j)
-                    hbeta[(size_t)i * mem + j] = hom[i] * hom[j] * ha[(size_t)i * mem + j] + (i == j ? w0 * w0 : 0.0);
-            for (int i = 0; i < it1; ++i) hgam[i] *= hom[i];
+            for (int i = 0; i < it1; ++i) gm[i] *= om[i];
             for (int j = 0; j < it1; ++j) {
-                double d = hbeta[(size_t)j * mem + j];
-                for (int k = 0; k < j; ++k) d -= hbeta[(size_t)j * mem + k] * hbeta[(size_t)j * mem + k];
+                double d = bt[(size_t)j * lbt + j];
+                for (int k = 0; k < j; ++k) d -= bt[(size_t)j * lbt + k] * bt[(size_t)j * lbt + k];
                 d = sqrt(d);
-                hbeta[(size_t)j * mem + j] = d;
+                bt[(size_t)j * lbt + j] = d;
                 for (int i = j + 1; i < it1; ++i) {
-                    double v = hbeta[(size_t)i * mem + j];
-                    for (int k = 0; k < j; ++k) v -= hbeta[(size_t)i * mem + k] * hbeta[(size_t)j * mem + k];
-                    hbeta[(size_t)i * mem + j] = v / d;
+                    double v = bt[(size_t)i * lbt + j];
+                    for (int k = 0; k < j; ++k) v -= bt[(size_t)i * lbt + k] * bt[(size_t)j * lbt + k];
+                    bt[(size_t)i * lbt + j] = v / d;
                 }
             }
             for (int i = 0; i < it1; ++i) {
-                double v = hgam[i];
-                for (int k = 0; k < i; ++k) v -= hbeta[(size_t)i * mem + k] * hgam[k];
-                hgam[i] = v / hbeta[(size_t)i * mem + i];
+                double v = gm[i];
+                for (int k = 0; k < i; ++k) v -= bt[(size_t)i * lbt + k] * gm[k];
+                gm[i] = v / bt[(size_t)i * lbt + i];
             }
             for (int i = it1 - 1; i >= 0; --i) {
-                double v = hgam[i];
-                for (int k = i + 1; k < it1; ++k) v -= hbeta[(size_t)k * mem + i] * hgam[k];
-                hgam[i] = v / hbeta[(size_t)i * mem + i];
+                double v = gm[i];
+                for (int k = i + 1; k < it1; ++k) v -= bt[(size_t)k * lbt + i] * gm[k];
+                gm[i] = v / bt[(size_t)i * lbt + i];
             }
-            for (int i = 0; i < it1; ++i) hgam[i] *= hom[i];
+            for (int i = 0; i < it1; ++i) gm[i] *= om[i];
         }
         __syncthreads();
         for (int j = tid; j < nx; j += QX_BLOCK) {
@@ -675,11 +812,13 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
             hdql[j] = dq;
             hql[j] = q;
             q += opt.damp * dq;
-            for (int i = 0; i < it1; ++i) q -= hgam[i] * hu[(size_t)i * nx + j];
+            for (int i = 0; i < it1; ++i) q -= gm[i] * hu[(size_t)i * nx + j];
             xin[j] = q;
         }
         __syncthreads();
+        QX_PH(9);
     }
+    QX_PH_ADD(13, it > opt.maxiter ? opt.maxiter : it);
     // results: moments, charges, potentials at the output moments
     for (int i = tid; i < nx; i += QX_BLOCK) {
         xg[i] = xout[i];
@@ -701,6 +840,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
     // W = C diag(2 f e) C^T ; U = L^T C for the next warm start
     block_gemm<false, true>(n, n, n, Cc, lb, Cc, lb, B.W + si.mat_off, ld, f2);
     block_gemm<true, false>(n, n, n, B.L + si.mat_off, ld, Cc, lb, B.U + si.mat_off, ld, nullptr);
+    QX_PH(10);
     if (tid == 0) {
         double* ep = B.eparts + 10 * sys;
         ep[0] = eband;
@@ -729,20 +869,22 @@ __global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
     __shared__ double xin[kMaxNx], xout[kMaxNx], vs[QX_MAX_NSH], vd[3 * QX_MAX_NAT],
         vq[6 * QX_MAX_NAT];
     __shared__ double eps[QX_MAX_NAO], focc[QX_MAX_NAO], f2[QX_MAX_NAO];
-    __shared__ double rot[QX_MAX_NAO];
-    __shared__ int rpq[QX_MAX_NAO];
+    __shared__ double2 rot[QX_MAX_NAO / 2];
+    __shared__ int2 rpq[QX_MAX_NAO / 2];
     __shared__ double s_e[4];
     __shared__ int s_any[2];
     extern __shared__ double dyn[];
     load_system(si, B, ss);
     const int m = (si.nao + 1) & ~1;
     if (m <= kFastM) {
-        const size_t mm = (size_t)m * m;
-        SccBuffers bf{dyn, dyn + mm, dyn + 2 * mm, dyn + 3 * mm, dyn + 4 * mm, m};
+        // odd leading dimension: column-wise LDS accesses hit distinct banks
+        const size_t mm = (size_t)m * (m + 1);
+        SccBuffers bf{dyn, dyn + mm, dyn + 2 * mm, dyn + 3 * mm, dyn + 4 * mm, m + 1};
         scc_body<true>(si, B, opt, ss, bf, xin, xout, vs, vd, vq, eps, focc, f2, rot, rpq, s_e, s_any);
     } else {
-        SccBuffers bf{B.W + si.mat_off, B.U + si.mat_off, B.C + si.mat_off, B.T + si.mat_off,
-                      B.P + si.mat_off, si.ld};
+        // the energy-weighted density slot doubles as the row-sum scratch
+        SccBuffers bf{B.T + si.mat_off, B.C + si.mat_off, B.U + si.mat_off, B.P + si.mat_off,
+                      B.W + si.mat_off, si.ld};
         scc_body<false>(si, B, opt, ss, bf, xin, xout, vs, vd, vq, eps, focc, f2, rot, rpq, s_e,
                         s_any);
     }

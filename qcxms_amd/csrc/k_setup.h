@@ -231,13 +231,15 @@ __global__ __launch_bounds__(QX_BLOCK) void k_setup(BatchPtrs B, int with_chol)
                     H[o1] = hs * s;
                     H[o2] = hs * s;
                 }
+                // moment integrals are stored transposed: element (mu, nu),
+                // taken about the atom of mu, sits at nu * ld + mu
                 for (int c = 0; c < 3; ++c) {
-                    D[c * ms + o1] = dpa[c];
-                    D[c * ms + o2] = dpb[c];
+                    D[c * ms + o2] = dpa[c];
+                    D[c * ms + o1] = dpb[c];
                 }
                 for (int k = 0; k < 6; ++k) {
-                    Q[k * ms + o1] = qta[k];
-                    Q[k * ms + o2] = qtb[k];
+                    Q[k * ms + o2] = qta[k];
+                    Q[k * ms + o1] = qtb[k];
                 }
             }
     }
@@ -277,30 +279,36 @@ __global__ __launch_bounds__(QX_BLOCK) void k_setup(BatchPtrs B, int with_chol)
                     disp * gw[k] * gw[l] * g_tab.c6ref[sel[i]][ii][sel[j]][jj];
             }
     }
-    // 6b. geometric factors of the damped multipole kernels (AES2), ordered pairs
-    double* ag = B.aesg + si.pair_off / 2;
-    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
-        const int i = p / nat, j = p % nat;
-        double* o = ag + (size_t)p * 8;
-        if (i == j) {
-            for (int k = 0; k < 8; ++k) o[k] = 0.0;
-            continue;
+    // 6b. damped multipole kernels (AES2) per ordered pair (i, j), rv = R_i - R_j,
+    //     stored as planes c * nat^2 + j * nat + i:
+    //     h3 rv (3), h5 rv_a rv_b (xx, xy, yy, xz, yz, zz), r^2 h5
+    double* ag = B.aesg + (size_t)si.pair_off / QX_PAIR_STRIDE * QX_AES_COMP;
+    const int nn = nat * nat;
+    for (int p = tid; p < nn; p += QX_BLOCK) {
+        const int j = p / nat, i = p % nat;
+        double o[QX_AES_COMP];
+        for (int k = 0; k < QX_AES_COMP; ++k) o[k] = 0.0;
+        if (i != j) {
+            const double rv0 = sx[3 * i] - sx[3 * j], rv1 = sx[3 * i + 1] - sx[3 * j + 1],
+                         rv2 = sx[3 * i + 2] - sx[3 * j + 2];
+            const double r2 = rv0 * rv0 + rv1 * rv1 + rv2 * rv2;
+            const double r = sqrt(r2);
+            const double r0 = 0.5 * (B.mrad[si.atom_off + i] + B.mrad[si.atom_off + j]);
+            const double r03 = r0 * r0 * r0;
+            const double h3 = 1.0 / (r2 * r + 6.0 * r03);
+            const double h5 = 1.0 / (r * (r2 * r2 + 6.0 * r03 * r0));
+            o[0] = h3 * rv0;
+            o[1] = h3 * rv1;
+            o[2] = h3 * rv2;
+            o[3] = h5 * rv0 * rv0;
+            o[4] = h5 * rv0 * rv1;
+            o[5] = h5 * rv1 * rv1;
+            o[6] = h5 * rv0 * rv2;
+            o[7] = h5 * rv1 * rv2;
+            o[8] = h5 * rv2 * rv2;
+            o[9] = r2 * h5;
         }
-        const double rv0 = sx[3 * i] - sx[3 * j], rv1 = sx[3 * i + 1] - sx[3 * j + 1],
-                     rv2 = sx[3 * i + 2] - sx[3 * j + 2];
-        const double r2 = rv0 * rv0 + rv1 * rv1 + rv2 * rv2;
-        const double r = sqrt(r2);
-        const double r0 = 0.5 * (B.mrad[si.atom_off + i] + B.mrad[si.atom_off + j]);
-        const double r03 = r0 * r0 * r0;
-        const double h5 = 1.0 / (r * (r2 * r2 + 6.0 * r03 * r0));
-        o[0] = rv0;
-        o[1] = rv1;
-        o[2] = rv2;
-        o[3] = 1.0 / (r2 * r + 6.0 * r03);
-        o[4] = h5;
-        o[5] = r2 * h5;
-        o[6] = 0.0;
-        o[7] = 0.0;
+        for (int k = 0; k < QX_AES_COMP; ++k) ag[(size_t)k * nn + p] = o[k];
     }
     // 7. repulsion energy
     double erep = 0.0;
