@@ -44,14 +44,20 @@ def workload(ntraj):
                          tadd_fs=100.0)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_scc_hbm_traffic_c8h18_256.txt")
+
+
 def traffic_from_profiles():
-    """HBM bytes per SCC launch measured with rocprofv3 counters (if recorded)."""
-    p = os.path.join(ROOT, "profiles", "scc_traffic.json")
-    if os.path.exists(p):
-        try:
-            return float(json.load(open(p))["bytes_per_launch"])
-        except Exception:
-            return None
+    """HBM bytes per SCC launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a plain-copy calibration), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("bytes_per_launch"):
+            try:
+                return float(line.split("=")[1].split()[0])
+            except (IndexError, ValueError):
+                return None
     return None
 
 
@@ -157,6 +163,20 @@ def run_cpu_baseline(steps):
                        f"one trajectory per thread; not QCxMS+tblite), wall {wall:.1f} s")
 
 
+def copy_calibration(mib=256, reps=5):
+    """A plain device copy of known size (for scaling the HBM traffic counters
+    FETCH_SIZE / WRITE_SIZE): reps copies of mib MiB."""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    x = torch.arange(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "reps": reps}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,12 +186,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=25)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-one-per-launch", action="store_true")
+    ap.add_argument("--no-graph-check", action="store_true",
+                    help="skip the hipGraph replay of the same steps")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs every trajectory on a single "
               "MI355X and has no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    g = run_gpu(args.ntraj, args.steps, args.warmup)
+    g = run_gpu(args.ntraj, args.steps, args.warmup, use_graph_check=not args.no_graph_check)
     line = {
         "metric": METRIC,
         "value": g["value"],

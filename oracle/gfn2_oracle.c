@@ -1459,10 +1459,8 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
     }
     if (it > opts.maxiter) it = opts.maxiter;
     /* final state: moments xo from P; potentials vs/vd/vq at xo (computed above) */
-    double eatm_dummy = 0.0;
     double *dedcn_d4 = (double *)calloc(nat, sizeof(double));
     double eatm = d4_atm(&y, grad ? grad : NULL, dedcn_d4);
-    (void)eatm_dummy;
     double eband = 0.0;
     for (int k = 0; k < n * n; ++k) eband += P[k] * y.H0[k];
     double etotal = etot + eatm + y.e_rep;
@@ -1563,7 +1561,7 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
                 double r3 = r * r * r, r4 = r3 * r, r04 = r0 * r0 * r0 * r0;
                 double dh3 = -3.0 * r * r * h3 * h3;            /* dh3/dr */
                 double dh5 = -(5.0 * r4 + 6.0 * r04) * h5 * h5;  /* dh5/dr */
-                double h5b = r * r * h5, dh5b = 2.0 * r * h5 + r * r * dh5;
+                double dh5b = 2.0 * r * h5 + r * r * dh5;
                 double dh3r0 = -18.0 * r0 * r0 * h3 * h3;
                 double dh5r0 = -24.0 * r0 * r0 * r0 * r * h5 * h5;
                 double dh5br0 = r * r * dh5r0;
@@ -1653,7 +1651,7 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
                     for (int b = 0; b < sb->nao; ++b) {
                         int mu = sa->ao0 + a, nu = sb->ao0 + b, o = a * 3 + b;
                         double pmn = P[(size_t)mu * n + nu], wmn = W[(size_t)mu * n + nu];
-                        double ws = 2.0 * pmn * hav * scal - 2.0 * wmn - pmn * (xo ? 1.0 : 1.0) * (vs[ia] + vs[ib]);
+                        double ws = 2.0 * pmn * hav * scal - 2.0 * wmn - pmn * (vs[ia] + vs[ib]);
                         psum += pmn * s[o];
                         for (int d = 0; d < 3; ++d) {
                             double g = ws * ds[d * 9 + o];

@@ -5,9 +5,9 @@ in include/qcxms_gpu.h; this package is its Python host layer, mirroring the
 reference's Fortran interface (src/tblite.f90 get_xtb_egrad, src/md.f90).
 """
 from .gpu import (EXIT_REASONS, GFN1_XTB, GFN2_XTB, IPEA1_XTB, Batch, LibraryMissing,
-                  MdEnsemble, egrad_batch, fragments_batch, get_xtb_egrad, impactscale_batch,
+                  MdEnsemble, egrad_batch, egrad_many, fragments_batch, get_xtb_egrad, impactscale_batch,
                   leapfrog_batch, library, library_path)
 
 __all__ = ["EXIT_REASONS", "GFN1_XTB", "GFN2_XTB", "IPEA1_XTB", "Batch", "LibraryMissing",
-           "MdEnsemble", "egrad_batch", "fragments_batch", "get_xtb_egrad", "impactscale_batch",
+           "MdEnsemble", "egrad_batch", "egrad_many", "fragments_batch", "get_xtb_egrad", "impactscale_batch",
            "leapfrog_batch", "library", "library_path"]

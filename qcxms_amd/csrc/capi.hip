@@ -199,6 +199,7 @@ struct qcxms_batch {
     int nsys = 0;
     std::vector<SysInfo> info;
     std::vector<int> elem;
+    std::vector<int2> items;  // shell-pair work items of all systems
     int natoms = 0, nshells = 0, nrefs = 0, nxtot = 0;
     size_t mat_total = 0, gam_total = 0, d4_total = 0, hist_total = 0, pair_total = 0;
     int max_ld = 0, max_m = 0;
@@ -273,6 +274,30 @@ int layout(qcxms_batch* b, int nsys, const int32_t* offsets, const int32_t* num,
         si.d4_off = d4;
         si.hist_off = hist;
         si.pair_off = pair;
+        // shell-pair work items: off-atom pairs (atom of ia > atom of ib)
+        // grouped p-p, p-s, s-p, s-s so that wavefronts run one code path,
+        // then the same-atom pairs (ia >= ib)
+        {
+            std::vector<int> sh_at, sh_l;
+            for (int i = 0; i < nat; ++i) {
+                const int e = b->elem[b->elem.size() - nat + i];
+                for (int k = 0; k < gfn2::kNsh[e]; ++k) {
+                    sh_at.push_back(i);
+                    sh_l.push_back(k);
+                }
+            }
+            si.item_off = (int)b->items.size();
+            for (int cls = 3; cls >= 0; --cls)
+                for (int ia = 0; ia < nsh; ++ia)
+                    for (int ib = 0; ib < ia; ++ib)
+                        if (sh_at[ia] > sh_at[ib] && sh_l[ia] * 2 + sh_l[ib] == cls)
+                            b->items.push_back(make_int2(ia, ib));
+            si.nitem_off = (int)b->items.size() - si.item_off;
+            for (int ia = 0; ia < nsh; ++ia)
+                for (int ib = 0; ib <= ia; ++ib)
+                    if (sh_at[ia] == sh_at[ib]) b->items.push_back(make_int2(ia, ib));
+            si.nitem = (int)b->items.size() - si.item_off;
+        }
         si.charge = charge ? charge[s] : 0;
         si.nel = nel - si.charge;
         si.kt = 300.0 * gfn2::kKtoAu;
@@ -308,6 +333,12 @@ int allocate(qcxms_batch* b)
     if ((st = dalloc(b, &(ptr), (size_t)(n))) != 0) return st;
     ALLOC(b->d_info, b->nsys);
     ALLOC(b->d_elem, b->natoms);
+    int2* d_items = nullptr;
+    ALLOC(d_items, b->items.size());
+    if (!b->items.empty())
+        HIPCHK(hipMemcpyAsync(d_items, b->items.data(), sizeof(int2) * b->items.size(), hipMemcpyHostToDevice,
+                              g_stream));
+    d.spitem = d_items;
     HIPCHK(hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * b->nsys, hipMemcpyHostToDevice, g_stream));
     HIPCHK(hipMemcpyAsync(b->d_elem, b->elem.data(), sizeof(int) * b->natoms, hipMemcpyHostToDevice, g_stream));
     d.nsys = b->nsys;
@@ -346,6 +377,7 @@ int allocate(qcxms_batch* b)
     ALLOC(d.focc, (size_t)b->nsys * QX_MAX_NAO);
     ALLOC(d.hist, b->hist_total);
     ALLOC(d.pairbuf, b->pair_total);
+    ALLOC(d.spbuf, 5 * b->gam_total);
     ALLOC(d.aesg, b->pair_total / QX_PAIR_STRIDE * QX_AES_COMP);
     ALLOC(d.xprev, b->nxtot);
     ALLOC(d.energy, b->nsys);
@@ -729,15 +761,57 @@ int qcxms_egrad_batch(int32_t nsys, const int32_t* offsets, const int32_t* num, 
             for (int s = 0; s < nsys; ++s) stat[s] = QCXMS_UNKNOWN_METHOD;
         return QCXMS_UNKNOWN_METHOD;
     }
-    qcxms_batch* b = qcxms_batch_create(nsys, offsets, num, charge, method);
+    if (!num || !xyz) {
+        g_err = "null argument";
+        return QCXMS_INVALID;
+    }
+    if (check_offsets(nsys, offsets, QX_MAX_NAT) != QCXMS_OK) {
+        if (stat)
+            for (int s = 0; s < nsys; ++s) stat[s] = QCXMS_INVALID;
+        return QCXMS_INVALID;
+    }
+    // Launch order: largest basis first, so the longest workgroups start first
+    // and the small systems fill the tail (longest-processing-time order).
+    std::vector<int> order(nsys), nao(nsys);
+    for (int s = 0; s < nsys; ++s) {
+        order[s] = s;
+        int n = 0;
+        for (int i = offsets[s]; i < offsets[s + 1]; ++i) n += num[i] == 1 ? 1 : 4;
+        nao[s] = n;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return nao[a] > nao[c]; });
+    std::vector<int32_t> po(nsys + 1, 0), pn, pc(nsys);
+    std::vector<double> px, pe(nsys);
+    for (int k = 0; k < nsys; ++k) {
+        const int s = order[k];
+        po[k + 1] = po[k] + (offsets[s + 1] - offsets[s]);
+        pn.insert(pn.end(), num + offsets[s], num + offsets[s + 1]);
+        px.insert(px.end(), xyz + 3 * (size_t)offsets[s], xyz + 3 * (size_t)offsets[s + 1]);
+        pc[k] = charge ? charge[s] : 0;
+        pe[k] = etemp ? etemp[s] : 300.0;
+    }
+    qcxms_batch* b = qcxms_batch_create(nsys, po.data(), pn.data(), pc.data(), method);
     if (!b) {
         const bool elem = g_err.find("no GFN2 parameters") != std::string::npos;
         if (stat)
             for (int s = 0; s < nsys; ++s) stat[s] = elem ? QCXMS_UNKNOWN_METHOD : QCXMS_INVALID;
         return elem ? QCXMS_UNKNOWN_METHOD : QCXMS_INVALID;
     }
-    const int st = qcxms_batch_egrad(b, xyz, etemp, energy, gradient, qat, stat);
+    const int natoms = po[nsys];
+    std::vector<double> e(nsys), g(3 * (size_t)natoms), q(natoms);
+    std::vector<int32_t> ps(nsys);
+    const int st = qcxms_batch_egrad(b, px.data(), pe.data(), e.data(), g.data(), q.data(), ps.data());
     qcxms_batch_destroy(b);
+    for (int k = 0; k < nsys; ++k) {
+        const int s = order[k];
+        const int na = po[k + 1] - po[k];
+        if (energy) energy[s] = e[k];
+        if (stat) stat[s] = ps[k];
+        if (qat) std::copy(q.begin() + po[k], q.begin() + po[k] + na, qat + offsets[s]);
+        if (gradient)
+            std::copy(g.begin() + 3 * (size_t)po[k], g.begin() + 3 * (size_t)(po[k] + na),
+                      gradient + 3 * (size_t)offsets[s]);
+    }
     return st;
 }
 

@@ -47,8 +47,11 @@ struct SysInfo {
     long long gam_off;   // nsh*nsh
     long long d4_off;    // ndim*ndim
     long long hist_off;  // Broyden history: 2*mem*nx + mem*mem
-    long long pair_off;  // nat*nat*8 gradient scratch
-    int charge, pad0;
+    long long pair_off;  // nat*nat*QX_PAIR_STRIDE gradient scratch
+    int item_off;        // shell-pair work items: off-atom pairs first (grouped by
+    int nitem_off;       // angular-momentum class), then same-atom pairs
+    int nitem;
+    int charge;
     double nel, kt;
 };
 
@@ -76,8 +79,8 @@ struct BatchPtrs {
     double* dgw;      // per D4 reference
     double* S;        // ld^2 per system
     double* H0;
-    double* D;        // 3 ld^2 per system (dipole integrals, origin at row atom)
-    double* Q;        // 6 ld^2 per system (traceless quadrupole, origin at row atom)
+    double* D;        // 3 ld^2 per system: dipole integral (mu, nu) about mu's atom at nu*ld+mu
+    double* Q;        // 6 ld^2 per system: traceless quadrupole, same layout
     double* L;        // Cholesky factor of S (lower)
     double* X;        // L^-T (upper): X^T S X = 1
     double* C;        // eigenvectors (AO basis)
@@ -96,6 +99,8 @@ struct BatchPtrs {
     double* focc;     // occupations, ld per system
     double* hist;     // Broyden history
     double* pairbuf;  // gradient scratch
+    const int2* spitem;  // shell-pair work items (ia, ib), ia > ib or same shell
+    double* spbuf;    // per shell pair (ia, ib): gradient on A (3), dE/dCN_A, dE/dCN_B
     double* aesg;     // multipole-kernel factors, QX_AES_COMP planes of nat x nat per system
     double* xprev;    // mixing vector of the step before (extrapolated MD start)
     double* energy;   // per system: total

@@ -16,6 +16,33 @@ namespace qx {
 
 constexpr int kPairStride = QX_PAIR_STRIDE;
 
+// Density-weighted derivative moments of one (L_A, L_B) shell pair
+template <int LA, int LB>
+QX_DEV void grad_shell_pair(const ElemParams& ea, int ka, const ElemParams& eb, int kb,
+                            const double* ra, const double* rb, const double* P, const double* W,
+                            int ld, int mu0, int nu0, DerivAcc& acc)
+{
+    constexpr int NA = LA ? 3 : 1, NB = LB ? 3 : 1;
+    double pm[9], wm[9];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const size_t o = (size_t)(mu0 + a) * ld + nu0 + b;
+            pm[a * 3 + b] = P[o];
+            wm[a * 3 + b] = W[o];
+        }
+    const int nga = ea.nprim[ka], ngb = eb.nprim[kb];
+    for (int ip = 0; ip < nga; ++ip) {
+        const double al = ea.alpha[ka][ip], ca = ea.coeff[ka][ip];
+        for (int jp = 0; jp < ngb; ++jp) {
+            PrimPair q;
+            prim_pair(al, eb.alpha[kb][jp], ca * eb.coeff[kb][jp], ra, rb, q);
+            shell_derivs<LA, LB>(q, pm, wm, acc);
+        }
+    }
+}
+
 __global__ __launch_bounds__(QX_BLOCK) void k_grad(BatchPtrs B)
 {
     const int sys = blockIdx.x;
@@ -77,78 +104,74 @@ __global__ __launch_bounds__(QX_BLOCK) void k_grad(BatchPtrs B)
     }
     __syncthreads();
 
-    // Phase A: integral derivative blocks for unordered atom pairs A > B
-    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
-        const int A = p / nat, Bt = p % nat;
-        if (Bt >= A) continue;
+    // Phase A: integral derivative blocks, one work item per off-atom shell
+    // pair (atom of ia > atom of ib), grouped by angular-momentum class.
+    // The AO-pair density and energy-weighted density enter the primitive
+    // sums; the potentials are contracted once per shell pair.
+    double* spb = B.spbuf + 5 * si.gam_off;
+    for (int it = tid; it < si.nitem_off; it += QX_BLOCK) {
+        const int2 pr = B.spitem[si.item_off + it];
+        const int ia = pr.x, ib = pr.y;
+        const ShellDesc sa = ss.sh[ia], sb = ss.sh[ib];
+        const int A = sa.at, Bt = sb.at;
+        const ElemParams& ea = g_tab.e[sa.el];
+        const ElemParams& eb = g_tab.e[sb.el];
         const double* ra = ss.sx + 3 * A;
         const double* rb = ss.sx + 3 * Bt;
         const double t[3] = {rb[0] - ra[0], rb[1] - ra[1], rb[2] - ra[2]};
         const double r = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-        double gA[3] = {0, 0, 0}, dhA = 0.0, dhB = 0.0;
-        for (int ia = ss.shoff[A]; ia < ss.shoff[A + 1]; ++ia) {
-            const ShellDesc sa = ss.sh[ia];
-            const ElemParams& ea = g_tab.e[sa.el];
-            for (int ib = ss.shoff[Bt]; ib < ss.shoff[Bt + 1]; ++ib) {
-                const ShellDesc sb = ss.sh[ib];
-                const ElemParams& eb = g_tab.e[sb.el];
-                double pir, dpir, kxy;
-                const double scal = h0_scale(ea, sa.k, eb, sb.k, r, pir, dpir, kxy);
-                const double hav = 0.5 * (hself[ia] + hself[ib]);
-                double psum = 0.0;
-                for (int a = 0; a < sa.nao; ++a)
-                    for (int b = 0; b < sb.nao; ++b) {
-                        int la[3], lb[3];
-                        ao_cart(sa.l, a, la);
-                        ao_cart(sb.l, b, lb);
-                        double s, dpb[3], qpb[6], ds[3], ddp[3][3], dqp[3][6];
-                        ao_pair_derivs(ea, sa.k, eb, sb.k, ra, rb, la, lb, s, dpb, qpb, ds, ddp, dqp);
-                        const int mu = sa.ao0 + a, nu = sb.ao0 + b;
-                        const double pmn = P[(size_t)mu * ld + nu], wmn = W[(size_t)mu * ld + nu];
-                        psum += pmn * s;
-                        const double wS = 2.0 * pmn * hav * scal - 2.0 * wmn - pmn * (vs[ia] + vs[ib]);
-                        double wDA[3], wDB[3], wQA[6], wQB[6], tA[6], tB[6];
-                        for (int c = 0; c < 3; ++c) {
-                            wDA[c] = -pmn * vd[3 * A + c];
-                            wDB[c] = -pmn * vd[3 * Bt + c];
-                        }
-                        for (int k = 0; k < 6; ++k) {
-                            wQA[k] = -pmn * vq[6 * A + k];
-                            wQB[k] = -pmn * vq[6 * Bt + k];
-                        }
-                        traceless6_t(wQA, tA);
-                        traceless6_t(wQB, tB);
-                        for (int d = 0; d < 3; ++d) {
-                            double g = wS * ds[d];
-                            for (int c = 0; c < 3; ++c) {
-                                g += wDB[c] * ddp[d][c];
-                                g += wDA[c] * (ddp[d][c] - (c == d ? s : 0.0) + t[c] * ds[d]);
-                            }
-                            for (int k = 0; k < 6; ++k) {
-                                const int u = qu(k), w = qw(k);
-                                g += tB[k] * dqp[d][k];
-                                double v = dqp[d][k];
-                                v += -(u == d ? dpb[w] : 0.0) - (w == d ? dpb[u] : 0.0);
-                                v += t[u] * ddp[d][w] + t[w] * ddp[d][u];
-                                v += -(u == d ? t[w] * s : 0.0) - (w == d ? t[u] * s : 0.0);
-                                v += t[u] * t[w] * ds[d];
-                                g += tA[k] * v;
-                            }
-                            gA[d] += g;
-                        }
-                    }
-                const double fpi = 2.0 * psum * hav * kxy * dpir;
-                for (int d = 0; d < 3; ++d) gA[d] += fpi * (-t[d] / r);
-                dhA += psum * scal * (-ea.kcn[sa.k]);
-                dhB += psum * scal * (-eb.kcn[sb.k]);
-            }
+        double pir, dpir, kxy;
+        const double scal = h0_scale(ea, sa.k, eb, sb.k, r, pir, dpir, kxy);
+        const double hav = 0.5 * (hself[ia] + hself[ib]);
+        DerivAcc acc;
+        acc.s = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            acc.dpb[d] = acc.ds[d] = acc.wds[d] = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc.ddp[d][c] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc.dqp[d][k] = 0.0;
         }
-        double* o = pb + (size_t)p * kPairStride;
-        o[0] = gA[0];
-        o[1] = gA[1];
-        o[2] = gA[2];
-        o[3] = dhA;
-        o[4] = dhB;
+        switch (sa.l * 2 + sb.l) {
+        case 3: grad_shell_pair<1, 1>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
+        case 2: grad_shell_pair<1, 0>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
+        case 1: grad_shell_pair<0, 1>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
+        default: grad_shell_pair<0, 0>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
+        }
+        // potential contraction: moments about A follow from those about B by
+        // the shift t, which brings in the value terms s and dpb
+        double TA[6], TB[6];
+        traceless6_t(vq + 6 * A, TA);
+        traceless6_t(vq + 6 * Bt, TB);
+        const double E00 = 2.0 * TA[0], E01 = TA[1], E11 = 2.0 * TA[2], E02 = TA[3], E12 = TA[4],
+                     E22 = 2.0 * TA[5];
+        const double Et[3] = {E00 * t[0] + E01 * t[1] + E02 * t[2], E01 * t[0] + E11 * t[1] + E12 * t[2],
+                              E02 * t[0] + E12 * t[1] + E22 * t[2]};
+        const double* vda = vd + 3 * A;
+        const double* vdb = vd + 3 * Bt;
+        const double a0 = 2.0 * hav * scal - vs[ia] - vs[ib] -
+                          (vda[0] * t[0] + vda[1] * t[1] + vda[2] * t[2]) -
+                          0.5 * (Et[0] * t[0] + Et[1] * t[1] + Et[2] * t[2]);
+        const double bc[3] = {vda[0] + vdb[0] + Et[0], vda[1] + vdb[1] + Et[1], vda[2] + vdb[2] + Et[2]};
+        const double dd[3] = {vda[0] + Et[0], vda[1] + Et[1], vda[2] + Et[2]};
+        const double ck[6] = {TA[0] + TB[0], TA[1] + TB[1], TA[2] + TB[2],
+                              TA[3] + TB[3], TA[4] + TB[4], TA[5] + TB[5]};
+        const double Em[3][3] = {{E00, E01, E02}, {E01, E11, E12}, {E02, E12, E22}};
+        const double psum = acc.s;
+        const double fpi = 2.0 * psum * hav * kxy * dpir;
+        double* o = spb + (size_t)(ia * nsh + ib) * 5;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double g = a0 * acc.ds[d] - 2.0 * acc.wds[d] + dd[d] * acc.s;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g += -bc[c] * acc.ddp[d][c] + Em[d][c] * acc.dpb[c];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) g -= ck[k] * acc.dqp[d][k];
+            o[d] = g + fpi * (-t[d] / r);
+        }
+        o[3] = psum * scal * (-ea.kcn[sa.k]);
+        o[4] = psum * scal * (-eb.kcn[sb.k]);
     }
 
     // Phase B: explicit pair terms, contribution to atom a from ordered pair (a, b)
@@ -298,12 +321,19 @@ __global__ __launch_bounds__(QX_BLOCK) void k_grad(BatchPtrs B)
             dc += o[8];
             dc4 += o[9];
             if (b < a) {
-                for (int c = 0; c < 3; ++c) g[c] += o[c];
-                dc += o[3];
+                for (int ia = ss.shoff[a]; ia < ss.shoff[a + 1]; ++ia)
+                    for (int ib = ss.shoff[b]; ib < ss.shoff[b + 1]; ++ib) {
+                        const double* sp = spb + (size_t)(ia * nsh + ib) * 5;
+                        for (int c = 0; c < 3; ++c) g[c] += sp[c];
+                        dc += sp[3];
+                    }
             } else {
-                const double* o2 = pb + (size_t)(b * nat + a) * kPairStride;
-                for (int c = 0; c < 3; ++c) g[c] -= o2[c];
-                dc += o2[4];
+                for (int ia = ss.shoff[b]; ia < ss.shoff[b + 1]; ++ia)
+                    for (int ib = ss.shoff[a]; ib < ss.shoff[a + 1]; ++ib) {
+                        const double* sp = spb + (size_t)(ia * nsh + ib) * 5;
+                        for (int c = 0; c < 3; ++c) g[c] -= sp[c];
+                        dc += sp[4];
+                    }
             }
         }
         // on-site self-energies

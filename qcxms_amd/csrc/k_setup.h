@@ -84,6 +84,62 @@ QX_DEV double d4_disp(const ElemParams& ei, const ElemParams& ej, double r2, dou
     return gfn2::kD4S6 * oor6 + gfn2::kD4S8 * r4r2ij * oor8;
 }
 
+struct IntegralOut {
+    double *S, *H, *D, *Q;
+    int ld;
+    size_t ms;
+};
+
+// S, H0 (off-atom), D and Q blocks of one bra AO (component A) of a shell
+// pair.  S and H0 are symmetric; the moment integrals are stored transposed:
+// element (mu, nu), taken about the atom of mu, sits at nu * ld + mu.
+template <int LA, int LB, int A>
+QX_DEV void setup_shell_row(const ShellDesc& sa, const ShellDesc& sb, const ElemParams& ea,
+                            const ElemParams& eb, const double* ra, const double* rb,
+                            const double t[3], double hs, bool off, const IntegralOut& io)
+{
+    double s[3], dpb[3][3], qpb[3][6];
+    shell_row_values<LA, LB, A>(ea, sa.k, eb, sb.k, ra, rb, s, dpb, qpb);
+    constexpr int NB = LB ? 3 : 1;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        double dpa[3], qpa[6], qta[6], qtb[6];
+        shift_to_a(t, s[b], dpb[b], qpb[b], dpa, qpa);
+        traceless6(qpa, qta);
+        traceless6(qpb[b], qtb);
+        const int mu = sa.ao0 + A, nu = sb.ao0 + b;
+        const size_t o1 = (size_t)mu * io.ld + nu, o2 = (size_t)nu * io.ld + mu;
+        io.S[o1] = s[b];
+        io.S[o2] = s[b];
+        if (off) {
+            io.H[o1] = hs * s[b];
+            io.H[o2] = hs * s[b];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            io.D[c * io.ms + o2] = dpa[c];
+            io.D[c * io.ms + o1] = dpb[b][c];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            io.Q[k * io.ms + o2] = qta[k];
+            io.Q[k * io.ms + o1] = qtb[k];
+        }
+    }
+}
+
+template <int LA, int LB>
+QX_DEV void setup_shell_pair(const ShellDesc& sa, const ShellDesc& sb, const ElemParams& ea,
+                             const ElemParams& eb, const double* ra, const double* rb,
+                             const double t[3], double hs, bool off, const IntegralOut& io)
+{
+    setup_shell_row<LA, LB, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io);
+    if constexpr (LA == 1) {
+        setup_shell_row<LA, LB, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, io);
+        setup_shell_row<LA, LB, 2>(sa, sb, ea, eb, ra, rb, t, hs, off, io);
+    }
+}
+
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(QX_BLOCK) void k_setup(BatchPtrs B, int with_chol)
 {
@@ -193,55 +249,31 @@ __global__ __launch_bounds__(QX_BLOCK) void k_setup(BatchPtrs B, int with_chol)
     for (size_t i = tid; i < 3 * ms; i += QX_BLOCK) D[i] = 0.0;
     for (size_t i = tid; i < 6 * ms; i += QX_BLOCK) Q[i] = 0.0;
     __syncthreads();
-    // 4. integrals over shell pairs (ia >= ib)
-    const int npair = nsh * (nsh + 1) / 2;
-    for (int p = tid; p < npair; p += QX_BLOCK) {
-        int ia = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
-        while (ia * (ia + 1) / 2 > p) --ia;
-        while ((ia + 1) * (ia + 2) / 2 <= p) ++ia;
-        const int ib = p - ia * (ia + 1) / 2;
-        const ShellDesc sa = sh[ia], sb = sh[ib];
+    // 4. integrals over shell pairs, one work item per pair (off-atom pairs
+    //    grouped by angular-momentum class, then same-atom pairs)
+    for (int it = tid; it < si.nitem; it += QX_BLOCK) {
+        const int2 pr = B.spitem[si.item_off + it];
+        const ShellDesc sa = sh[pr.x], sb = sh[pr.y];
         const ElemParams& ea = g_tab.e[sa.el];
         const ElemParams& eb = g_tab.e[sb.el];
         const double* ra = sx + 3 * sa.at;
         const double* rb = sx + 3 * sb.at;
         const double t[3] = {rb[0] - ra[0], rb[1] - ra[1], rb[2] - ra[2]};
         double hs = 0.0;
-        if (sa.at != sb.at) {
+        const bool off = sa.at != sb.at;
+        if (off) {
             const double r = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
             double pir, dpir, kxy;
-            hs = 0.5 * (B.hself[si.shell_off + ia] + B.hself[si.shell_off + ib]) *
+            hs = 0.5 * (B.hself[si.shell_off + pr.x] + B.hself[si.shell_off + pr.y]) *
                  h0_scale(ea, sa.k, eb, sb.k, r, pir, dpir, kxy);
         }
-        for (int a = 0; a < sa.nao; ++a)
-            for (int b = 0; b < sb.nao; ++b) {
-                int la[3], lb[3];
-                ao_cart(sa.l, a, la);
-                ao_cart(sb.l, b, lb);
-                double s, dpb[3], qpb[6], dpa[3], qpa[6], qta[6], qtb[6];
-                ao_pair_values(ea, sa.k, eb, sb.k, ra, rb, la, lb, s, dpb, qpb);
-                shift_to_a(t, s, dpb, qpb, dpa, qpa);
-                traceless6(qpa, qta);
-                traceless6(qpb, qtb);
-                const int mu = sa.ao0 + a, nu = sb.ao0 + b;
-                const size_t o1 = (size_t)mu * ld + nu, o2 = (size_t)nu * ld + mu;
-                S[o1] = s;
-                S[o2] = s;
-                if (sa.at != sb.at) {
-                    H[o1] = hs * s;
-                    H[o2] = hs * s;
-                }
-                // moment integrals are stored transposed: element (mu, nu),
-                // taken about the atom of mu, sits at nu * ld + mu
-                for (int c = 0; c < 3; ++c) {
-                    D[c * ms + o2] = dpa[c];
-                    D[c * ms + o1] = dpb[c];
-                }
-                for (int k = 0; k < 6; ++k) {
-                    Q[k * ms + o2] = qta[k];
-                    Q[k * ms + o1] = qtb[k];
-                }
-            }
+        const IntegralOut io{S, H, D, Q, ld, ms};
+        switch (sa.l * 2 + sb.l) {
+        case 3: setup_shell_pair<1, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        case 2: setup_shell_pair<1, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        case 1: setup_shell_pair<0, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        default: setup_shell_pair<0, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        }
     }
     __syncthreads();
     for (int s = tid; s < nsh; s += QX_BLOCK)

@@ -167,6 +167,34 @@ def egrad_batch(systems, xyz_list, etemp=300.0, method=GFN2_XTB):
             [q[off[i]:off[i + 1]] for i in range(len(systems))], st)
 
 
+def egrad_many(systems, xyz_list, etemp=300.0, method=GFN2_XTB):
+    """Independent single points of ragged systems through qcxms_egrad_batch
+    (one launch set, systems ordered largest basis first inside the library,
+    default SCC thresholds): the per-trajectory egrad calls of
+    src/iniqm.f90:641-655 for many structures at once.  systems:
+    [(num, charge[, multiplicity])].  Returns (energy[nsys], [grad], [qat],
+    stat[nsys])."""
+    L = library()
+    nsys = len(systems)
+    off = np.cumsum([0] + [len(s[0]) for s in systems]).astype(np.int32)
+    num = np.ascontiguousarray(np.concatenate([s[0] for s in systems]), dtype=np.int32)
+    chg = np.array([s[1] for s in systems], dtype=np.int32)
+    mult = np.array([s[2] if len(s) > 2 else 1 for s in systems], dtype=np.int32)
+    xyz = np.ascontiguousarray(np.concatenate(xyz_list), dtype=np.float64)
+    et = np.ascontiguousarray(np.broadcast_to(np.asarray(etemp, dtype=np.float64), (nsys,)))
+    e = np.zeros(nsys)
+    g = np.zeros((off[-1], 3))
+    q = np.zeros(off[-1])
+    st = np.zeros(nsys, dtype=np.int32)
+    rc = L.qcxms_egrad_batch(nsys, _p(off, _ip), _p(num, _ip), _p(xyz), _p(chg, _ip),
+                             _p(mult, _ip), int(method), _p(et), _p(q), _p(e), _p(g),
+                             _p(st, _ip))
+    if rc not in (STAT_OK, STAT_FATAL, STAT_UNKNOWN_METHOD):
+        raise RuntimeError(f"qcxms_egrad_batch failed ({rc}): " + last_error())
+    return (e, [g[off[i]:off[i + 1]] for i in range(nsys)],
+            [q[off[i]:off[i + 1]] for i in range(nsys)], st)
+
+
 # ---------------------------------------------------------------- MD ensemble
 class MdParams(C.Structure):
     _fields_ = [("tstep", C.c_double), ("etempin", C.c_double), ("ax", C.c_double),

@@ -197,3 +197,25 @@ def md_run(traj, steps_limit, tstep=0.5 * 41.3413733365614, max_steps=2000, nfra
                  ptr(velo), ptr(grad), ptr(qat), ptr(lst, ip), fn, C.cast(C.pointer(o), C.c_void_p),
                  int(steps_limit))
     return dict(state=st, xyz=xyz, velo=velo, grad=grad, qat=qat, list=lst)
+
+
+def gfn2_egrad_batch(systems, xyz_list, etemp, opts=None, nthreads=None):
+    """Oracle single points over many systems [(num, charge)] with host threads.
+    Returns (energy[nsys], [grad], [qat], stat[nsys])."""
+    L = oracle()
+    nsys = len(systems)
+    off = np.cumsum([0] + [len(n) for n, _ in systems]).astype(np.int32)
+    num = np.ascontiguousarray(np.concatenate([n for n, _ in systems]), dtype=np.int32)
+    chg = np.array([c for _, c in systems], dtype=np.int32)
+    xyz = np.ascontiguousarray(np.concatenate(xyz_list), dtype=np.float64)
+    et = np.ascontiguousarray(np.broadcast_to(np.asarray(etemp, dtype=np.float64), (nsys,)))
+    e = np.zeros(nsys)
+    g = np.zeros((off[-1], 3))
+    q = np.zeros(off[-1])
+    st = np.zeros(nsys, dtype=np.int32)
+    o = opts if opts is not None else default_opts()
+    nt = nthreads or max(1, min(os.cpu_count() or 1, 16))
+    L.orc_gfn2_egrad_batch(nsys, ptr(off, ip), ptr(num, ip), ptr(xyz), ptr(chg, ip), ptr(et),
+                           C.byref(o), ptr(e), ptr(g), ptr(q), ptr(st, ip), nt)
+    return (e, [g[off[i]:off[i + 1]] for i in range(nsys)],
+            [q[off[i]:off[i + 1]] for i in range(nsys)], st)

@@ -124,3 +124,29 @@ def test_md_fragment_exit_matches_oracle():
     ref = ob.md_run(tr, steps_limit=0, max_steps=50, nfragexit=1)["state"]
     assert (res.nstep, res.nfrag, res.exit_reason, res.mdok, res.fragstate) == \
         (ref.nstep, ref.nfrag, ref.exit_reason, ref.mdok, ref.fragstate) == (1, 2, 4, 1, 1)
+
+
+def test_bench_workload_at_default_thresholds_meets_parity():
+    """The batched MD exactly as bench.py runs it (default SCC thresholds,
+    warm-started and extrapolated SCC, hipGraph replay) stays within the
+    parity bar of the tightly converged oracle MD: per-step energies to
+    1e-8 Eh, forces to 1e-6 Eh/bohr, charges to 1e-6 e."""
+    import bench
+    trajs = bench.workload(4)
+    nsteps = 12
+    md = q().MdEnsemble(trajs, max_steps=nsteps, use_graph=True)
+    md.run(nsteps)
+    out = md.read()
+    md.close()
+    opts = ob.default_opts(econv=1e-12, pconv=1e-10)
+    for t, tr in enumerate(trajs):
+        ref = ob.md_run(tr, steps_limit=0, max_steps=nsteps, opts=opts)
+        st, res = ref["state"], out["results"][t]
+        sl = slice(md.offsets[t], md.offsets[t + 1])
+        de = abs(res.epot - st.epot)
+        dg = np.abs(out["grad"][sl] - ref["grad"]).max()
+        dq = np.abs(out["qat"][sl] - ref["qat"]).max()
+        dx = np.abs(out["xyz"][sl] - ref["xyz"]).max()
+        print(f"bench traj {t}: dE={de:.2e} dg={dg:.2e} dq={dq:.2e} dx={dx:.2e}", flush=True)
+        assert res.nstep == st.nstep == nsteps
+        assert de <= 1e-8 and dg <= 1e-6 and dq <= 1e-6
