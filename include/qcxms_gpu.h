@@ -102,7 +102,9 @@ typedef struct {
     int32_t nfragexit;  /* exit when more fragments than this appear (input.f90:216: 3) */
     int32_t max_steps;  /* nmax = tmax / tstep */
     int32_t use_graph;  /* != 0: replay the five kernels of a step as one hipGraph */
-    int32_t pad;
+    int32_t extrapolate; /* SCC start of each step from the previous steps' converged
+                            moments: 1 linear, 2 quadratic, -1 none (previous step's
+                            moments), 0 library default (2) */
     double econv, pconv;  /* SCC thresholds (defaults 1e-9, 1e-8 when 0) */
 } qcxms_md_params;
 

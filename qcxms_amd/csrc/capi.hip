@@ -380,6 +380,7 @@ int allocate(qcxms_batch* b)
     ALLOC(d.spbuf, 5 * b->gam_total);
     ALLOC(d.aesg, b->pair_total / QX_PAIR_STRIDE * QX_AES_COMP);
     ALLOC(d.xprev, b->nxtot);
+    ALLOC(d.xprev2, b->nxtot);
     ALLOC(d.energy, b->nsys);
     ALLOC(d.eparts, 10 * (size_t)b->nsys);
     int* iters;
@@ -855,7 +856,12 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
     md->p.nfragexit = params->nfragexit;
     md->p.max_steps = params->max_steps;
     b->warm_start = 1;
-    b->opts.extrapolate = 1;
+    if (params->extrapolate < -1 || params->extrapolate > 2) {
+        delete md;  // owns the batch
+        g_err = "extrapolate must be -1, 0, 1 or 2";
+        return nullptr;
+    }
+    b->opts.extrapolate = params->extrapolate == 0 ? 2 : (params->extrapolate < 0 ? 0 : params->extrapolate);
     if (params->econv > 0.0) b->opts.econv = params->econv;
     if (params->pconv > 0.0) b->opts.pconv = params->pconv;
     const int na = b->natoms;

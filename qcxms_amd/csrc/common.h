@@ -58,7 +58,7 @@ struct SysInfo {
 struct SccOpts {
     double econv, pconv, damp, jacobi_tol;
     int maxiter, mem;
-    int extrapolate;  // 1: start MD steps from 2 x(n-1) - x(n-2)
+    int extrapolate;  // MD start: 1 linear 2 x(n-1) - x(n-2), 2 quadratic 3 (x(n-1) - x(n-2)) + x(n-3)
     int pad;
 };
 
@@ -103,6 +103,7 @@ struct BatchPtrs {
     double* spbuf;    // per shell pair (ia, ib): gradient on A (3), dE/dCN_A, dE/dCN_B
     double* aesg;     // multipole-kernel factors, QX_AES_COMP planes of nat x nat per system
     double* xprev;    // mixing vector of the step before (extrapolated MD start)
+    double* xprev2;   // and of the step before that
     double* energy;   // per system: total
     double* eparts;   // per system: 10 parts (band, es2, es3, aes, axc, d4, atm, rep, ts, scc)
     int* iters;       // per system

@@ -175,35 +175,57 @@ QX_DEV void tri_index(int b, int& P, int& R)
     R = b - P * (P + 1) / 2;
 }
 
-// Rotate the 2 x 2 blocks (P, R) and (R, P) of A for one round.  All reads of
-// a thread's entries precede its writes; no entry is touched by two blocks.
-QX_DEV void jacobi_block(double* A, int lda, const double2* rot, const int2* rpq, int P, int R)
+// Jacobi rotation (c, s) that zeroes a_pq:
+//   r = sqrt(d^2 + 4 a^2), d = a_qq - a_pp, u = r + |d|,
+//   c = sqrt(u / 2r), s = 2 a sign(d) c / u   (c^2 + s^2 = 1 exactly)
+// i.e. t = s / c = 2 a sign(d) / (|d| + r), the smaller root of
+// t^2 + t d / a - 1 = 0.  Three dependent long-latency operations.
+QX_DEV double2 jacobi_rotation(double app, double aqq, double apq)
 {
-    const double2 rp = rot[P], rr = rot[R];
+    const double d = aqq - app, ad = fabs(d);
+    const double r = sqrt(d * d + 4.0 * apq * apq);
+    const double u = r + ad;
+    const double ir = 1.0 / r, iu = 1.0 / u;
+    const double c = sqrt(0.5 * u * ir);
+    const double s = 2.0 * (d >= 0.0 ? apq : -apq) * c * iu;
+    return make_double2(c, s);
+}
+
+// Rotated entries of the 2 x 2 blocks (P, R) and (R, P): input b00 = A[p][r],
+// b01 = A[p][s], b10 = A[q][r], b11 = A[q][s] with (p, q) the pair of P and
+// (r, s) the pair of R; on the diagonal block (P == R) the off-diagonal
+// entries become exact zeros.
+QX_DEV void rotate_block(double2 rp, double2 rr, bool diag, const double b[4], double n[4])
+{
     const double cp = rp.x, sp = rp.y, cr = rr.x, sr = rr.y;
-    if (sp == 0.0 && sr == 0.0) return;
-    const int2 ip = rpq[P], ir = rpq[R];
-    const int p = ip.x, q = ip.y, r2 = ir.x, s2 = ir.y;
-    const double b00 = A[(size_t)p * lda + r2], b01 = A[(size_t)p * lda + s2];
-    const double b10 = A[(size_t)q * lda + r2], b11 = A[(size_t)q * lda + s2];
-    const double m00 = b00 * cr - b01 * sr, m01 = b00 * sr + b01 * cr;
-    const double m10 = b10 * cr - b11 * sr, m11 = b10 * sr + b11 * cr;
-    double n00 = cp * m00 - sp * m10, n01 = cp * m01 - sp * m11;
-    double n10 = sp * m00 + cp * m10, n11 = sp * m01 + cp * m11;
-    if (P == R) {
-        n01 = 0.0;
-        n10 = 0.0;
+    const double m00 = b[0] * cr - b[1] * sr, m01 = b[0] * sr + b[1] * cr;
+    const double m10 = b[2] * cr - b[3] * sr, m11 = b[2] * sr + b[3] * cr;
+    n[0] = cp * m00 - sp * m10;
+    n[1] = diag ? 0.0 : cp * m01 - sp * m11;
+    n[2] = diag ? 0.0 : sp * m00 + cp * m10;
+    n[3] = sp * m01 + cp * m11;
+}
+
+QX_DEV void store_block(double* A, int lda, int2 ip, int2 ir, bool diag, const double n[4])
+{
+    A[(size_t)ip.x * lda + ir.x] = n[0];
+    A[(size_t)ip.x * lda + ir.y] = n[1];
+    A[(size_t)ip.y * lda + ir.x] = n[2];
+    A[(size_t)ip.y * lda + ir.y] = n[3];
+    if (!diag) {
+        A[(size_t)ir.x * lda + ip.x] = n[0];
+        A[(size_t)ir.y * lda + ip.x] = n[1];
+        A[(size_t)ir.x * lda + ip.y] = n[2];
+        A[(size_t)ir.y * lda + ip.y] = n[3];
     }
-    A[(size_t)p * lda + r2] = n00;
-    A[(size_t)p * lda + s2] = n01;
-    A[(size_t)q * lda + r2] = n10;
-    A[(size_t)q * lda + s2] = n11;
-    if (P != R) {
-        A[(size_t)r2 * lda + p] = n00;
-        A[(size_t)s2 * lda + p] = n01;
-        A[(size_t)r2 * lda + q] = n10;
-        A[(size_t)s2 * lda + q] = n11;
-    }
+}
+
+QX_DEV void load_block(const double* A, int lda, int2 ip, int2 ir, double b[4])
+{
+    b[0] = A[(size_t)ip.x * lda + ir.x];
+    b[1] = A[(size_t)ip.x * lda + ir.y];
+    b[2] = A[(size_t)ip.y * lda + ir.x];
+    b[3] = A[(size_t)ip.y * lda + ir.y];
 }
 
 // Parallel cyclic threshold Jacobi on symmetric A (m x m, m even, leading
@@ -213,6 +235,9 @@ QX_DEV void jacobi_block(double* A, int lda, const double2* rot, const int2* rpq
 // dimension lda), so Cv ends as Cv V with A = V^T A_in V diagonal.
 // One round = rotation parameters (one thread per pair) | barrier | block and
 // column updates | barrier; rounds without any rotation skip the updates.
+// A thread's first two blocks (P, R) and its column pair are the same in
+// every round, so their parameters and entries are fetched in two LDS round
+// trips (parameters, then data) before any store.
 QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, int maxsweep,
                   double2* rot, int2* rpq, double* red, int* s_any)
 {
@@ -224,13 +249,13 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
     }
     __syncthreads();
     const int nblk = half * (half + 1) / 2;
-    // the first two blocks of this thread are the same in every round
     int bP0 = 0, bR0 = 0, bP1 = 0, bR1 = 0;
     const bool has0 = threadIdx.x < nblk, has1 = threadIdx.x + QX_BLOCK < nblk;
     if (has0) tri_index(threadIdx.x, bP0, bR0);
     if (has1) tri_index(threadIdx.x + QX_BLOCK, bP1, bR1);
-    // columns of Cv: pair slot from the low 5 thread bits, rows ic + 8 j
+    // column pair slot from the low 5 thread bits, rows ic + 8 j
     const int kc = threadIdx.x & 31, ic = threadIdx.x >> 5;
+    const bool hasC = kc < half;
     int sweep = 0;
     QX_PT_INIT(jt)
     while (sweep < maxsweep) {
@@ -243,15 +268,12 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
                 rr_pair(m, r, k, p, q);
                 const double apq = A[(size_t)p * lda + q];
                 const double app = A[(size_t)p * lda + p], aqq = A[(size_t)q * lda + q];
-                double c = 1.0, s = 0.0;
+                double2 cs = make_double2(1.0, 0.0);
                 if (fabs(apq) >= tol) {
-                    const double tau = (aqq - app) / (2.0 * apq);
-                    const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    c = 1.0 / sqrt(1.0 + t * t);
-                    s = t * c;
+                    cs = jacobi_rotation(app, aqq, apq);
                     s_any[f] = 1;
                 }
-                rot[k] = make_double2(c, s);
+                rot[k] = cs;
                 rpq[k] = make_int2(p, q);
             }
             __syncthreads();
@@ -259,35 +281,69 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
             QX_PH_ADD(19, 1);
             if (s_any[f]) {
                 QX_PH_ADD(18, 1);
-                if (has0) jacobi_block(A, lda, rot, rpq, bP0, bR0);
-                if (has1) jacobi_block(A, lda, rot, rpq, bP1, bR1);
+                // parameters
+                const double2 rp0 = rot[bP0], rr0 = rot[bR0], rp1 = rot[bP1], rr1 = rot[bR1];
+                const int2 ip0 = rpq[bP0], ir0 = rpq[bR0], ip1 = rpq[bP1], ir1 = rpq[bR1];
+                const double2 rc = rot[hasC ? kc : 0];
+                const int2 pc = rpq[hasC ? kc : 0];
+                const bool do0 = has0 && (rp0.y != 0.0 || rr0.y != 0.0);
+                const bool do1 = has1 && (rp1.y != 0.0 || rr1.y != 0.0);
+                const bool doC = hasC && rc.y != 0.0;
+                // data
+                double b0[4] = {0, 0, 0, 0}, b1[4] = {0, 0, 0, 0}, vp[8], vq[8];
+                if (do0) load_block(A, lda, ip0, ir0, b0);
+                if (do1) load_block(A, lda, ip1, ir1, b1);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int i = ic + 8 * u;
+                    const bool ok = doC && i < nrow;
+                    vp[u] = ok ? Cv[(size_t)i * lda + pc.x] : 0.0;
+                    vq[u] = ok ? Cv[(size_t)i * lda + pc.y] : 0.0;
+                }
+                // rotate and store
+                if (do0) {
+                    double n0[4];
+                    const bool dg = bP0 == bR0;
+                    rotate_block(rp0, rr0, dg, b0, n0);
+                    store_block(A, lda, ip0, ir0, dg, n0);
+                }
+                if (do1) {
+                    double n1[4];
+                    const bool dg = bP1 == bR1;
+                    rotate_block(rp1, rr1, dg, b1, n1);
+                    store_block(A, lda, ip1, ir1, dg, n1);
+                }
+                if (doC) {
+                    const double c = rc.x, s = rc.y;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int i = ic + 8 * u;
+                        if (i < nrow) {
+                            Cv[(size_t)i * lda + pc.x] = c * vp[u] - s * vq[u];
+                            Cv[(size_t)i * lda + pc.y] = s * vp[u] + c * vq[u];
+                        }
+                    }
+                }
+                // larger systems: remaining blocks, rows and column pairs
                 for (int b = threadIdx.x + 2 * QX_BLOCK; b < nblk; b += QX_BLOCK) {
                     int P, R;
                     tri_index(b, P, R);
-                    jacobi_block(A, lda, rot, rpq, P, R);
+                    const double2 xp = rot[P], xr = rot[R];
+                    if (xp.y == 0.0 && xr.y == 0.0) continue;
+                    const int2 jp = rpq[P], jr = rpq[R];
+                    double bb[4], nn[4];
+                    load_block(A, lda, jp, jr, bb);
+                    rotate_block(xp, xr, P == R, bb, nn);
+                    store_block(A, lda, jp, jr, P == R, nn);
                 }
                 for (int k = kc; k < half; k += 32) {
                     const double2 cs = rot[k];
-                    const double c = cs.x, s = cs.y;
-                    if (s == 0.0) continue;
+                    if (cs.y == 0.0) continue;
                     const int2 pq = rpq[k];
-                    const int p = pq.x, q = pq.y;
-                    for (int i0 = ic; i0 < nrow; i0 += 64) {
-                        double vp[8], vq[8];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) {
-                            const int i = i0 + 8 * u;
-                            vp[u] = i < nrow ? Cv[(size_t)i * lda + p] : 0.0;
-                            vq[u] = i < nrow ? Cv[(size_t)i * lda + q] : 0.0;
-                        }
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) {
-                            const int i = i0 + 8 * u;
-                            if (i < nrow) {
-                                Cv[(size_t)i * lda + p] = c * vp[u] - s * vq[u];
-                                Cv[(size_t)i * lda + q] = s * vp[u] + c * vq[u];
-                            }
-                        }
+                    for (int i = (k == kc ? ic + 64 : ic); i < nrow; i += 8) {
+                        const double xv = Cv[(size_t)i * lda + pq.x], yv = Cv[(size_t)i * lda + pq.y];
+                        Cv[(size_t)i * lda + pq.x] = cs.x * xv - cs.y * yv;
+                        Cv[(size_t)i * lda + pq.y] = cs.y * xv + cs.x * yv;
                     }
                 }
             }
@@ -542,7 +598,8 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
     const double* Dm = B.D + 3 * si.mat_off;  // transposed: (mu, nu) at nu * ld + mu
     const double* Qm = B.Q + 6 * si.mat_off;
     double* xg = B.x + si.x_off;
-    double* xp = B.xprev + si.x_off;
+    double* xp = B.xprev + si.x_off;    // converged moments of the step before
+    double* xpp = B.xprev2 + si.x_off;  // and of the one before that
     const int warm = B.warm[sys];
     double* A = bf.A;
     double* Cc = bf.C;
@@ -551,12 +608,15 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
     double* rs = bf.rs;
 
     QX_PH_INIT
-    // initial moments (extrapolated in MD) and basis C = X U (padding column zero)
+    // initial moments (linear or quadratic extrapolation over the previous
+    // steps in MD) and basis C = X U (padding column zero)
     for (int i = tid; i < nx; i += QX_BLOCK) {
         double v = 0.0;
-        if (warm >= 2 && opt.extrapolate) v = 2.0 * xg[i] - xp[i];
+        if (warm >= 3 && opt.extrapolate >= 2) v = 3.0 * (xg[i] - xp[i]) + xpp[i];
+        else if (warm >= 2 && opt.extrapolate >= 1) v = 2.0 * xg[i] - xp[i];
         else if (warm >= 1) v = xg[i];
         xin[i] = v;
+        if (warm >= 2) xpp[i] = xp[i];
         if (warm >= 1) xp[i] = xg[i];
     }
     if (warm) {
@@ -856,7 +916,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         B.iters[sys] = itn;
         B.iters_total[sys] += (unsigned long long)itn;
         B.status[sys] = conv ? 0 : -1;
-        B.warm[sys] = warm >= 1 ? 2 : 1;
+        B.warm[sys] = warm >= 3 ? 3 : warm + 1;
     }
 }
 

@@ -18,8 +18,9 @@ DRIVER = os.path.join(ROOT, "qcxms_amd", "fortran", "_build", "egrad_driver")
 
 
 def run_driver(tmp_path, num, xyz, charge, mult, method, etemp):
-    lines = [f"{len(num)} {charge} {mult} {method} {etemp!r}"]
-    lines += [f"{z} {x!r} {y!r} {w!r}" for z, (x, y, w) in zip(num, xyz)]
+    lines = [f"{len(num)} {charge} {mult} {method} {float(etemp):.17e}"]
+    lines += [f"{int(z)} {float(x):.17e} {float(y):.17e} {float(w):.17e}"
+              for z, (x, y, w) in zip(num, xyz)]
     print(f"fortran driver: nat={len(num)} method={method}", flush=True)
     p = subprocess.run([DRIVER], input="\n".join(lines) + "\n", capture_output=True, text=True,
                        cwd=tmp_path, timeout=120)
