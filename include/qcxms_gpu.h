@@ -106,6 +106,9 @@ typedef struct {
                             moments: 1 linear, 2 quadratic, -1 none (previous step's
                             moments), 0 library default (2) */
     double econv, pconv;  /* SCC thresholds (defaults 1e-9, 1e-8 when 0) */
+    int32_t broyden_keep; /* > 0: keep up to this many Broyden secant pairs from step to
+                             step (same fixed point); 0: fresh mixing history per step */
+    int32_t pad;
 } qcxms_md_params;
 
 /* Final or current state of one trajectory (md.f90:684-700 outputs). */

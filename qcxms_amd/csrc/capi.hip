@@ -472,7 +472,7 @@ int enqueue_egrad(qcxms_batch* b, bool with_grad)
          })))
         return st;
     if ((st = timed(b, b->t_scc, [&] {
-             hipLaunchKernelGGL(qx::k_scc, dim3(b->nsys), dim3(QX_BLOCK), l2, g_stream, b->d, b->opts);
+             hipLaunchKernelGGL(qx::k_scc, dim3(b->nsys), dim3(QX_SCC_BLOCK), l2, g_stream, b->d, b->opts);
          })))
         return st;
     if (with_grad &&
@@ -856,6 +856,12 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
     md->p.nfragexit = params->nfragexit;
     md->p.max_steps = params->max_steps;
     b->warm_start = 1;
+    if (params->broyden_keep < 0 || params->broyden_keep > b->opts.mem) {
+        delete md;
+        g_err = "broyden_keep must lie in [0, 100]";
+        return nullptr;
+    }
+    b->opts.keep = params->broyden_keep;
     if (params->extrapolate < -1 || params->extrapolate > 2) {
         delete md;  // owns the batch
         g_err = "extrapolate must be -1, 0, 1 or 2";

@@ -10,7 +10,8 @@
 
 #include "gfn2_params.h"
 
-#define QX_BLOCK 256
+#define QX_BLOCK 256       // threads per system in the setup and gradient kernels
+#define QX_SCC_BLOCK 256   // threads per system in the SCC kernel
 #define QX_MAX_NAT 64
 #define QX_MAX_NSH 128
 #define QX_MAX_NAO 256
@@ -59,7 +60,7 @@ struct SccOpts {
     double econv, pconv, damp, jacobi_tol;
     int maxiter, mem;
     int extrapolate;  // MD start: 1 linear 2 x(n-1) - x(n-2), 2 quadratic 3 (x(n-1) - x(n-2)) + x(n-3)
-    int pad;
+    int keep;         // > 0: Broyden secant pairs kept across warm-started calls (ring of keep)
 };
 
 // Device-resident batch state.  Every pointer is a flat array over all systems.

@@ -22,6 +22,7 @@
 
 namespace qx {
 
+constexpr int kNT = QX_SCC_BLOCK;  // threads of the SCC kernel
 constexpr int kMaxNx = 128 + 9 * QX_MAX_NAT;
 constexpr int kMaxRef = QX_MAX_REF * QX_MAX_NAT;
 constexpr int kFastM = 56;
@@ -46,19 +47,29 @@ __device__ unsigned long long g_phase[kPhaseSys * kPhaseSlots];
     do {                                                                           \
         if (threadIdx.x == 0 && blockIdx.x < kPhaseSys) g_phase[blockIdx.x * kPhaseSlots + (k)] += (v); \
     } while (0)
-#define QX_PT_INIT(tv) unsigned long long tv = clock64();
+#define QX_PT_INIT(tv) unsigned long long tv = clock64(), tv##_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #define QX_PT(k, tv)                                                               \
     do {                                                                           \
-        if (threadIdx.x == 0 && blockIdx.x < kPhaseSys) {                          \
-            const unsigned long long t_ = clock64();                               \
-            g_phase[blockIdx.x * kPhaseSlots + (k)] += t_ - tv;                    \
-            tv = t_;                                                               \
-        }                                                                          \
+        const unsigned long long t_ = clock64();                                   \
+        tv##_acc[(k) & 7] += t_ - tv;                                              \
+        tv = t_;                                                                   \
+    } while (0)
+#define QX_PT_COUNT(tv, k) (tv##_acc[(k)&7] += 1)
+#define QX_PT_FLUSH(tv)                                                            \
+    do {                                                                           \
+        if (threadIdx.x == 0 && blockIdx.x < kPhaseSys)                            \
+            for (int k_ = 0; k_ < 8; ++k_) g_phase[blockIdx.x * kPhaseSlots + 16 + k_] += tv##_acc[k_]; \
     } while (0)
 #else
 #define QX_PT_INIT(tv)
 #define QX_PT(k, tv) \
     do {             \
+    } while (0)
+#define QX_PT_FLUSH(tv) \
+    do {                \
+    } while (0)
+#define QX_PT_COUNT(tv, k) \
+    do {                   \
     } while (0)
 #define QX_PH_INIT
 #define QX_PH(k) \
@@ -74,20 +85,21 @@ template <bool TA, bool TB>
 QX_DEV void block_gemm(int M, int N, int K, const double* A, int lda, const double* Bm, int ldb,
                        double* Cm, int ldc, const double* d)
 {
-    const int tm = (M + 3) >> 2, tn = (N + 3) >> 2;
-    for (int t = threadIdx.x; t < tm * tn; t += QX_BLOCK) {
-        // rows i0..i0+3, columns jb + b tn: neighbouring lanes read neighbouring columns
-        const int i0 = (t / tn) * 4, jb = t % tn;
-        double acc[4][4];
+    constexpr int TR = kNT >= 512 ? 2 : 4;  // rows per register tile (tiles are TR x 4)
+    const int tm = (M + TR - 1) / TR, tn = (N + 3) >> 2;
+    for (int t = threadIdx.x; t < tm * tn; t += kNT) {
+        // rows i0..i0+TR-1, columns jb + b tn: neighbouring lanes read neighbouring columns
+        const int i0 = (t / tn) * TR, jb = t % tn;
+        double acc[TR][4];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < TR; ++a)
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
         for (int k = 0; k < K; ++k) {
-            double av[4], bv[4];
+            double av[TR], bv[4];
             const double dk = d ? d[k] : 1.0;
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
+            for (int a = 0; a < TR; ++a) {
                 const int i = i0 + a < M ? i0 + a : M - 1;
                 av[a] = (TA ? A[(size_t)k * lda + i] : A[(size_t)i * lda + k]) * dk;
             }
@@ -97,12 +109,12 @@ QX_DEV void block_gemm(int M, int N, int K, const double* A, int lda, const doub
                 bv[b] = TB ? Bm[(size_t)j * ldb + k] : Bm[(size_t)k * ldb + j];
             }
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < TR; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
         }
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < TR; ++a)
 #pragma unroll
             for (int b = 0; b < 4; ++b)
                 if (i0 + a < M && jb + b * tn < N) Cm[(size_t)(i0 + a) * ldc + jb + b * tn] = acc[a][b];
@@ -117,7 +129,7 @@ QX_DEV double block_max(double v, double* smem)
     if (l == 0) smem[w] = v;
     __syncthreads();
     double s = smem[0];
-    for (int i = 1; i < QX_BLOCK / 64; ++i) s = fmax(s, smem[i]);
+    for (int i = 1; i < kNT / 64; ++i) s = fmax(s, smem[i]);
     __syncthreads();
     return s;
 }
@@ -136,7 +148,7 @@ QX_DEV void block_sums(double* v, double* smem)
     __syncthreads();
     for (int k = 0; k < K; ++k) {
         double s = 0.0;
-        for (int i = 0; i < QX_BLOCK / 64; ++i) s += smem[i * K + k];
+        for (int i = 0; i < kNT / 64; ++i) s += smem[i * K + k];
         v[k] = s;
     }
     __syncthreads();
@@ -160,7 +172,7 @@ QX_DEV void rr_pair(int m, int r, int k, int& p, int& q)
 QX_DEV double offdiag_max(const double* A, int m, int lda, double* red)
 {
     double mx = 0.0;
-    for (int i = threadIdx.x >> 6; i < m; i += QX_BLOCK / 64)
+    for (int i = threadIdx.x >> 6; i < m; i += kNT / 64)
         for (int j = threadIdx.x & 63; j < m; j += 64)
             if (i != j) mx = fmax(mx, fabs(A[(size_t)i * lda + j]));
     return block_max(mx, red);
@@ -250,10 +262,11 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
     __syncthreads();
     const int nblk = half * (half + 1) / 2;
     int bP0 = 0, bR0 = 0, bP1 = 0, bR1 = 0;
-    const bool has0 = threadIdx.x < nblk, has1 = threadIdx.x + QX_BLOCK < nblk;
+    const bool has0 = threadIdx.x < nblk, has1 = threadIdx.x + kNT < nblk;
     if (has0) tri_index(threadIdx.x, bP0, bR0);
-    if (has1) tri_index(threadIdx.x + QX_BLOCK, bP1, bR1);
-    // column pair slot from the low 5 thread bits, rows ic + 8 j
+    if (has1) tri_index(threadIdx.x + kNT, bP1, bR1);
+    // column pair slot from the low 5 thread bits, rows ic + kRG j
+    constexpr int kRG = kNT / 32, kRU = 64 / kRG;  // row groups, rows per thread in the first 64
     const int kc = threadIdx.x & 31, ic = threadIdx.x >> 5;
     const bool hasC = kc < half;
     int sweep = 0;
@@ -263,7 +276,7 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
         for (int r = 0; r < m - 1; ++r) {
             const int f = r & 1;
             QX_PT(20, jt);
-            for (int k = threadIdx.x; k < half; k += QX_BLOCK) {
+            for (int k = threadIdx.x; k < half; k += kNT) {
                 int p, q;
                 rr_pair(m, r, k, p, q);
                 const double apq = A[(size_t)p * lda + q];
@@ -278,9 +291,9 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
             }
             __syncthreads();
             QX_PT(16, jt);
-            QX_PH_ADD(19, 1);
+            QX_PT_COUNT(jt, 3);
             if (s_any[f]) {
-                QX_PH_ADD(18, 1);
+                QX_PT_COUNT(jt, 2);
                 // parameters
                 const double2 rp0 = rot[bP0], rr0 = rot[bR0], rp1 = rot[bP1], rr1 = rot[bR1];
                 const int2 ip0 = rpq[bP0], ir0 = rpq[bR0], ip1 = rpq[bP1], ir1 = rpq[bR1];
@@ -290,12 +303,12 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
                 const bool do1 = has1 && (rp1.y != 0.0 || rr1.y != 0.0);
                 const bool doC = hasC && rc.y != 0.0;
                 // data
-                double b0[4] = {0, 0, 0, 0}, b1[4] = {0, 0, 0, 0}, vp[8], vq[8];
+                double b0[4] = {0, 0, 0, 0}, b1[4] = {0, 0, 0, 0}, vp[kRU], vq[kRU];
                 if (do0) load_block(A, lda, ip0, ir0, b0);
                 if (do1) load_block(A, lda, ip1, ir1, b1);
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int i = ic + 8 * u;
+                for (int u = 0; u < kRU; ++u) {
+                    const int i = ic + kRG * u;
                     const bool ok = doC && i < nrow;
                     vp[u] = ok ? Cv[(size_t)i * lda + pc.x] : 0.0;
                     vq[u] = ok ? Cv[(size_t)i * lda + pc.y] : 0.0;
@@ -316,8 +329,8 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
                 if (doC) {
                     const double c = rc.x, s = rc.y;
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int i = ic + 8 * u;
+                    for (int u = 0; u < kRU; ++u) {
+                        const int i = ic + kRG * u;
                         if (i < nrow) {
                             Cv[(size_t)i * lda + pc.x] = c * vp[u] - s * vq[u];
                             Cv[(size_t)i * lda + pc.y] = s * vp[u] + c * vq[u];
@@ -325,7 +338,7 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
                     }
                 }
                 // larger systems: remaining blocks, rows and column pairs
-                for (int b = threadIdx.x + 2 * QX_BLOCK; b < nblk; b += QX_BLOCK) {
+                for (int b = threadIdx.x + 2 * kNT; b < nblk; b += kNT) {
                     int P, R;
                     tri_index(b, P, R);
                     const double2 xp = rot[P], xr = rot[R];
@@ -336,11 +349,17 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
                     rotate_block(xp, xr, P == R, bb, nn);
                     store_block(A, lda, jp, jr, P == R, nn);
                 }
-                for (int k = kc; k < half; k += 32) {
+                if (doC)
+                    for (int i = ic + 64; i < nrow; i += kRG) {
+                        const double xv = Cv[(size_t)i * lda + pc.x], yv = Cv[(size_t)i * lda + pc.y];
+                        Cv[(size_t)i * lda + pc.x] = rc.x * xv - rc.y * yv;
+                        Cv[(size_t)i * lda + pc.y] = rc.y * xv + rc.x * yv;
+                    }
+                for (int k = kc + 32; k < half; k += 32) {
                     const double2 cs = rot[k];
                     if (cs.y == 0.0) continue;
                     const int2 pq = rpq[k];
-                    for (int i = (k == kc ? ic + 64 : ic); i < nrow; i += 8) {
+                    for (int i = ic; i < nrow; i += kRG) {
                         const double xv = Cv[(size_t)i * lda + pq.x], yv = Cv[(size_t)i * lda + pq.y];
                         Cv[(size_t)i * lda + pq.x] = cs.x * xv - cs.y * yv;
                         Cv[(size_t)i * lda + pq.y] = cs.y * xv + cs.x * yv;
@@ -354,6 +373,7 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
         if (threadIdx.x == 0) s_any[0] = 0;  // set by the sweep's last round
         if (offdiag_max(A, m, lda, red) < tol) break;
     }
+    QX_PT_FLUSH(jt);
     return sweep;
 }
 
@@ -420,7 +440,7 @@ struct SysShared {
     double mrad[QX_MAX_NAT];
     double qat[QX_MAX_NAT], vat[QX_MAX_NAT];
     double zr[kMaxRef], dzr[kMaxRef], yr[kMaxRef];
-    double red[8 * (QX_BLOCK / 64)];
+    double red[8 * (kNT / 64)];
     double eparts[6];  // es2, es3, aes, axc, d4, unused
 };
 
@@ -440,13 +460,13 @@ QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, con
     const double* dp = x + nsh;
     const double* qp = x + nsh + 3 * nat;
     QX_PT_INIT(pt)
-    for (int a = tid; a < nat; a += QX_BLOCK) {
+    for (int a = tid; a < nat; a += kNT) {
         double q = 0.0;
         for (int s = ss.shoff[a]; s < ss.shoff[a + 1]; ++s) q += qsh[s];
         ss.qat[a] = q;
     }
     __syncthreads();
-    for (int k = tid; k < nd; k += QX_BLOCK) {
+    for (int k = tid; k < nd; k += kNT) {
         int a = 0;
         while (ss.refoff[a + 1] <= k) ++a;
         const ElemParams& e = g_tab.e[ss.sel[a]];
@@ -511,7 +531,7 @@ QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, con
             ss.vat[i] = v0;
         }
     } else {
-        const int t0 = tid - 64, nt = QX_BLOCK - 64;
+        const int t0 = tid - 64, nt = kNT - 64;
         const double* WD = B.wd4 + si.d4_off;
         for (int k = t0; k < nd; k += nt) {
             double y = 0.0;
@@ -534,7 +554,7 @@ QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, con
     }
     __syncthreads();
     QX_PT(22, pt);
-    for (int a = tid; a < nsh; a += QX_BLOCK) {
+    for (int a = tid; a < nsh; a += kNT) {
         const int at = ss.sh[a].at;
         double vdisp = 0.0;
         for (int k = ss.refoff[at]; k < ss.refoff[at + 1]; ++k) vdisp += -ss.dzr[k] * ss.yr[k];
@@ -546,13 +566,14 @@ QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, con
         for (int k = 0; k < 5; ++k) ss.eparts[k] = e5[k];
     __syncthreads();
     QX_PT(23, pt);
+    QX_PT_FLUSH(pt);
 }
 
 QX_DEV void load_system(const SysInfo& si, const BatchPtrs& B, SysShared& ss)
 {
-    const int tid = threadIdx.x, nat = si.nat;
-    for (int i = tid; i < 3 * nat; i += QX_BLOCK) ss.sx[i] = B.xyz[3 * (size_t)si.atom_off + i];
-    for (int i = tid; i < nat; i += QX_BLOCK) {
+    const int tid = threadIdx.x, nt = blockDim.x, nat = si.nat;
+    for (int i = tid; i < 3 * nat; i += nt) ss.sx[i] = B.xyz[3 * (size_t)si.atom_off + i];
+    for (int i = tid; i < nat; i += nt) {
         ss.sel[i] = B.elem[si.atom_off + i];
         ss.mrad[i] = B.mrad[si.atom_off + i];
     }
@@ -610,7 +631,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
     QX_PH_INIT
     // initial moments (linear or quadratic extrapolation over the previous
     // steps in MD) and basis C = X U (padding column zero)
-    for (int i = tid; i < nx; i += QX_BLOCK) {
+    for (int i = tid; i < nx; i += kNT) {
         double v = 0.0;
         if (warm >= 3 && opt.extrapolate >= 2) v = 3.0 * (xg[i] - xp[i]) + xpp[i];
         else if (warm >= 2 && opt.extrapolate >= 1) v = 2.0 * xg[i] - xp[i];
@@ -623,10 +644,10 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         block_gemm<false, false>(n, n, n, B.X + si.mat_off, ld, B.U + si.mat_off, ld, Cc, lb, nullptr);
     } else {
         const double* X = B.X + si.mat_off;
-        for (int p = tid; p < n * n; p += QX_BLOCK) Cc[(size_t)(p / n) * lb + p % n] = X[(size_t)(p / n) * ld + p % n];
+        for (int p = tid; p < n * n; p += kNT) Cc[(size_t)(p / n) * lb + p % n] = X[(size_t)(p / n) * ld + p % n];
     }
     if (m > n)
-        for (int i = tid; i < n; i += QX_BLOCK) Cc[(size_t)i * lb + n] = 0.0;
+        for (int i = tid; i < n; i += kNT) Cc[(size_t)i * lb + n] = 0.0;
     if (tid == 0) s_e[1] = -1e300;
     __syncthreads();
     QX_PH(11);
@@ -641,6 +662,11 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
     double* hdql = hql + nx;
     double* hbeta = hdql + nx;
     double* hgam = hbeta + (size_t)mem * mem;
+    // secant pairs form a ring of `ring` slots; with opt.keep > 0 a warm
+    // start continues from the pairs of the previous call (count in hcnt)
+    const int ring = (opt.keep > 0 && opt.keep < mem) ? opt.keep : mem;
+    double* hcnt = hgam + mem;
+    int hcount = (opt.keep > 0 && warm >= 1) ? (int)hcnt[0] : 0;
 
     const double nalpha = 0.5 * si.nel;
     double eold = 0.0, etot = 0.0, ts = 0.0, eband = 0.0, errp = 1e-3;
@@ -655,11 +681,11 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
                          w3 = vd[3 * am + 2];
             const double* q = vq + 6 * am;
             const double w4 = q[0], w5 = q[1], w6 = q[2], w7 = q[3], w8 = q[4], w9 = q[5];
-            for (int nu0 = wave; nu0 < n; nu0 += 4 * (QX_BLOCK / 64)) {
+            for (int nu0 = wave; nu0 < n; nu0 += 4 * (kNT / 64)) {
                 double g[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int nu = nu0 + u * (QX_BLOCK / 64);
+                    const int nu = nu0 + u * (kNT / 64);
                     const size_t o = (size_t)(nu < n ? nu : 0) * ld + mu;
                     g[u] = S[o] * w0 + Dm[o] * w1 + Dm[ms + o] * w2 + Dm[2 * ms + o] * w3 +
                            Qm[o] * w4 + Qm[ms + o] * w5 + Qm[2 * ms + o] * w6 +
@@ -667,13 +693,13 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int nu = nu0 + u * (QX_BLOCK / 64);
+                    const int nu = nu0 + u * (kNT / 64);
                     if (nu < n) Ft[(size_t)nu * lb + mu] = g[u];
                 }
             }
         }
         __syncthreads();
-        for (int p = tid; p < n * n; p += QX_BLOCK) {
+        for (int p = tid; p < n * n; p += kNT) {
             const int mu = p / n, nu = p % n;
             if (nu < mu) continue;
             const double gmn = Ft[(size_t)mu * lb + nu], gnm = Ft[(size_t)nu * lb + mu];
@@ -688,7 +714,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         __syncthreads();
         block_gemm<true, false>(n, n, n, Cc, lb, P, lb, A, lb, nullptr);
         if (m > n)
-            for (int j = tid; j < m; j += QX_BLOCK) {
+            for (int j = tid; j < m; j += kNT) {
                 A[(size_t)n * lb + j] = 0.0;
                 A[(size_t)j * lb + n] = 0.0;
             }
@@ -700,7 +726,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         QX_PH(3);
         QX_PH_ADD(12, nsweep);
         (void)nsweep;
-        for (int i = tid; i < n; i += QX_BLOCK) eps[i] = A[(size_t)i * lb + i];
+        for (int i = tid; i < n; i += kNT) eps[i] = A[(size_t)i * lb + i];
         __syncthreads();
         if (tid < 64) {
             double mu = s_e[1];
@@ -721,7 +747,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         __syncthreads();
         QX_PH(5);
         // row sums of P o M for M = S, D (3), Q (6), H0 as coalesced column sweeps
-        for (int t = tid; t < kRs * n; t += QX_BLOCK) {
+        for (int t = tid; t < kRs * n; t += kNT) {
             const int c = t / n, mu = t - c * n;
             const double* M = c == 0 ? S : c < 4 ? Dm + (size_t)(c - 1) * ms : c < 10 ? Qm + (size_t)(c - 4) * ms : H;
             double acc = 0.0;
@@ -730,13 +756,13 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
             rs[(size_t)mu * kRs + c] = acc;
         }
         __syncthreads();
-        for (int s = tid; s < nsh; s += QX_BLOCK) {
+        for (int s = tid; s < nsh; s += kNT) {
             const ShellDesc d = ss.sh[s];
             double pop = 0.0;
             for (int a = 0; a < d.nao; ++a) pop += rs[(size_t)(d.ao0 + a) * kRs];
             xout[s] = g_tab.e[d.el].refocc[d.k] - pop;
         }
-        for (int t = tid; t < 9 * nat; t += QX_BLOCK) {
+        for (int t = tid; t < 9 * nat; t += kNT) {
             const int a = t / 9, c = t % 9;
             double v = 0.0;
             for (int mu = ss.aooff[a]; mu < ss.aooff[a + 1]; ++mu) v += rs[(size_t)mu * kRs + 1 + c];
@@ -744,12 +770,12 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
             else xout[nsh + 3 * nat + 6 * a + (c - 3)] = -v;
         }
         double red2[2] = {0.0, 0.0};
-        for (int mu = tid; mu < n; mu += QX_BLOCK) red2[0] += rs[(size_t)mu * kRs + 10];
+        for (int mu = tid; mu < n; mu += kNT) red2[0] += rs[(size_t)mu * kRs + 10];
         __syncthreads();
         QX_PH(6);
         potentials(si, B, ss, xout, vs, vd, vq);
         QX_PH(7);
-        for (int i = tid; i < nx; i += QX_BLOCK) {
+        for (int i = tid; i < nx; i += kNT) {
             const double dd = xout[i] - xin[i];
             red2[1] += dd * dd;
         }
@@ -767,8 +793,8 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         if (it == opt.maxiter) break;
         QX_PH(8);
         // modified Broyden update of xin with dq = xout - xin (stored into xout)
-        if (it == 1) {
-            for (int i = tid; i < nx; i += QX_BLOCK) {
+        if (it == 1 && hcount == 0) {
+            for (int i = tid; i < nx; i += kNT) {
                 const double dq = xout[i] - xin[i];
                 hdql[i] = dq;
                 hql[i] = xin[i];
@@ -778,45 +804,55 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
             QX_PH(9);
             continue;
         }
-        int it1 = it - 1;
-        if (it1 > mem) it1 = mem;
-        const int k1 = it1 - 1;
-        double nr[2] = {0.0, 0.0};
-        for (int i = tid; i < nx; i += QX_BLOCK) {
-            const double dq = xout[i] - xin[i];
-            xout[i] = dq;
-            const double d = dq - hdql[i];
-            hdf[(size_t)k1 * nx + i] = d;
-            nr[0] += d * d;
-            nr[1] += dq * dq;
+        // a new secant pair from this call's previous iterate (it >= 2) into
+        // ring slot k1; on the first iteration of a continued call only the
+        // kept pairs enter
+        const bool add = it >= 2;
+        const int k1 = hcount % ring;
+        if (add) {
+            double nr[2] = {0.0, 0.0};
+            for (int i = tid; i < nx; i += kNT) {
+                const double dq = xout[i] - xin[i];
+                xout[i] = dq;
+                const double d = dq - hdql[i];
+                hdf[(size_t)k1 * nx + i] = d;
+                nr[0] += d * d;
+                nr[1] += dq * dq;
+            }
+            block_sums<2>(nr, ss.red);
+            const double inv = nr[0] > 0.0 ? 1.0 / sqrt(nr[0]) : 0.0;
+            for (int i = tid; i < nx; i += kNT) {
+                const double d = hdf[(size_t)k1 * nx + i] * inv;
+                hdf[(size_t)k1 * nx + i] = d;
+                hu[(size_t)k1 * nx + i] = opt.damp * d + inv * (xin[i] - hql[i]);
+            }
+            if (tid == 0) {
+                double w = sqrt(nr[1]);
+                w = (w > 0.01 / 100000.0) ? 0.01 / w : 100000.0;
+                if (w < 1.0) w = 1.0;
+                hom[k1] = w;
+            }
+            ++hcount;
+        } else {
+            for (int i = tid; i < nx; i += kNT) xout[i] = xout[i] - xin[i];
         }
-        block_sums<2>(nr, ss.red);
-        const double inv = nr[0] > 0.0 ? 1.0 / sqrt(nr[0]) : 0.0;
-        for (int i = tid; i < nx; i += QX_BLOCK) {
-            const double d = hdf[(size_t)k1 * nx + i] * inv;
-            hdf[(size_t)k1 * nx + i] = d;
-            hu[(size_t)k1 * nx + i] = opt.damp * d + inv * (xin[i] - hql[i]);
-        }
-        if (tid == 0) {
-            double w = sqrt(nr[1]);
-            w = (w > 0.01 / 100000.0) ? 0.01 / w : 100000.0;
-            if (w < 1.0) w = 1.0;
-            hom[k1] = w;
-        }
+        const int it1 = hcount < ring ? hcount : ring;  // live pairs
         __syncthreads();
         // dot products df_i . df_k1 and df_i . dq: wavefront per i
-        for (int i = wave; i < it1; i += QX_BLOCK / 64) {
+        for (int i = wave; i < it1; i += kNT / 64) {
             double s = 0.0, c = 0.0;
             for (int j = lane; j < nx; j += 64) {
                 const double di = hdf[(size_t)i * nx + j];
-                s += di * hdf[(size_t)k1 * nx + j];
+                if (add) s += di * hdf[(size_t)k1 * nx + j];
                 c += di * xout[j];
             }
             s = wave_sum(s);
             c = wave_sum(c);
             if (lane == 0) {
-                ha[(size_t)i * mem + k1] = s;
-                ha[(size_t)k1 * mem + i] = s;
+                if (add) {
+                    ha[(size_t)i * mem + k1] = s;
+                    ha[(size_t)k1 * mem + i] = s;
+                }
                 hgam[i] = c;
             }
         }
@@ -829,13 +865,13 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         double* gm = lds_solve ? Ft + (size_t)it1 * it1 : hgam;
         double* om = lds_solve ? gm + it1 : hom;
         if (lds_solve) {
-            for (int i = tid; i < it1; i += QX_BLOCK) {
+            for (int i = tid; i < it1; i += kNT) {
                 om[i] = hom[i];
                 gm[i] = hgam[i];
             }
             __syncthreads();
         }
-        for (int p = tid; p < it1 * it1; p += QX_BLOCK) {
+        for (int p = tid; p < it1 * it1; p += kNT) {
             const int i = p / it1, j = p % it1;
             bt[(size_t)i * lbt + j] = om[i] * om[j] * ha[(size_t)i * mem + j] + (i == j ? 1e-4 : 0.0);
         }
@@ -866,7 +902,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
             for (int i = 0; i < it1; ++i) gm[i] *= om[i];
         }
         __syncthreads();
-        for (int j = tid; j < nx; j += QX_BLOCK) {
+        for (int j = tid; j < nx; j += kNT) {
             const double dq = xout[j];
             double q = xin[j];
             hdql[j] = dq;
@@ -879,23 +915,24 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         QX_PH(9);
     }
     QX_PH_ADD(13, it > opt.maxiter ? opt.maxiter : it);
+    if (opt.keep > 0 && tid == 0) hcnt[0] = (double)hcount;
     // results: moments, charges, potentials at the output moments
-    for (int i = tid; i < nx; i += QX_BLOCK) {
+    for (int i = tid; i < nx; i += kNT) {
         xg[i] = xout[i];
         B.xo[si.x_off + i] = xout[i];
     }
-    for (int a = tid; a < nat; a += QX_BLOCK) B.qat[si.atom_off + a] = ss.qat[a];
-    for (int s = tid; s < nsh; s += QX_BLOCK) B.vs[si.shell_off + s] = vs[s];
-    for (int i = tid; i < 3 * nat; i += QX_BLOCK) B.vd[3 * (size_t)si.atom_off + i] = vd[i];
-    for (int i = tid; i < 6 * nat; i += QX_BLOCK) B.vq[6 * (size_t)si.atom_off + i] = vq[i];
-    for (int i = tid; i < n; i += QX_BLOCK) {
+    for (int a = tid; a < nat; a += kNT) B.qat[si.atom_off + a] = ss.qat[a];
+    for (int s = tid; s < nsh; s += kNT) B.vs[si.shell_off + s] = vs[s];
+    for (int i = tid; i < 3 * nat; i += kNT) B.vd[3 * (size_t)si.atom_off + i] = vd[i];
+    for (int i = tid; i < 6 * nat; i += kNT) B.vq[6 * (size_t)si.atom_off + i] = vq[i];
+    for (int i = tid; i < n; i += kNT) {
         B.eps[(size_t)sys * QX_MAX_NAO + i] = eps[i];
         B.focc[(size_t)sys * QX_MAX_NAO + i] = focc[i];
         f2[i] = 2.0 * focc[i] * eps[i];
     }
     double* Pg = B.P + si.mat_off;
     if (P != Pg)
-        for (int p = tid; p < n * n; p += QX_BLOCK) Pg[(size_t)(p / n) * ld + p % n] = P[(size_t)(p / n) * lb + p % n];
+        for (int p = tid; p < n * n; p += kNT) Pg[(size_t)(p / n) * ld + p % n] = P[(size_t)(p / n) * lb + p % n];
     __syncthreads();
     // W = C diag(2 f e) C^T ; U = L^T C for the next warm start
     block_gemm<false, true>(n, n, n, Cc, lb, Cc, lb, B.W + si.mat_off, ld, f2);
@@ -920,7 +957,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
     }
 }
 
-__global__ __launch_bounds__(QX_BLOCK) void k_scc(BatchPtrs B, SccOpts opt)
+__global__ __launch_bounds__(kNT) void k_scc(BatchPtrs B, SccOpts opt)
 {
     const int sys = blockIdx.x;
     if (B.active && !B.active[sys]) return;

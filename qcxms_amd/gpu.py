@@ -200,7 +200,7 @@ class MdParams(C.Structure):
     _fields_ = [("tstep", C.c_double), ("etempin", C.c_double), ("ax", C.c_double),
                 ("ieetemp", C.c_double), ("nfragexit", C.c_int32), ("max_steps", C.c_int32),
                 ("use_graph", C.c_int32), ("extrapolate", C.c_int32), ("econv", C.c_double),
-                ("pconv", C.c_double)]
+                ("pconv", C.c_double), ("broyden_keep", C.c_int32), ("pad", C.c_int32)]
 
 
 class MdResult(C.Structure):
@@ -242,7 +242,7 @@ class MdEnsemble:
 
     def __init__(self, trajectories, tstep=0.5 * 41.3413733365614, max_steps=2000,
                  nfragexit=3, etempin=-1.0, ax=0.0, ieetemp=0.0, use_graph=True,
-                 econv=0.0, pconv=0.0, extrapolate=0):
+                 econv=0.0, pconv=0.0, extrapolate=0, broyden_keep=0):
         L = library()
         _md_sigs(L)
         nums = [np.asarray(t["num"], dtype=np.int32) for t in trajectories]
@@ -262,7 +262,7 @@ class MdEnsemble:
         eimp = np.array([float(t["eimp"]) for t in trajectories])
         tadd = np.array([float(t["tadd"]) for t in trajectories])
         p = MdParams(tstep, etempin, ax, ieetemp, nfragexit, max_steps, int(bool(use_graph)),
-                     int(extrapolate), econv, pconv)
+                     int(extrapolate), econv, pconv, int(broyden_keep), 0)
         h = L.qcxms_md_create(self.ntraj, _p(self.offsets, _ip), _p(self.num, _ip), _p(charge, _ip),
                               _p(mass), _p(imass, _ip), _p(xyz), _p(velo), _p(velof), _p(eimp),
                               _p(tadd), C.byref(p))
