@@ -292,6 +292,10 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
             __syncthreads();
             QX_PT(16, jt);
             QX_PT_COUNT(jt, 3);
+#ifdef QX_PHASE_TIMING
+            if (threadIdx.x == 0)
+                for (int k = 0; k < half; ++k) jt_acc[5] += rot[k].y != 0.0 ? 1 : 0;
+#endif
             if (s_any[f]) {
                 QX_PT_COUNT(jt, 2);
                 // parameters
@@ -349,17 +353,11 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
                     rotate_block(xp, xr, P == R, bb, nn);
                     store_block(A, lda, jp, jr, P == R, nn);
                 }
-                if (doC)
-                    for (int i = ic + 64; i < nrow; i += kRG) {
-                        const double xv = Cv[(size_t)i * lda + pc.x], yv = Cv[(size_t)i * lda + pc.y];
-                        Cv[(size_t)i * lda + pc.x] = rc.x * xv - rc.y * yv;
-                        Cv[(size_t)i * lda + pc.y] = rc.y * xv + rc.x * yv;
-                    }
-                for (int k = kc + 32; k < half; k += 32) {
+                for (int k = kc; k < half; k += 32) {
                     const double2 cs = rot[k];
                     if (cs.y == 0.0) continue;
                     const int2 pq = rpq[k];
-                    for (int i = ic; i < nrow; i += kRG) {
+                    for (int i = (k == kc ? ic + 64 : ic); i < nrow; i += kRG) {
                         const double xv = Cv[(size_t)i * lda + pq.x], yv = Cv[(size_t)i * lda + pq.y];
                         Cv[(size_t)i * lda + pq.x] = cs.x * xv - cs.y * yv;
                         Cv[(size_t)i * lda + pq.y] = cs.y * xv + cs.x * yv;
@@ -476,7 +474,7 @@ QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, con
         ss.dzr[k] = d4_dzeta(gfn2::kD4Ga, c, e.refq[r] + e.zeff, ss.qat[a] + e.zeff);
     }
     __syncthreads();
-    QX_PT(21, pt);
+    QX_PT(22, pt);
     double eaes = 0.0, eaxc = 0.0, es2 = 0.0, es3 = 0.0, ed4 = 0.0;
     if (wave == 0) {
         const int i = lane;
