@@ -183,7 +183,7 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--ntraj", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=25)
+    ap.add_argument("--cpu-steps", type=int, default=250)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-one-per-launch", action="store_true")
     ap.add_argument("--no-graph-check", action="store_true",
