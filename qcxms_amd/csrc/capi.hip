@@ -651,6 +651,9 @@ int qcxms_batch_set_scc(qcxms_batch* b, double econv, double pconv, int32_t maxi
 {
     if (!b) return fail("null batch");
     if (maxiter < 2 || maxiter > b->opts.mem) return fail("maxiter must lie in [2, 100]");
+    if (!(econv > 0.0) || !(pconv > 0.0) || !(jacobi_tol > 0.0))
+        return fail("econv, pconv and jacobi_tol must be positive");
+    if (!(damping > 0.0 && damping <= 1.0)) return fail("damping must lie in (0, 1]");
     b->opts.econv = econv;
     b->opts.pconv = pconv;
     b->opts.maxiter = maxiter;

@@ -121,6 +121,54 @@ QX_DEV void block_gemm(int M, int N, int K, const double* A, int lda, const doub
     }
 }
 
+// C[i][j] = sum_k opA[i][k] * d[k] * opB[k][j] on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64).  A wavefront computes a 16 x 64 strip as four
+// 16 x 16 tiles sharing the A operand; operand lanes: A[i0 + l%16][k0 + l/16],
+// B[k0 + l/16][j0 + l%16]; accumulator element r of lane l is
+// C[i0 + 4r + l/16][j0 + l%16].  Out-of-range rows, columns and k are zero.
+typedef double qx_d4 __attribute__((ext_vector_type(4)));
+template <bool TA, bool TB>
+QX_DEV void mfma_gemm(int M, int N, int K, const double* A, int lda, const double* Bm, int ldb,
+                      double* Cm, int ldc, const double* d)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int tm = (M + 15) >> 4, tc = (N + 63) >> 6;
+    for (int t = wave; t < tm * tc; t += kNT / 64) {
+        const int i0 = (t / tc) * 16, jc = (t % tc) * 64;
+        const int i = i0 + li;
+        qx_d4 acc[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = qx_d4{0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < K; k0 += 4) {
+            const int k = k0 + lk;
+            const bool kin = k < K;
+            double a = 0.0;
+            if (i < M && kin) a = (TA ? A[(size_t)k * lda + i] : A[(size_t)i * lda + k]) * (d ? d[k] : 1.0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = jc + 16 * u + li;
+                double b = 0.0;
+                if (j < N && kin) b = TB ? Bm[(size_t)j * ldb + k] : Bm[(size_t)k * ldb + j];
+                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = i0 + 4 * r + lk, col = jc + 16 * u + li;
+                if (row < M && col < N) Cm[(size_t)row * ldc + col] = acc[u][r];
+            }
+    }
+}
+
+#ifdef QX_VALU_GEMM
+#define QX_GEMM block_gemm
+#else
+#define QX_GEMM mfma_gemm
+#endif
+
 QX_DEV double block_max(double v, double* smem)
 {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -177,6 +225,14 @@ QX_DEV double offdiag_max(const double* A, int m, int lda, double* red)
             if (i != j) mx = fmax(mx, fabs(A[(size_t)i * lda + j]));
     return block_max(mx, red);
 }
+
+// The Jacobi loop is compiled as a separate function so that its registers
+// are allocated independently of the rest of the SCC body.
+#ifdef QX_INLINE_JACOBI
+#define QX_JACOBI_ATTR QX_DEV
+#else
+#define QX_JACOBI_ATTR __device__ __noinline__
+#endif
 
 // pair-ordered 2 x 2 block index b -> (P, R) with R <= P
 QX_DEV void tri_index(int b, int& P, int& R)
@@ -250,7 +306,7 @@ QX_DEV void load_block(const double* A, int lda, int2 ip, int2 ir, double b[4])
 // A thread's first two blocks (P, R) and its column pair are the same in
 // every round, so their parameters and entries are fetched in two LDS round
 // trips (parameters, then data) before any store.
-QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, int maxsweep,
+QX_JACOBI_ATTR int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, int maxsweep,
                   double2* rot, int2* rpq, double* red, int* s_any)
 {
     const int half = m / 2;
@@ -639,7 +695,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         if (warm >= 1) xp[i] = xg[i];
     }
     if (warm) {
-        block_gemm<false, false>(n, n, n, B.X + si.mat_off, ld, B.U + si.mat_off, ld, Cc, lb, nullptr);
+        QX_GEMM<false, false>(n, n, n, B.X + si.mat_off, ld, B.U + si.mat_off, ld, Cc, lb, nullptr);
     } else {
         const double* X = B.X + si.mat_off;
         for (int p = tid; p < n * n; p += kNT) Cc[(size_t)(p / n) * lb + p % n] = X[(size_t)(p / n) * ld + p % n];
@@ -708,9 +764,9 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         __syncthreads();
         QX_PH(1);
         // A = C^T F C (F C staged in the density buffer)
-        block_gemm<false, false>(n, n, n, Ft, lb, Cc, lb, P, lb, nullptr);
+        QX_GEMM<false, false>(n, n, n, Ft, lb, Cc, lb, P, lb, nullptr);
         __syncthreads();
-        block_gemm<true, false>(n, n, n, Cc, lb, P, lb, A, lb, nullptr);
+        QX_GEMM<true, false>(n, n, n, Cc, lb, P, lb, A, lb, nullptr);
         if (m > n)
             for (int j = tid; j < m; j += kNT) {
                 A[(size_t)n * lb + j] = 0.0;
@@ -741,7 +797,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         __syncthreads();
         ts = s_e[0];
         QX_PH(4);
-        block_gemm<false, true>(n, n, n, Cc, lb, Cc, lb, P, lb, f2);
+        QX_GEMM<false, true>(n, n, n, Cc, lb, Cc, lb, P, lb, f2);
         __syncthreads();
         QX_PH(5);
         // row sums of P o M for M = S, D (3), Q (6), H0 as coalesced column sweeps
@@ -933,8 +989,8 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         for (int p = tid; p < n * n; p += kNT) Pg[(size_t)(p / n) * ld + p % n] = P[(size_t)(p / n) * lb + p % n];
     __syncthreads();
     // W = C diag(2 f e) C^T ; U = L^T C for the next warm start
-    block_gemm<false, true>(n, n, n, Cc, lb, Cc, lb, B.W + si.mat_off, ld, f2);
-    block_gemm<true, false>(n, n, n, B.L + si.mat_off, ld, Cc, lb, B.U + si.mat_off, ld, nullptr);
+    QX_GEMM<false, true>(n, n, n, Cc, lb, Cc, lb, B.W + si.mat_off, ld, f2);
+    QX_GEMM<true, false>(n, n, n, B.L + si.mat_off, ld, Cc, lb, B.U + si.mat_off, ld, nullptr);
     QX_PH(10);
     if (tid == 0) {
         double* ep = B.eparts + 10 * sys;
