@@ -80,50 +80,9 @@ __device__ unsigned long long g_phase[kPhaseSys * kPhaseSlots];
     } while (0)
 #endif
 
-// C[i][j] = sum_k opA[i][k] * d[k] * opB[k][j] over i < M, j < N, k < K
-template <bool TA, bool TB>
-QX_DEV void block_gemm(int M, int N, int K, const double* A, int lda, const double* Bm, int ldb,
-                       double* Cm, int ldc, const double* d)
-{
-    constexpr int TR = kNT >= 512 ? 2 : 4;  // rows per register tile (tiles are TR x 4)
-    const int tm = (M + TR - 1) / TR, tn = (N + 3) >> 2;
-    for (int t = threadIdx.x; t < tm * tn; t += kNT) {
-        // rows i0..i0+TR-1, columns jb + b tn: neighbouring lanes read neighbouring columns
-        const int i0 = (t / tn) * TR, jb = t % tn;
-        double acc[TR][4];
-#pragma unroll
-        for (int a = 0; a < TR; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-        for (int k = 0; k < K; ++k) {
-            double av[TR], bv[4];
-            const double dk = d ? d[k] : 1.0;
-#pragma unroll
-            for (int a = 0; a < TR; ++a) {
-                const int i = i0 + a < M ? i0 + a : M - 1;
-                av[a] = (TA ? A[(size_t)k * lda + i] : A[(size_t)i * lda + k]) * dk;
-            }
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int j = jb + b * tn < N ? jb + b * tn : N - 1;
-                bv[b] = TB ? Bm[(size_t)j * ldb + k] : Bm[(size_t)k * ldb + j];
-            }
-#pragma unroll
-            for (int a = 0; a < TR; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
-        }
-#pragma unroll
-        for (int a = 0; a < TR; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-                if (i0 + a < M && jb + b * tn < N) Cm[(size_t)(i0 + a) * ldc + jb + b * tn] = acc[a][b];
-    }
-}
-
 // C[i][j] = sum_k opA[i][k] * d[k] * opB[k][j] on the FP64 matrix cores
-// (v_mfma_f64_16x16x4f64).  A wavefront computes a 16 x 64 strip as four
-// 16 x 16 tiles sharing the A operand; operand lanes: A[i0 + l%16][k0 + l/16],
+// (v_mfma_f64_16x16x4f64).  A wavefront computes a strip of two 16 x 16
+// tiles sharing the A operand; operand lanes: A[i0 + l%16][k0 + l/16],
 // B[k0 + l/16][j0 + l%16]; accumulator element r of lane l is
 // C[i0 + 4r + l/16][j0 + l%16].  Out-of-range rows, columns and k are zero.
 typedef double qx_d4 __attribute__((ext_vector_type(4)));
@@ -131,22 +90,23 @@ template <bool TA, bool TB>
 QX_DEV void mfma_gemm(int M, int N, int K, const double* A, int lda, const double* Bm, int ldb,
                       double* Cm, int ldc, const double* d)
 {
+    constexpr int NU = 2;  // 16 x 16 tiles per strip (more tiles cost the SCC body registers)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int tm = (M + 15) >> 4, tc = (N + 63) >> 6;
+    const int tm = (M + 15) >> 4, tc = (N + 16 * NU - 1) / (16 * NU);
     for (int t = wave; t < tm * tc; t += kNT / 64) {
-        const int i0 = (t / tc) * 16, jc = (t % tc) * 64;
+        const int i0 = (t / tc) * 16, jc = (t % tc) * 16 * NU;
         const int i = i0 + li;
-        qx_d4 acc[4];
+        qx_d4 acc[NU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] = qx_d4{0.0, 0.0, 0.0, 0.0};
+        for (int u = 0; u < NU; ++u) acc[u] = qx_d4{0.0, 0.0, 0.0, 0.0};
         for (int k0 = 0; k0 < K; k0 += 4) {
             const int k = k0 + lk;
             const bool kin = k < K;
             double a = 0.0;
             if (i < M && kin) a = (TA ? A[(size_t)k * lda + i] : A[(size_t)i * lda + k]) * (d ? d[k] : 1.0);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < NU; ++u) {
                 const int j = jc + 16 * u + li;
                 double b = 0.0;
                 if (j < N && kin) b = TB ? Bm[(size_t)j * ldb + k] : Bm[(size_t)k * ldb + j];
@@ -154,7 +114,7 @@ QX_DEV void mfma_gemm(int M, int N, int K, const double* A, int lda, const doubl
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < NU; ++u)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = i0 + 4 * r + lk, col = jc + 16 * u + li;
@@ -162,12 +122,6 @@ QX_DEV void mfma_gemm(int M, int N, int K, const double* A, int lda, const doubl
             }
     }
 }
-
-#ifdef QX_VALU_GEMM
-#define QX_GEMM block_gemm
-#else
-#define QX_GEMM mfma_gemm
-#endif
 
 QX_DEV double block_max(double v, double* smem)
 {
@@ -225,14 +179,6 @@ QX_DEV double offdiag_max(const double* A, int m, int lda, double* red)
             if (i != j) mx = fmax(mx, fabs(A[(size_t)i * lda + j]));
     return block_max(mx, red);
 }
-
-// The Jacobi loop is compiled as a separate function so that its registers
-// are allocated independently of the rest of the SCC body.
-#ifdef QX_INLINE_JACOBI
-#define QX_JACOBI_ATTR QX_DEV
-#else
-#define QX_JACOBI_ATTR __device__ __noinline__
-#endif
 
 // pair-ordered 2 x 2 block index b -> (P, R) with R <= P
 QX_DEV void tri_index(int b, int& P, int& R)
@@ -306,7 +252,7 @@ QX_DEV void load_block(const double* A, int lda, int2 ip, int2 ir, double b[4])
 // A thread's first two blocks (P, R) and its column pair are the same in
 // every round, so their parameters and entries are fetched in two LDS round
 // trips (parameters, then data) before any store.
-QX_JACOBI_ATTR int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, int maxsweep,
+QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, int maxsweep,
                   double2* rot, int2* rpq, double* red, int* s_any)
 {
     const int half = m / 2;
@@ -695,7 +641,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         if (warm >= 1) xp[i] = xg[i];
     }
     if (warm) {
-        QX_GEMM<false, false>(n, n, n, B.X + si.mat_off, ld, B.U + si.mat_off, ld, Cc, lb, nullptr);
+        mfma_gemm<false, false>(n, n, n, B.X + si.mat_off, ld, B.U + si.mat_off, ld, Cc, lb, nullptr);
     } else {
         const double* X = B.X + si.mat_off;
         for (int p = tid; p < n * n; p += kNT) Cc[(size_t)(p / n) * lb + p % n] = X[(size_t)(p / n) * ld + p % n];
@@ -764,9 +710,9 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         __syncthreads();
         QX_PH(1);
         // A = C^T F C (F C staged in the density buffer)
-        QX_GEMM<false, false>(n, n, n, Ft, lb, Cc, lb, P, lb, nullptr);
+        mfma_gemm<false, false>(n, n, n, Ft, lb, Cc, lb, P, lb, nullptr);
         __syncthreads();
-        QX_GEMM<true, false>(n, n, n, Cc, lb, P, lb, A, lb, nullptr);
+        mfma_gemm<true, false>(n, n, n, Cc, lb, P, lb, A, lb, nullptr);
         if (m > n)
             for (int j = tid; j < m; j += kNT) {
                 A[(size_t)n * lb + j] = 0.0;
@@ -797,7 +743,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         __syncthreads();
         ts = s_e[0];
         QX_PH(4);
-        QX_GEMM<false, true>(n, n, n, Cc, lb, Cc, lb, P, lb, f2);
+        mfma_gemm<false, true>(n, n, n, Cc, lb, Cc, lb, P, lb, f2);
         __syncthreads();
         QX_PH(5);
         // row sums of P o M for M = S, D (3), Q (6), H0 as coalesced column sweeps
@@ -989,8 +935,8 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         for (int p = tid; p < n * n; p += kNT) Pg[(size_t)(p / n) * ld + p % n] = P[(size_t)(p / n) * lb + p % n];
     __syncthreads();
     // W = C diag(2 f e) C^T ; U = L^T C for the next warm start
-    QX_GEMM<false, true>(n, n, n, Cc, lb, Cc, lb, B.W + si.mat_off, ld, f2);
-    QX_GEMM<true, false>(n, n, n, B.L + si.mat_off, ld, Cc, lb, B.U + si.mat_off, ld, nullptr);
+    mfma_gemm<false, true>(n, n, n, Cc, lb, Cc, lb, B.W + si.mat_off, ld, f2);
+    mfma_gemm<true, false>(n, n, n, B.L + si.mat_off, ld, Cc, lb, B.U + si.mat_off, ld, nullptr);
     QX_PH(10);
     if (tid == 0) {
         double* ep = B.eparts + 10 * sys;
