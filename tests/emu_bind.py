@@ -1,0 +1,53 @@
+"""Runs qcxms_egrad_batch through the CPU-emulated build of the HIP library
+(tests/cpu_emu: kernels on host threads under AddressSanitizer).  Test
+infrastructure for bounds checking the kernels without a GPU."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EMU_DIR = os.path.join(ROOT, "tests", "cpu_emu")
+EMU_BIN = os.path.join(EMU_DIR, "_build", "egrad_emu")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", EMU_DIR], check=True)
+
+
+def egrad_batch(systems, xyz_list, etemp, method=2, timeout=1800):
+    """systems: [(num, charge[, multiplicity])]; returns (rc, energy, [grad],
+    [qat], stat, stderr)."""
+    nsys = len(systems)
+    off = np.cumsum([0] + [len(s[0]) for s in systems]).astype(np.int32)
+    num = np.concatenate([np.asarray(s[0], dtype=np.int32) for s in systems])
+    xyz = np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3) for x in xyz_list])
+    chg = np.array([s[1] for s in systems], dtype=np.int32)
+    mult = np.array([s[2] if len(s) > 2 else 1 for s in systems], dtype=np.int32)
+    et = np.broadcast_to(np.asarray(etemp, dtype=np.float64), (nsys,)).copy()
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([nsys, method], dtype=np.int32).tofile(f)
+            off.tofile(f)
+            num.tofile(f)
+            xyz.tofile(f)
+            chg.tofile(f)
+            mult.tofile(f)
+            et.tofile(f)
+        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+        p = subprocess.run([EMU_BIN, fin, fout], capture_output=True, text=True, timeout=timeout, env=env)
+        if p.returncode != 0:
+            raise RuntimeError(f"emulated run failed ({p.returncode}):\n{p.stderr[-6000:]}")
+        raw = open(fout, "rb").read()
+    nat = int(off[-1])
+    rc = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    o = 4
+    e = np.frombuffer(raw, np.float64, nsys, o).copy(); o += 8 * nsys
+    g = np.frombuffer(raw, np.float64, 3 * nat, o).reshape(nat, 3).copy(); o += 24 * nat
+    q = np.frombuffer(raw, np.float64, nat, o).copy(); o += 8 * nat
+    st = np.frombuffer(raw, np.int32, nsys, o).copy()
+    grads = [g[off[i]:off[i + 1]] for i in range(nsys)]
+    qs = [q[off[i]:off[i + 1]] for i in range(nsys)]
+    return rc, e, grads, qs, st, p.stderr
