@@ -140,6 +140,11 @@ int qcxms_md_run(qcxms_md *md, int32_t nsteps, int32_t *running);
 /* Copy state back: any pointer may be NULL.  list = fragment labels. */
 int qcxms_md_read(qcxms_md *md, double *xyz, double *velo, double *grad, double *qat,
                   int32_t *list, qcxms_md_result *results);
+/* Averages over the last avdump (50) steps that md returns to main
+ * (md.f90:355-363, 691-698, EI branch): axyz[3*natoms] = average geometry
+ * (Bohr), the structure the fragment IPs are computed at, and
+ * achrg[natoms] = average Mulliken charges.  Either pointer may be NULL. */
+int qcxms_md_read_averages(qcxms_md *md, double *axyz, double *achrg);
 /* Timing hooks of the ensemble's underlying batch (see qcxms_batch_kernel_time). */
 qcxms_batch *qcxms_md_batch(qcxms_md *md);
 
@@ -159,6 +164,58 @@ int qcxms_leapfrog_batch(int32_t nsys, const int32_t *offsets, const double *gra
 int qcxms_impactscale_batch(int32_t nsys, const int32_t *offsets, double *velo,
                             const double *mass, const double *velof, const double *eimp,
                             const double *ff, const double *e0, int32_t *iterations);
+
+/* ---- post-MD fragment analysis (once per finished trajectory) ---- */
+/* analyse (src/analyse.f90:14-448; GFN2 IPs, iprog 8: etemp 300 K) for many
+ * trajectories at once: the neutral and ion single points of every fragment
+ * run as one ragged batch on the GPU.  Trajectory t owns atoms
+ * offsets[t] .. offsets[t+1]-1 with averaged coordinates axyz (Bohr) and
+ * fragment labels list (1..nfrag[t], nfrag <= 10); mchrg[t] is its total
+ * charge (1 <= |mchrg| <= 3).  Out: fragip[30*t + 10*(j-1) + (i-1)] = IP
+ * (EA when mchrg < 0) of fragment i for charge j, eV, zero where not computed
+ * (one fragment: analyse.f90:125-130); ipok[t] = 0 when an energy failed or an
+ * IP left the bounds of analyse.f90:383-400 (IPs then zeroed as after the
+ * last attempt, analyse.f90:406-414; the reference's third attempt would use
+ * IPEA1, which is not built).  A fragment left without electrons takes eself
+ * (iniqm.f90:430-451); a charge that needs a negative electron count or an SCC
+ * failure gives stat[t] = -1 (the reference stops there, iniqm.f90:411-413). */
+int qcxms_fragment_ips_batch(int32_t ntraj, const int32_t *offsets, const int32_t *num,
+                             const double *axyz, const int32_t *list, const int32_t *nfrag,
+                             const int32_t *mchrg, double *fragip, int32_t *ipok, int32_t *stat);
+
+/* State of one trajectory for manage_fragments. */
+typedef struct {
+    int32_t mchrg;      /* in/out: total charge (write_fragments.f90:302-310) */
+    int32_t nfrag_ok;   /* in/out: run still counted (0 after more than 5 fragments) */
+    double chrgcont;    /* in/out: charge carried by the continued fragment */
+    int32_t nfrag, tcont, ipok, nres, stat, pad;  /* out; tcont 0: no continuation */
+    double fragchrg3[10]; /* IP-Boltzmann fragment charges (fractional) */
+    double fragchrg[10];  /* summed Mulliken charges qat per fragment */
+    double fragm[10];     /* fragment masses (amu) */
+    double fragip[30];    /* IPs as qcxms_fragment_ips_batch */
+    char asave[128];      /* record of the continued fragment ("" when tcont = 0) */
+    char res[10][128];    /* records for qcxms.res, nres of them */
+} qcxms_fragment_report;
+
+/* manage_fragments (src/write_fragments.f90:12-457, EI records) for many
+ * finished trajectories: fragment_structure + fragmass on the final
+ * coordinates xyz (list[natoms] out), IPs of the fragments at the averaged
+ * coordinates axyz (qcxms_md_read_averages), Boltzmann charges at
+ * atlast[t] * btf (boltz, utility.f90:469-498), the continued fragment, the new
+ * total charge and charge contribution, and the qcxms.res records with the
+ * reference's format (F10.7,i3,2i5,2i2,2x,i3,2x,20(i4,i3)).  qat: the run's
+ * averaged Mulliken charges; mass in m_e; imass isotope numbers or <= 0
+ * (NULL: none); itrj[t] trajectory number; isec / maxsec the fragmentation
+ * generation and its limit; at most 64 atoms per trajectory.  The first IP
+ * difference of each fragment is its first IP, relative to the neutral
+ * (write_fragments.f90:119-121 zero fragip(i,1) and then read fragip(i,0), an
+ * element before the array, so the reference's value there depends on heap
+ * contents). */
+int qcxms_manage_fragments_batch(int32_t ntraj, const int32_t *offsets, const int32_t *num,
+                                 const double *xyz, const double *axyz, const double *qat,
+                                 const double *mass, const int32_t *imass, const double *atlast,
+                                 const int32_t *itrj, int32_t isec, int32_t maxsec, double btf,
+                                 int32_t *list, qcxms_fragment_report *report);
 
 #ifdef __cplusplus
 }

@@ -321,10 +321,23 @@ int orc_md_run(orc_md_state *st, const int32_t *iat, const int32_t *imass, const
                const double *velof, double *xyz, double *velo, double *grad, double *achrg,
                int32_t *list, orc_egrad_fn egrad, void *ctx, int steps_limit)
 {
+    return orc_md_run_avg(st, iat, imass, mass, velof, xyz, velo, grad, achrg, list, egrad, ctx,
+                          steps_limit, NULL, NULL);
+}
+
+/* As orc_md_run; also returns the averages md hands back to main
+ * (md.f90:355-363, 691-698, EI branch): axyz = avxyz / kdump and
+ * avg_chrg = avchrg / kdump (either may be NULL). */
+int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, const double *mass,
+                   const double *velof, double *xyz, double *velo, double *grad, double *achrg,
+                   int32_t *list, orc_egrad_fn egrad, void *ctx, int steps_limit, double *axyz,
+                   double *avg_chrg)
+{
     const int nuc = st->nat;
     const int more = 250, avdump = 50;
     double ekin, t, epot;
     double *avchrg = (double *)calloc((size_t)nuc, sizeof(double));
+    double *avxyz = (double *)calloc(3 * (size_t)nuc, sizeof(double));
     st->mdok = 0;
     st->nfrag = 1;
     st->fragstate = 0;
@@ -342,6 +355,7 @@ int orc_md_run(orc_md_state *st, const int32_t *iat, const int32_t *imass, const
     if (epot == 0.0) {
         st->exit_reason = ORC_MD_ERR_SCF;
         free(avchrg);
+        free(avxyz);
         return 0;
     }
     double tav = 0, epav = 0, ekav = 0, edum = 0, eerror = 0, atlast = 0, dtime = 0, ttime = 0;
@@ -378,9 +392,11 @@ int orc_md_run(orc_md_state *st, const int32_t *iat, const int32_t *imass, const
         if (kdump > avdump - 1) {
             kdump = 0;
             for (int i = 0; i < nuc; ++i) avchrg[i] = 0.0;
+            for (int i = 0; i < 3 * nuc; ++i) avxyz[i] = 0.0;
             atlast = 0;
         }
         for (int i = 0; i < nuc; ++i) avchrg[i] = avchrg[i] + achrg[i];
+        for (int i = 0; i < 3 * nuc; ++i) avxyz[i] = avxyz[i] + xyz[i];
         atlast = atlast + t;
 
         orc_leapfrog(nuc, grad, mass, tstep, xyz, velo, &ekin);
@@ -459,6 +475,11 @@ int orc_md_run(orc_md_state *st, const int32_t *iat, const int32_t *imass, const
         st->fragm[i] = fragm[i];
         st->fragT[i] = fragT[i];
     }
+    for (int i = 0; i < nuc; ++i)
+        if (avg_chrg) avg_chrg[i] = (nstep > 0 && kdump) ? avchrg[i] / kdump : 0.0;
+    for (int i = 0; i < 3 * nuc; ++i)
+        if (axyz) axyz[i] = (nstep > 0 && kdump) ? avxyz[i] / kdump : 0.0;
     free(avchrg);
+    free(avxyz);
     return nstep;
 }

@@ -73,6 +73,12 @@ int orc_md_run(orc_md_state *st, const int32_t *iat, const int32_t *imass,
                const double *mass, const double *velof, double *xyz, double *velo,
                double *grad, double *achrg, int32_t *list, orc_egrad_fn egrad, void *ctx,
                int steps_limit);
+/* orc_md_run plus the averages md returns to main: axyz[3*nat] and
+ * avg_chrg[nat] over the last kdump steps (md.f90:691-698); NULL skips. */
+int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass,
+                   const double *mass, const double *velof, double *xyz, double *velo,
+                   double *grad, double *achrg, int32_t *list, orc_egrad_fn egrad, void *ctx,
+                   int steps_limit, double *axyz, double *avg_chrg);
 
 /* ---- GFN2-xTB single point with analytic gradient ---- */
 typedef struct {
@@ -125,6 +131,22 @@ int orc_md_run_batch(int ntraj, const int32_t *offsets, const int32_t *num, cons
                      const double *mass, const double *velof, double *xyz, double *velo,
                      double *grad, double *qat, int32_t *list, orc_md_state *states,
                      const orc_scc_opts *opts, int steps_limit, int nthreads);
+
+/* ---- post-MD fragment analysis (postmd_oracle.c) ---- */
+int orc_nel(int n, const int32_t *iat, int chrg);
+int orc_eself(int n, const int32_t *iat, double *e);
+void orc_boltz(int nfrag, int mchrg, double temp, const double *ip, double *fragchrg2);
+void orc_res_record(double q, int mchrg, int itrj, int icoll, int isec, int j, int l,
+                    const int32_t *idum2, const int32_t *idum1, char *out);
+int orc_analyse_ips(int nuc, const int32_t *iat, const double *axyz, const int32_t *list,
+                    int nfrag, int mchrg, const orc_scc_opts *opts, double *fragip, int *ipok);
+int orc_manage_fragments(int nuc, const int32_t *iat, const double *xyz, const double *axyz,
+                         const double *qat, const double *mass, const int32_t *imass,
+                         double atlast, int itrj, int isec, int maxsec, double btf,
+                         const orc_scc_opts *opts, int *mchrg, double *chrgcont, int *nfrag_ok,
+                         int32_t *list, int *nfrag_out, int *tcont_out, double *fragchrg3,
+                         double *fragchrg, double *fragip, int *ipok, char *asave, char *res,
+                         int *nres);
 
 #ifdef __cplusplus
 }

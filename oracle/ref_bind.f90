@@ -13,6 +13,9 @@
 !   atomic_mass          src/atomic_masses.f90:58-67
 !   zeta/dzeta/trapzd/cngw/d4dim/d4/covncoord/dispgrad
 !                        src/dftd4.f90:292-1337, 1463-1780
+!   gam, gam3            src/dftd4.f90:38-72 (the tables eself uses)
+!   ref_res_record       a record written with the format of
+!                        src/write_fragments.f90:402-441 (the qcxms.res line)
 module qcxms_ref_bind
    use iso_c_binding
    use xtb_mctc_accuracy, only: wp
@@ -184,6 +187,41 @@ subroutine ref_dispgrad(nat, ndim, at, q, xyz, s6, s8, s9, a1, a2, alp, wf, g_a,
    call d4(int(nat), int(ndim), at_, wf, g_a, g_c, cn, gw, c6abns)
    call dispgrad(int(nat), int(ndim), at_, q, xyz, s6, s8, s9, a1, a2, int(alp), wf, &
       & g_a, g_c, c6abns, 3, g, eout)
+end subroutine
+
+function ref_gam(z) result(g) bind(C)
+   integer(c_int), value :: z
+   real(c_double) :: g
+   g = gam(z)
+end function
+
+function ref_gam3(z) result(g) bind(C)
+   integer(c_int), value :: z
+   real(c_double) :: g
+   g = gam3(z)
+end function
+
+! One qcxms.res record, items as write_fragments.f90 passes them: EI
+! (icoll < 0) without the collision number, CID with it.  out receives the
+! record blank-padded to outlen characters.
+subroutine ref_res_record(q, mchrg, itrj, icoll, isec, j, l, idum2, idum1, out, outlen) bind(C)
+   real(c_double), value :: q
+   integer(c_int), value :: mchrg, itrj, icoll, isec, j, l, outlen
+   integer(c_int), intent(in) :: idum2(*), idum1(*)
+   character(kind=c_char), intent(out) :: out(outlen)
+   character(len=400) :: line
+   integer :: m
+   line = ''
+   if (icoll < 0) then
+      write(line,'(F10.7,i3,2i5,2i2,2x,i3,2x,20(i4,i3))') q, mchrg, itrj, isec, j, &
+         & l, (idum2(m), idum1(m), m = 1, l)
+   else
+      write(line,'(F10.7,i3,2i5,2i2,2x,i3,2x,20(i4,i3))') q, mchrg, itrj, icoll, isec, &
+         & j, l, (idum2(m), idum1(m), m = 1, l)
+   endif
+   do m = 1, outlen
+      out(m) = line(m:m)
+   enddo
 end subroutine
 
 end module qcxms_ref_bind

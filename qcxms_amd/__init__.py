@@ -4,10 +4,12 @@ The compute runs in libqcxms_gpu.so (HIP, gfx950) through the C ABI declared
 in include/qcxms_gpu.h; this package is its Python host layer, mirroring the
 reference's Fortran interface (src/tblite.f90 get_xtb_egrad, src/md.f90).
 """
-from .gpu import (EXIT_REASONS, GFN1_XTB, GFN2_XTB, IPEA1_XTB, Batch, LibraryMissing,
-                  MdEnsemble, egrad_batch, egrad_many, fragments_batch, get_xtb_egrad, impactscale_batch,
-                  leapfrog_batch, library, library_path)
+from .gpu import (EXIT_REASONS, GFN1_XTB, GFN2_XTB, IPEA1_XTB, Batch, FragmentReport,
+                  LibraryMissing, MdEnsemble, egrad_batch, egrad_many, fragment_ips_batch,
+                  fragments_batch, get_xtb_egrad, impactscale_batch, leapfrog_batch, library,
+                  library_path, manage_fragments_batch)
 
-__all__ = ["EXIT_REASONS", "GFN1_XTB", "GFN2_XTB", "IPEA1_XTB", "Batch", "LibraryMissing",
-           "MdEnsemble", "egrad_batch", "egrad_many", "fragments_batch", "get_xtb_egrad", "impactscale_batch",
-           "leapfrog_batch", "library", "library_path"]
+__all__ = ["EXIT_REASONS", "GFN1_XTB", "GFN2_XTB", "IPEA1_XTB", "Batch", "FragmentReport",
+           "LibraryMissing", "MdEnsemble", "egrad_batch", "egrad_many", "fragment_ips_batch",
+           "fragments_batch", "get_xtb_egrad", "impactscale_batch", "leapfrog_batch", "library",
+           "library_path", "manage_fragments_batch"]

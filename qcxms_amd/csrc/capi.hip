@@ -902,6 +902,7 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
         (st = mdalloc(md, &d_velof, (size_t)na, velof)) || (st = mdalloc(md, &d_mass, (size_t)na, mass)) ||
         (st = mdalloc(md, &d_imass, (size_t)na, im.data())) || (st = mdalloc(md, &d_num, (size_t)na, nm.data())) ||
         (st = mdalloc(md, &md->m.list, (size_t)na, lst.data())) || (st = mdalloc(md, &md->m.avchrg, (size_t)na)) ||
+        (st = mdalloc(md, &md->m.avxyz, 3 * (size_t)na)) ||
         (st = mdalloc(md, &md->m.active, (size_t)ntraj, act.data()))) {
         delete md;
         return nullptr;
@@ -1022,6 +1023,31 @@ int qcxms_md_read(qcxms_md* md, double* xyz, double* velo, double* grad, double*
     return QCXMS_OK;
 }
 
+int qcxms_md_read_averages(qcxms_md* md, double* axyz, double* achrg)
+{
+    if (!md) return fail("null ensemble");
+    qcxms_batch* b = md->b;
+    const size_t na = b->natoms;
+    std::vector<double> sx(3 * na), sq(na);
+    std::vector<qx::MdTraj> tr(md->ntraj);
+    HIPCHK(hipMemcpyAsync(sx.data(), md->m.avxyz, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(sq.data(), md->m.avchrg, sizeof(double) * na, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(tr.data(), md->m.traj, sizeof(qx::MdTraj) * md->ntraj, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    // md.f90:691-698 (EI: axyz = avxyz / kdump); a trajectory that has not
+    // completed a step yet has nothing averaged and gets zeros
+    for (int s = 0; s < md->ntraj; ++s) {
+        const int kd = tr[s].nstep > 0 ? tr[s].kdump : 0;
+        const SysInfo& si = b->info[s];
+        for (int i = si.atom_off; i < si.atom_off + si.nat; ++i) {
+            if (achrg) achrg[i] = kd ? sq[i] / kd : 0.0;
+            if (axyz)
+                for (int c = 0; c < 3; ++c) axyz[3 * (size_t)i + c] = kd ? sx[3 * (size_t)i + c] / kd : 0.0;
+        }
+    }
+    return QCXMS_OK;
+}
+
 int qcxms_fragments_batch(int32_t nsys, const int32_t* offsets, const int32_t* num, const double* xyz,
                           const double* mass, int32_t* list, int32_t* nfrag, double* fragm)
 {
@@ -1095,6 +1121,234 @@ int qcxms_impactscale_batch(int32_t nsys, const int32_t* offsets, double* velo, 
     HIPCHK(hipMemcpyAsync(velo, d_v, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipMemcpyAsync(iterations, d_it, sizeof(int) * nsys, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
+    return QCXMS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Post-MD fragment analysis: analyse + manage_fragments (EI) for many
+// finished trajectories; the single points of all fragments run as one batch.
+#include "post_md.h"
+
+extern "C" {
+
+int qcxms_fragment_ips_batch(int32_t ntraj, const int32_t* offsets, const int32_t* num, const double* axyz,
+                             const int32_t* list, const int32_t* nfrag, const int32_t* mchrg, double* fragip,
+                             int32_t* ipok, int32_t* stat)
+{
+    using namespace qx_post;
+    g_err.clear();
+    if (ntraj < 0 || !offsets || !num || !axyz || !list || !nfrag || !mchrg || !fragip || !ipok || !stat)
+        return fail("null argument");
+    for (int t = 0; t < ntraj; ++t) {
+        if (offsets[t + 1] < offsets[t]) return fail("offsets must be non-decreasing");
+        if (nfrag[t] < 0 || nfrag[t] > kMaxFrag) return fail("nfrag outside 0..10");
+        if (mchrg[t] == 0 || std::abs(mchrg[t]) > kMaxCharge) return fail("|mchrg| outside 1..3");
+        for (int i = offsets[t]; i < offsets[t + 1]; ++i)
+            if (list[i] < 1 || list[i] > nfrag[t]) return fail("fragment label outside 1..nfrag");
+    }
+    // avg_frag_struc (analyse.f90:452-502): fragment i = its atoms in input
+    // order.  One job per (trajectory, fragment, charge 0..|mchrg|).
+    struct Job {
+        int t, frag, j, charge, sys;
+        double e;
+        int ok;
+    };
+    std::vector<Job> jobs;
+    std::vector<int32_t> s_off{0}, s_num, s_chg, s_mult;
+    std::vector<double> s_xyz, s_et;
+    for (int t = 0; t < ntraj; ++t) {
+        for (int k = 0; k < 10 * kMaxCharge; ++k) fragip[10 * kMaxCharge * (size_t)t + k] = 0.0;
+        ipok[t] = 1;
+        stat[t] = 0;
+        if (nfrag[t] <= 1) continue;  // analyse.f90:125-130: no IPs for one fragment
+        const int nch = std::abs(mchrg[t]), sgn = mchrg[t] < 0 ? -1 : 1;
+        for (int f = 1; f <= nfrag[t]; ++f) {
+            std::vector<int> z;
+            std::vector<double> x;
+            for (int i = offsets[t]; i < offsets[t + 1]; ++i)
+                if (list[i] == f) {
+                    z.push_back(num[i]);
+                    x.insert(x.end(), axyz + 3 * (size_t)i, axyz + 3 * (size_t)i + 3);
+                }
+            for (int j = 0; j <= nch; ++j) {
+                Job jb{t, f, j, sgn * j, -1, 0.0, 1};
+                const int nel = valence_electrons(z.data(), (int)z.size(), jb.charge);
+                if (nel == 0) {
+                    if (!eself(z.data(), (int)z.size(), jb.e)) jb.ok = 0;
+                } else if (nel < 0) {
+                    jb.ok = 0;
+                } else {
+                    jb.sys = (int)s_chg.size();
+                    s_num.insert(s_num.end(), z.begin(), z.end());
+                    s_xyz.insert(s_xyz.end(), x.begin(), x.end());
+                    s_off.push_back(s_off.back() + (int32_t)z.size());
+                    s_chg.push_back(jb.charge);
+                    s_mult.push_back(1);
+                    s_et.push_back(kIpEtemp);
+                }
+                jobs.push_back(jb);
+            }
+        }
+    }
+    const int nsys = (int)s_chg.size();
+    if (nsys > 0) {
+        std::vector<double> e(nsys);
+        std::vector<int32_t> st(nsys);
+        const int rc = qcxms_egrad_batch(nsys, s_off.data(), s_num.data(), s_xyz.data(), s_chg.data(),
+                                         s_mult.data(), QCXMS_GFN2_XTB, s_et.data(), nullptr, e.data(), nullptr,
+                                         st.data());
+        if (rc != QCXMS_OK && rc != QCXMS_FATAL) return rc;
+        for (Job& jb : jobs)
+            if (jb.sys >= 0) {
+                jb.e = e[jb.sys];
+                jb.ok = (st[jb.sys] == 0);
+            }
+    }
+    // eqm (iniqm.f90:400-421) and analyse.f90:344-400 per trajectory
+    size_t k = 0;
+    while (k < jobs.size()) {
+        const int t = jobs[k].t, nch = std::abs(mchrg[t]);
+        double* ip = fragip + 10 * kMaxCharge * (size_t)t;
+        bool good = true;
+        for (; k < jobs.size() && jobs[k].t == t; k += nch + 1) {
+            const Job& jn = jobs[k];
+            if (!jn.ok) stat[t] = -1;
+            if (std::fabs(jn.e) < 1.0e-10 || !jn.ok) good = false;
+            for (int j = 1; j <= nch; ++j) {
+                const Job& ji = jobs[k + j];
+                if (!ji.ok) stat[t] = -1;
+                if (std::fabs(ji.e) < 1.0e-10 || !ji.ok) good = false;
+                if (ji.e != 0.0 && jn.e != 0.0) {
+                    const double v = (ji.e - jn.e) * kAutoEV;
+                    ip[(j - 1) * 10 + (jn.frag - 1)] = v;
+                    if (mchrg[t] < 0) {
+                        if (v > 40.0 || v < -35.0) good = false;
+                    } else if (mchrg[t] == 1) {
+                        if (v < 0.0 || v > 50.0) good = false;
+                    } else if (v < 0.0 || v > 100.0) {
+                        good = false;
+                    }
+                }
+            }
+        }
+        if (!good) {
+            // the second attempt repeats GFN2 and the third needs IPEA1
+            // (not built): total failure, analyse.f90:406-414
+            ipok[t] = 0;
+            for (int j = 1; j <= mchrg[t]; ++j)
+                for (int f = 0; f < nfrag[t]; ++f) ip[(j - 1) * 10 + f] = 0.0;
+        }
+    }
+    return QCXMS_OK;
+}
+
+int qcxms_manage_fragments_batch(int32_t ntraj, const int32_t* offsets, const int32_t* num, const double* xyz,
+                                 const double* axyz, const double* qat, const double* mass, const int32_t* imass,
+                                 const double* atlast, const int32_t* itrj, int32_t isec, int32_t maxsec, double btf,
+                                 int32_t* list, qcxms_fragment_report* report)
+{
+    using namespace qx_post;
+    g_err.clear();
+    if (ntraj < 0 || !offsets || !num || !xyz || !axyz || !qat || !mass || !atlast || !itrj || !list || !report)
+        return fail("null argument");
+    for (int t = 0; t < ntraj; ++t) {
+        if (report[t].mchrg == 0 || std::abs(report[t].mchrg) > kMaxCharge) return fail("|mchrg| outside 1..3");
+        for (int i = offsets[t]; i < offsets[t + 1]; ++i)
+            if (imass && imass[i] > 99) return fail("isotope mass number above 99");
+    }
+    const size_t na = ntraj > 0 ? (size_t)offsets[ntraj] : 0;
+    std::vector<int32_t> nf(ntraj);
+    std::vector<double> fm(10 * (size_t)ntraj);
+    int st;
+    if (ntraj > 0 && (st = qcxms_fragments_batch(ntraj, offsets, num, xyz, mass, list, nf.data(), fm.data())))
+        return st;
+    // trajectories whose fragments get IPs: counted runs with 2..5 fragments
+    std::vector<int> sel;
+    std::vector<int32_t> o{0}, n, l, c, fnf;
+    std::vector<double> x;
+    for (int t = 0; t < ntraj; ++t) {
+        qcxms_fragment_report& r = report[t];
+        r.nfrag = nf[t];
+        r.tcont = 0;
+        r.ipok = 1;
+        r.nres = 0;
+        r.stat = 0;
+        r.asave[0] = '\0';
+        for (int f = 0; f < 10; ++f) {
+            r.fragchrg3[f] = r.fragchrg[f] = 0.0;
+            r.fragm[f] = fm[10 * (size_t)t + f];
+        }
+        for (int f = 0; f < 10 * kMaxCharge; ++f) r.fragip[f] = 0.0;
+        if (nf[t] > 5) r.nfrag_ok = 0;  // write_fragments.f90:73-80
+        if (!r.nfrag_ok || nf[t] <= 1) continue;
+        sel.push_back(t);
+        for (int i = offsets[t]; i < offsets[t + 1]; ++i) {
+            n.push_back(num[i]);
+            l.push_back(list[i]);
+            x.insert(x.end(), axyz + 3 * (size_t)i, axyz + 3 * (size_t)i + 3);
+        }
+        o.push_back((int32_t)n.size());
+        c.push_back(r.mchrg);
+        fnf.push_back(nf[t]);
+    }
+    const int ns = (int)sel.size();
+    std::vector<double> ips(10 * kMaxCharge * (size_t)ns);
+    std::vector<int32_t> ok(ns), sst(ns);
+    if (ns > 0 && (st = qcxms_fragment_ips_batch(ns, o.data(), n.data(), x.data(), l.data(), fnf.data(), c.data(),
+                                                 ips.data(), ok.data(), sst.data())))
+        return st;
+    std::vector<int> slot(ntraj, -1);
+    for (int k = 0; k < ns; ++k) slot[sel[k]] = k;
+    for (int t = 0; t < ntraj; ++t) {
+        qcxms_fragment_report& r = report[t];
+        if (!r.nfrag_ok) continue;
+        const int nfr = nf[t];
+        // per-fragment atom counts, first element and element histogram
+        // (fragmass, fragments.f90:27-35; isotopes at 100 + imass)
+        int natf[kMaxFrag] = {0}, zfirst[kMaxFrag] = {0};
+        std::vector<int> fragel(kMaxElem * kMaxFrag, 0);
+        for (int i = offsets[t]; i < offsets[t + 1]; ++i) {
+            const int f = list[i] - 1;
+            if (natf[f]++ == 0) zfirst[f] = num[i];
+            const int e = (imass && imass[i] > 0) ? 100 + imass[i] : num[i];
+            fragel[f * kMaxElem + (e - 1)] += 1;
+            r.fragchrg[f] = r.fragchrg[f] + qat[i];  // tm/orca/xtb branch, write_fragments.f90:320-343
+        }
+        if (slot[t] >= 0) {
+            const double* ip = ips.data() + 10 * kMaxCharge * (size_t)slot[t];
+            for (int f = 0; f < 10 * kMaxCharge; ++f) r.fragip[f] = ip[f];
+            r.ipok = ok[slot[t]];
+            r.stat = sst[slot[t]];
+        }
+        ChargeState cs;
+        cs.nfrag = nfr;
+        cs.mchrg = r.mchrg;
+        cs.chrgcont = r.chrgcont;
+        // fragip(nfrag, |mchrg|) column-major from the 10 x 3 slots
+        double fip[kMaxFrag * kMaxCharge] = {0};
+        for (int j = 0; j < std::abs(r.mchrg); ++j)
+            for (int f = 0; f < nfr; ++f) fip[f + j * nfr] = r.fragip[j * 10 + f];
+        assign_charges(cs, fip, natf, zfirst, atlast[t] * btf, isec, maxsec);
+        r.mchrg = cs.mchrg;
+        r.chrgcont = cs.chrgcont;
+        r.tcont = cs.tcont;
+        for (int f = 0; f < 10; ++f) r.fragchrg3[f] = cs.fragchrg3[f];
+        // records (write_fragments.f90:368-450, EI)
+        for (int j = 1; j <= nfr; ++j) {
+            int zel[kMaxElem], cnt[kMaxElem], nl = 0;
+            for (int e = 0; e < kMaxElem; ++e)
+                if (fragel[(j - 1) * kMaxElem + e] != 0) {
+                    zel[nl] = e + 1;
+                    cnt[nl] = fragel[(j - 1) * kMaxElem + e];
+                    ++nl;
+                }
+            const std::string rec = res_record(r.fragchrg3[j - 1], r.mchrg, itrj[t], -1, isec, j, nl, zel, cnt);
+            char* dst = (r.tcont > 0 && r.tcont == j) ? r.asave : r.res[r.nres++];
+            std::snprintf(dst, sizeof r.asave, "%s", rec.c_str());
+        }
+    }
     return QCXMS_OK;
 }
 

@@ -46,6 +46,7 @@ struct MdPtrs {
     const int* num;  // atomic numbers
     int* list;       // fragment labels per atom
     double* avchrg;  // running charge sums (md.f90:355-361)
+    double* avxyz;   // running coordinate sums, 3 per atom (md.f90:357,363)
     int* active;     // per trajectory: 1 while running
 };
 
@@ -305,7 +306,12 @@ __global__ __launch_bounds__(64) void k_md_pre(BatchPtrs B, MdPtrs M, MdParams p
     for (int i = lane; i < nat; i += 64) {
         const double prev = reset ? 0.0 : M.avchrg[off + i];
         M.avchrg[off + i] = prev + B.qat[off + i];
+        for (int c = 0; c < 3; ++c) {
+            const size_t k = 3 * (size_t)(off + i) + c;
+            M.avxyz[k] = (reset ? 0.0 : M.avxyz[k]) + B.xyz[k];
+        }
     }
+    __syncthreads();
     // leap-frog (md.f90:749-773)
     const double ke = dev_leapfrog(nat, M.mass + off, p.tstep, B.grad + 3 * (size_t)off,
                                    B.xyz + 3 * (size_t)off, M.velo + 3 * (size_t)off, term);

@@ -225,6 +225,7 @@ def _md_sigs(L):
     L.qcxms_md_destroy.argtypes = [C.c_void_p]
     L.qcxms_md_run.argtypes = [C.c_void_p, C.c_int32, _ip]
     L.qcxms_md_read.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _ip, C.POINTER(MdResult)]
+    L.qcxms_md_read_averages.argtypes = [C.c_void_p, _dp, _dp]
     L.qcxms_md_batch.restype = C.c_void_p
     L.qcxms_md_batch.argtypes = [C.c_void_p]
     L.qcxms_fragments_batch.argtypes = [C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
@@ -296,6 +297,14 @@ class MdEnsemble:
             raise RuntimeError("qcxms_md_read failed: " + last_error())
         return dict(xyz=xyz, velo=velo, grad=grad, qat=qat, list=lst, results=list(res))
 
+    def read_averages(self):
+        """(axyz [natoms,3], achrg [natoms]): the average geometry and
+        Mulliken charges over the last avdump steps (md.f90:691-698)."""
+        axyz = np.zeros((self.natoms, 3)); achrg = np.zeros(self.natoms)
+        if library().qcxms_md_read_averages(self._h, _p(axyz), _p(achrg)):
+            raise RuntimeError("qcxms_md_read_averages failed: " + last_error())
+        return axyz, achrg
+
     def batch_handle(self):
         return library().qcxms_md_batch(self._h)
 
@@ -363,3 +372,75 @@ def impactscale_batch(velos, masses, velofs, eimp, ff, e0):
                                  _p(e0a), _p(it, _ip)):
         raise RuntimeError(last_error())
     return v, it
+
+
+class FragmentReport(C.Structure):
+    """qcxms_fragment_report: one trajectory's manage_fragments state."""
+    _fields_ = ([("mchrg", C.c_int32), ("nfrag_ok", C.c_int32), ("chrgcont", C.c_double)]
+                + [(n, C.c_int32) for n in ("nfrag", "tcont", "ipok", "nres", "stat", "pad")]
+                + [("fragchrg3", C.c_double * 10), ("fragchrg", C.c_double * 10),
+                   ("fragm", C.c_double * 10), ("fragip", C.c_double * 30),
+                   ("asave", C.c_char * 128), ("res", (C.c_char * 128) * 10)])
+
+    def records(self):
+        return [self.res[k].value.decode() for k in range(self.nres)]
+
+
+def _post_sigs(L):
+    if getattr(L, "_post_sigs", False):
+        return
+    L.qcxms_fragment_ips_batch.argtypes = [C.c_int32, _ip, _ip, _dp, _ip, _ip, _ip, _dp, _ip, _ip]
+    L.qcxms_manage_fragments_batch.argtypes = [C.c_int32, _ip, _ip, _dp, _dp, _dp, _dp, _ip, _dp,
+                                               _ip, C.c_int32, C.c_int32, C.c_double, _ip,
+                                               C.POINTER(FragmentReport)]
+    L._post_sigs = True
+
+
+def fragment_ips_batch(nums, axyzs, lists, mchrgs):
+    """analyse (src/analyse.f90:14-448) for many trajectories: returns
+    (fragip [ntraj, 10, 3] eV with [t, i-1, j-1] = IP of fragment i for
+    charge j, ipok [ntraj], stat [ntraj])."""
+    L = library()
+    _post_sigs(L)
+    n = len(nums)
+    off = np.cumsum([0] + [len(x) for x in nums]).astype(np.int32)
+    num = np.ascontiguousarray(np.concatenate(nums), dtype=np.int32)
+    x = np.ascontiguousarray(np.concatenate(axyzs), dtype=np.float64)
+    lst = np.ascontiguousarray(np.concatenate(lists), dtype=np.int32)
+    nf = np.array([int(np.max(l)) for l in lists], dtype=np.int32)
+    mc = np.ascontiguousarray(mchrgs, dtype=np.int32)
+    fip = np.zeros((n, 3, 10))
+    ok = np.zeros(n, dtype=np.int32)
+    st = np.zeros(n, dtype=np.int32)
+    if L.qcxms_fragment_ips_batch(n, _p(off, _ip), _p(num, _ip), _p(x), _p(lst, _ip), _p(nf, _ip),
+                                  _p(mc, _ip), _p(fip), _p(ok, _ip), _p(st, _ip)):
+        raise RuntimeError("qcxms_fragment_ips_batch failed: " + last_error())
+    return fip.transpose(0, 2, 1).copy(), ok, st
+
+
+def manage_fragments_batch(nums, xyzs, axyzs, qats, masses, atlast, itrj, isec, maxsec, btf,
+                           mchrg, chrgcont, nfrag_ok=None, imasses=None):
+    """manage_fragments (src/write_fragments.f90:12-457, EI) for many finished
+    trajectories.  Returns (lists, [FragmentReport])."""
+    L = library()
+    _post_sigs(L)
+    n = len(nums)
+    off = np.cumsum([0] + [len(x) for x in nums]).astype(np.int32)
+    cat = lambda a, dt: np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=dt).reshape(-1)
+                                                             for v in a]))
+    num = cat(nums, np.int32)
+    im = cat(imasses, np.int32) if imasses is not None else None
+    rep = (FragmentReport * n)()
+    for t in range(n):
+        rep[t].mchrg = int(mchrg[t])
+        rep[t].chrgcont = float(chrgcont[t])
+        rep[t].nfrag_ok = 1 if nfrag_ok is None else int(nfrag_ok[t])
+    lst = np.zeros(len(num), dtype=np.int32)
+    if L.qcxms_manage_fragments_batch(n, _p(off, _ip), _p(num, _ip), _p(cat(xyzs, np.float64)),
+                                      _p(cat(axyzs, np.float64)), _p(cat(qats, np.float64)),
+                                      _p(cat(masses, np.float64)), _p(im, _ip),
+                                      _p(np.ascontiguousarray(atlast, dtype=np.float64)),
+                                      _p(np.ascontiguousarray(itrj, dtype=np.int32), _ip),
+                                      int(isec), int(maxsec), float(btf), _p(lst, _ip), rep):
+        raise RuntimeError("qcxms_manage_fragments_batch failed: " + last_error())
+    return [lst[off[t]:off[t + 1]] for t in range(n)], list(rep)

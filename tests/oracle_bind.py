@@ -65,6 +65,8 @@ def oracle():
                                            ip, C.c_int]
         L.orc_md_run.argtypes = [C.c_void_p, ip, ip, dp, dp, dp, dp, dp, dp, ip, EGRAD_FN,
                                  C.c_void_p, C.c_int]
+        L.orc_md_run_avg.argtypes = [C.c_void_p, ip, ip, dp, dp, dp, dp, dp, dp, ip, EGRAD_FN,
+                                     C.c_void_p, C.c_int, dp, dp]
         _oracle = L
     return _oracle
 
@@ -193,10 +195,13 @@ def md_run(traj, steps_limit, tstep=0.5 * 41.3413733365614, max_steps=2000, nfra
     st.ax, st.ieetemp, st.etempin = 0.0, 0.0, etempin
     o = opts if opts is not None else default_opts()
     fn = C.cast(L.orc_gfn2_md_egrad, EGRAD_FN)
-    L.orc_md_run(C.byref(st), ptr(num, ip), ptr(imass, ip), ptr(mass), ptr(velof), ptr(xyz),
-                 ptr(velo), ptr(grad), ptr(qat), ptr(lst, ip), fn, C.cast(C.pointer(o), C.c_void_p),
-                 int(steps_limit))
-    return dict(state=st, xyz=xyz, velo=velo, grad=grad, qat=qat, list=lst)
+    axyz = np.zeros((nat, 3))
+    achrg = np.zeros(nat)
+    L.orc_md_run_avg(C.byref(st), ptr(num, ip), ptr(imass, ip), ptr(mass), ptr(velof), ptr(xyz),
+                     ptr(velo), ptr(grad), ptr(qat), ptr(lst, ip), fn,
+                     C.cast(C.pointer(o), C.c_void_p), int(steps_limit), ptr(axyz), ptr(achrg))
+    return dict(state=st, xyz=xyz, velo=velo, grad=grad, qat=qat, list=lst, axyz=axyz,
+                achrg=achrg)
 
 
 def gfn2_egrad_batch(systems, xyz_list, etemp, opts=None, nthreads=None):
@@ -219,3 +224,94 @@ def gfn2_egrad_batch(systems, xyz_list, etemp, opts=None, nthreads=None):
                            C.byref(o), ptr(e), ptr(g), ptr(q), ptr(st, ip), nt)
     return (e, [g[off[i]:off[i + 1]] for i in range(nsys)],
             [q[off[i]:off[i + 1]] for i in range(nsys)], st)
+
+
+# ---- post-MD fragment analysis (oracle/postmd_oracle.c) ----
+def _post_sigs(L):
+    if getattr(L, "_post_sigs", False):
+        return
+    L.orc_eself.argtypes = [C.c_int, ip, dp]
+    L.orc_nel.argtypes = [C.c_int, ip, C.c_int]
+    L.orc_boltz.argtypes = [C.c_int, C.c_int, C.c_double, dp, dp]
+    L.orc_res_record.argtypes = [C.c_double] + [C.c_int] * 6 + [ip, ip, C.c_char_p]
+    L.orc_analyse_ips.argtypes = [C.c_int, ip, dp, ip, C.c_int, C.c_int, C.c_void_p, dp,
+                                  C.POINTER(C.c_int)]
+    L.orc_manage_fragments.argtypes = ([C.c_int, ip, dp, dp, dp, dp, ip, C.c_double, C.c_int,
+                                        C.c_int, C.c_int, C.c_double, C.c_void_p]
+                                       + [C.POINTER(C.c_int), dp, C.POINTER(C.c_int), ip,
+                                          C.POINTER(C.c_int), C.POINTER(C.c_int), dp, dp, dp,
+                                          C.POINTER(C.c_int), C.c_char_p, C.c_char_p,
+                                          C.POINTER(C.c_int)])
+    L._post_sigs = True
+
+
+def res_record(q, mchrg, itrj, icoll, isec, j, pairs):
+    L = oracle()
+    _post_sigs(L)
+    z = np.array([p[0] for p in pairs] or [0], dtype=np.int32)
+    n = np.array([p[1] for p in pairs] or [0], dtype=np.int32)
+    buf = C.create_string_buffer(200)
+    L.orc_res_record(q, mchrg, itrj, icoll, isec, j, len(pairs), ptr(z, ip), ptr(n, ip), buf)
+    return buf.value.decode()
+
+
+def boltz(ip_mat, temp):
+    """ip_mat [nfrag, nchrg] (eV) -> populations, same shape."""
+    L = oracle()
+    _post_sigs(L)
+    a = np.asfortranarray(ip_mat, dtype=np.float64)
+    out = np.zeros_like(a, order="F")
+    L.orc_boltz(a.shape[0], a.shape[1], temp, a.ctypes.data_as(dp), out.ctypes.data_as(dp))
+    return out
+
+
+def eself(num):
+    L = oracle()
+    _post_sigs(L)
+    z = np.ascontiguousarray(num, dtype=np.int32)
+    e = C.c_double()
+    st = L.orc_eself(len(z), ptr(z, ip), C.byref(e))
+    return e.value, st
+
+
+def analyse_ips(num, axyz, lst, nfrag, mchrg, opts=None):
+    """fragip [nfrag, |mchrg|] (eV), ipok, rc."""
+    L = oracle()
+    _post_sigs(L)
+    num = np.ascontiguousarray(num, dtype=np.int32)
+    x = np.ascontiguousarray(axyz, dtype=np.float64)
+    l = np.ascontiguousarray(lst, dtype=np.int32)
+    fip = np.zeros(max(nfrag, 1) * abs(mchrg))
+    ok = C.c_int()
+    o = opts if opts is not None else default_opts()
+    rc = L.orc_analyse_ips(len(num), ptr(num, ip), ptr(x), ptr(l, ip), nfrag, mchrg,
+                           C.cast(C.pointer(o), C.c_void_p), ptr(fip), C.byref(ok))
+    return fip.reshape(abs(mchrg), max(nfrag, 1)).T.copy(), ok.value, rc
+
+
+def manage_fragments(num, xyz, axyz, qat, mass, atlast, itrj, isec, maxsec, btf, mchrg,
+                     chrgcont, nfrag_ok=1, imass=None, opts=None):
+    L = oracle()
+    _post_sigs(L)
+    num = np.ascontiguousarray(num, dtype=np.int32)
+    nat = len(num)
+    im = np.ascontiguousarray(imass if imass is not None else np.full(nat, -1), dtype=np.int32)
+    lst = np.zeros(nat, dtype=np.int32)
+    mc, cc, ok = C.c_int(mchrg), C.c_double(chrgcont), C.c_int(nfrag_ok)
+    nf, tc, ipok, nres = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    q3, fq, fip = np.zeros(10), np.zeros(10), np.zeros(30)
+    asave = C.create_string_buffer(160)
+    res = C.create_string_buffer(160 * 10)
+    o = opts if opts is not None else default_opts()
+    rc = L.orc_manage_fragments(nat, ptr(num, ip), ptr(np.ascontiguousarray(xyz, dtype=np.float64)),
+                                ptr(np.ascontiguousarray(axyz, dtype=np.float64)),
+                                ptr(np.ascontiguousarray(qat, dtype=np.float64)),
+                                ptr(np.ascontiguousarray(mass, dtype=np.float64)), ptr(im, ip),
+                                atlast, itrj, isec, maxsec, btf, C.cast(C.pointer(o), C.c_void_p),
+                                C.byref(mc), C.byref(cc), C.byref(ok), ptr(lst, ip), C.byref(nf),
+                                C.byref(tc), ptr(q3), ptr(fq), ptr(fip), C.byref(ipok), asave, res,
+                                C.byref(nres))
+    recs = [res.raw[160 * k:160 * (k + 1)].split(b"\0", 1)[0].decode() for k in range(nres.value)]
+    return dict(rc=rc, mchrg=mc.value, chrgcont=cc.value, nfrag_ok=ok.value, list=lst,
+                nfrag=nf.value, tcont=tc.value, fragchrg3=q3, fragchrg=fq, fragip=fip,
+                ipok=ipok.value, asave=asave.value.decode(), res=recs)

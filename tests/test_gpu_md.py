@@ -72,10 +72,11 @@ def ensemble(n, seed0=1, tadd_fs=5.0):
 
 def test_md_ensemble_matches_oracle_md():
     trajs = ensemble(3)
-    nsteps = 24
+    nsteps = 56  # crosses the avdump = 50 reset of the running averages
     md = q().MdEnsemble(trajs, max_steps=nsteps, use_graph=False, econv=1e-11, pconv=1e-9)
     running = md.run(nsteps + 5)
     out = md.read()
+    axyz, achrg = md.read_averages()
     assert running == 0
     opts = ob.default_opts(econv=1e-12, pconv=1e-10)
     for t, tr in enumerate(trajs):
@@ -94,6 +95,10 @@ def test_md_ensemble_matches_oracle_md():
         assert abs(res.tav - st.tav) < 1e-4
         assert np.abs(out["xyz"][sl] - ref["xyz"]).max() < 1e-8
         assert np.abs(out["velo"][sl] - ref["velo"]).max() < 1e-10
+        # averages handed to the fragment analysis (md.f90:691-698)
+        assert np.abs(axyz[sl] - ref["axyz"]).max() < 1e-8
+        assert np.abs(achrg[sl] - ref["achrg"]).max() < 1e-6
+        assert abs(res.atlast - st.atlast) < 1e-4
 
 
 def test_md_graph_replay_equals_eager():

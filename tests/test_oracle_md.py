@@ -147,3 +147,46 @@ def test_live_against_reference_fortran():
         xyz = rng.normal(0, float(rng.choice([2.0, 5.0, 10.0])), size=(nat, 3))
         assert np.array_equal(ob.fragment_structure(L, "orc_", oz, xyz),
                               ob.fragment_structure(R, "ref_", oz, xyz))
+
+
+def test_md_averages_window_closed_form():
+    """md.f90:352-363, 691-698: the running sums restart when kdump exceeds
+    avdump - 1 = 49 (at the first step and every 50 steps), collect the
+    geometry and charges before each leap-frog step, and are divided by
+    kdump at the end.  Under a constant force the geometries are the exact
+    quadratic x_n = x0 + n dt v0 + dt^2 a n(n+1)/2, so after 56 steps the
+    average is the mean of x_50 .. x_55 (six terms)."""
+    import ctypes as C
+    L = ob.oracle()
+    num = np.array([6, 1], dtype=np.int32)
+    mass = np.array([12.0, 1.008]) * 1822.888486
+    x0 = np.array([[0.0, 0.0, 0.0], [2.05, 0.0, 0.0]])
+    v0 = np.array([[1e-5, -2e-5, 0.5e-5], [1e-5, -2e-5, 0.5e-5]])
+    g = np.array([[1e-4, 0.0, -1e-4], [1e-4 * 1.008 / 12.0, 0.0, -1e-4 * 1.008 / 12.0]])
+    q = np.array([0.3, 0.7])
+
+    @ob.EGRAD_FN
+    def egrad(ctx, nat, iat, xyz, chrg, etemp, e, grad, qat):
+        e[0] = -5.0
+        for k in range(3 * nat):
+            grad[k] = g.reshape(-1)[k]
+        for k in range(nat):
+            qat[k] = q[k]
+        return 0
+
+    st = ob.MdState()
+    st.nat, st.chrg, st.nfragexit, st.max_steps = 2, 1, 3, 56
+    dt = 0.5 * 41.3413733365614
+    st.tstep, st.tadd, st.eimp, st.ax, st.ieetemp, st.etempin = dt, 0.0, 0.0, 0.0, 0.0, 300.0
+    x, v = x0.copy(), v0.copy()
+    grad, qat = np.zeros((2, 3)), np.zeros(2)
+    lst = np.zeros(2, dtype=np.int32)
+    axyz, achrg = np.zeros((2, 3)), np.zeros(2)
+    n = L.orc_md_run_avg(C.byref(st), ob.ptr(num, ob.ip), ob.ptr(np.full(2, -1, np.int32), ob.ip),
+                         ob.ptr(mass), ob.ptr(np.ones(2)), ob.ptr(x), ob.ptr(v), ob.ptr(grad),
+                         ob.ptr(qat), ob.ptr(lst, ob.ip), egrad, None, 0, ob.ptr(axyz), ob.ptr(achrg))
+    assert n == 56 and st.exit_reason == 7
+    a = -g / mass[:, None]
+    xs = [x0 + k * dt * v0 + dt * dt * a * k * (k + 1) / 2 for k in range(50, 56)]
+    np.testing.assert_allclose(axyz, np.mean(xs, axis=0), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(achrg, q, rtol=0, atol=1e-15)
