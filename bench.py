@@ -108,6 +108,88 @@ def run_gpu(ntraj, steps, warmup, use_graph_check=True):
     return out
 
 
+def run_singlepoints(name, reps):
+    """Batched single points of the uneven (1024 random C/H/N/O chains, 3-62
+    atoms) or stretched (64 x 40-atom C12H24O2N2+) configuration: cold-started
+    SCC on every call, systems ordered largest basis first."""
+    import qcxms_amd
+    from qcxms_amd import workloads as W
+    if name == "uneven1024":
+        sysl = []
+        for s in range(1024):
+            num, xyz, charge = W.random_chain(5000 + s)
+            sysl.append((num, charge, xyz))
+        desc = "1024 single points: random saturated C/H/N/O chains, 1-20 heavy atoms (3-62 atoms)"
+    else:
+        sysl = []
+        for s in range(64):
+            num, xyz = W.chain_c12h24o2n2(seed=s)
+            sysl.append((num, 1, xyz))
+        desc = "64 single points: C12H24O2N2+ (40 atoms, 88 basis functions), bonds stretched <= +30 %"
+    nbf = lambda num: int(sum(1 if z == 1 else 4 for z in num))
+    sysl.sort(key=lambda t: -nbf(t[0]))
+    b = qcxms_amd.Batch([(t[0], t[1]) for t in sysl])
+    b.set_warm_start(0)
+    xyz = np.concatenate([t[2] for t in sysl])
+    b.egrad(xyz, 5000.0)
+    b.set_profiling(True)
+    for k in ("setup", "scc", "grad"):
+        b.kernel_time(k)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e, g, q, st = b.egrad(xyz, 5000.0)
+    wall = time.perf_counter() - t0
+    kt = {k: b.kernel_time(k) for k in ("setup", "scc", "grad")}
+    iters = np.asarray(b.info()[0], dtype=np.float64)
+    b.close()
+    assert np.all(st == 0)
+    return {"metric": "GFN2-xTB energy+gradient single points/sec on 1 GPU (batched)",
+            "value": len(sysl) * reps / wall, "unit": "single-points/s", "n_gpus": 1,
+            "reps": reps, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": desc, "nsys": len(sysl),
+                       "scc_iterations_mean": float(iters.mean())},
+            "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()}}
+
+
+def run_c20_md(ntraj, steps, warmup):
+    """Batched fragmentation MD of C20H36O2N2+ (60 atoms, 132 basis
+    functions, matrices in global memory), IEE Poisson mean 0.8 eV/atom."""
+    import qcxms_amd
+    from qcxms_amd import workloads as W
+    trajs = W.md_ensemble(ntraj, base_seed=1, mean_ev=0.8, max_ev=2.0, temp=500.0,
+                          builder=W.chain_c20h36o2n2, tadd_fs=100.0)
+    md = qcxms_amd.MdEnsemble(trajs, max_steps=10000, use_graph=True)
+    md.run(warmup)
+    md.set_profiling(True)
+    for k in ("setup", "scc", "grad", "md_pre", "md_post"):
+        md.kernel_time(k)
+    it0 = md.scc_iterations_total().astype(np.float64).sum()
+    n0 = sum(r.nstep for r in md.read()["results"])
+    t0 = time.perf_counter()
+    md.run(steps)
+    wall = time.perf_counter() - t0
+    res = md.read()["results"]
+    n1 = sum(r.nstep for r in res)
+    it1 = md.scc_iterations_total().astype(np.float64).sum()
+    kt = {k: md.kernel_time(k) for k in ("setup", "scc", "grad", "md_pre", "md_post")}
+    md.close()
+    scc_ms, nl = kt["scc"]
+    flops = (it1 - it0) * scc_flops(132)
+    ach = flops / (scc_ms * 1e-3) / 1e12 if scc_ms > 0 else 0.0
+    return {"metric": "trajectory-MD-steps/sec on 1 GPU (batched)", "value": (n1 - n0) / wall,
+            "unit": "trajectory-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"batched fragmentation MD: {ntraj} x C20H36O2N2+ (60 atoms, "
+                                   "132 basis functions), IEE Poisson 0.8 eV/atom (<= 2.0)",
+                       "ntraj": ntraj,
+                       "scc_iterations_per_trajectory_step": (it1 - it0) / max(n1 - n0, 1),
+                       "nfrag_histogram": np.bincount([r.nfrag for r in res]).tolist()},
+            "roofline": {"bound": "mfma", "kernel": "k_scc", "achieved": ach,
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": ach / FP64_PEAK_TFLOPS, "avg_launch_ms": scc_ms / max(nl, 1)},
+            "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()}}
+
+
 def run_one_per_launch(steps, warmup):
     import qcxms_amd
     trajs = workload(1)
@@ -190,9 +272,22 @@ def main():
                     help="skip the hipGraph replay of the same steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration)")
+    ap.add_argument("--config", default="md256",
+                    choices=["md256", "uneven1024", "stretched64", "c20md"],
+                    help="md256: the headline batched MD (default); the others print the "
+                         "throughput of the other workload configurations")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.config != "md256":
+        if args.gpus != 1:
+            sys.exit("bench.py measures one GPU only")
+        if args.config == "c20md":
+            out = run_c20_md(args.ntraj if args.ntraj != 256 else 4096, args.steps, args.warmup)
+        else:
+            out = run_singlepoints(args.config, max(1, args.steps))
+        print(json.dumps(out), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs every trajectory on a single "
