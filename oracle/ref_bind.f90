@@ -16,12 +16,20 @@
 !   gam, gam3            src/dftd4.f90:38-72 (the tables eself uses)
 !   ref_res_record       a record written with the format of
 !                        src/write_fragments.f90:402-441 (the qcxms.res line)
+!   ref_write_start      qcxms.start with the write statements of
+!                        src/utility.f90:358-369 (wrstart)
+!   ref_write_gs         qcxms.gs with the write statements of
+!                        src/md.f90:151-152, 383-386
+!   getmaxiee, getieeab, gauss0, poiss0
+!                        src/iee.f90:12-135
+!   calctrelax           src/impact.f90:58-81
 module qcxms_ref_bind
    use iso_c_binding
    use xtb_mctc_accuracy, only: wp
    use qcxms_fragments, only: fragment_structure, fragmass
    use qcxms_mdinit, only: ekinet
-   use qcxms_impact, only: impactscale
+   use qcxms_impact, only: impactscale, calctrelax
+   use qcxms_iee, only: getmaxiee, getieeab, gauss0, poiss0
    use covalent_radii, only: Rad, covalent_radius_d3
    use xtb_atomic_masses, only: atomic_mass
    use dftd4
@@ -223,5 +231,89 @@ subroutine ref_res_record(q, mchrg, itrj, icoll, isec, j, l, idum2, idum1, out, 
       out(m) = line(m:m)
    enddo
 end subroutine
+
+subroutine ref_write_start(path, plen, traj, nat, velo, velof, eimp, tadd) bind(C)
+   integer(c_int), value :: plen, traj, nat
+   character(kind=c_char), intent(in) :: path(plen)
+   real(c_double), intent(in) :: velo(3, nat), velof(nat)
+   real(c_double), value :: eimp, tadd
+   character(len=plen) :: fname
+   integer :: u, j
+   do j = 1, plen
+      fname(j:j) = path(j)
+   enddo
+   open(file=fname, newunit=u)
+   write(u,'(i4)') traj
+   write(u,'(D22.14)') eimp
+   write(u,'(D22.14)') tadd
+   do j = 1, nat
+      write(u,'(7D22.14)') velo(1:3, j), velof(j)
+   enddo
+   close(u)
+end subroutine
+
+subroutine ref_write_gs(path, plen, nmax, ndump, nat, xyz, velo) bind(C)
+   integer(c_int), value :: plen, nmax, ndump, nat
+   character(kind=c_char), intent(in) :: path(plen)
+   real(c_double), intent(in) :: xyz(3, nat, ndump), velo(3, nat, ndump)
+   character(len=plen) :: fname
+   integer :: u, i, j, k
+   do j = 1, plen
+      fname(j:j) = path(j)
+   enddo
+   open(file=fname, newunit=u)
+   write(u,*) nmax
+   do k = 1, ndump
+      do i = 1, nat
+         write(u,'(6d16.8)') (xyz(j, i, k), j = 1, 3), (velo(j, i, k), j = 1, 3)
+      enddo
+   enddo
+   close(u)
+end subroutine
+
+subroutine ref_getmaxiee(a, b, ieeel, ityp, exc, ieemax, pmax, e_avg) bind(C)
+   real(c_double), value :: a, b, ieeel, exc
+   integer(c_int), value :: ityp
+   real(c_double), intent(out) :: ieemax, pmax, e_avg
+   real(wp) :: a_, b_, el_, ex_
+   a_ = a; b_ = b; el_ = ieeel; ex_ = exc
+   call getmaxiee(a_, b_, el_, int(ityp), ex_, ieemax, pmax, e_avg)
+end subroutine
+
+subroutine ref_getieeab(ieeel, ityp, exc, nbnd, ieeatm, a, b) bind(C)
+   real(c_double), value :: ieeel, exc, ieeatm
+   integer(c_int), value :: ityp, nbnd
+   real(c_double), intent(out) :: a, b
+   real(wp) :: el_, ex_, at_
+   el_ = ieeel; ex_ = exc; at_ = ieeatm
+   call getieeab(a, b, el_, int(ityp), ex_, int(nbnd), at_)
+end subroutine
+
+function ref_gauss0(a, b, ieeel, x) result(p) bind(C)
+   real(c_double), value :: a, b, ieeel, x
+   real(c_double) :: p
+   real(wp) :: a_, b_, el_, x_
+   a_ = a; b_ = b; el_ = ieeel; x_ = x
+   call gauss0(a_, b_, el_, x_, p)
+end function
+
+function ref_poiss0(a, b, ieeel, x) result(p) bind(C)
+   real(c_double), value :: a, b, ieeel, x
+   real(c_double) :: p
+   real(wp) :: a_, b_, el_, x_
+   a_ = a; b_ = b; el_ = ieeel; x_ = x
+   call poiss0(a_, b_, el_, x_, p)
+end function
+
+function ref_calctrelax(n, emo, na, i, trelax) result(t) bind(C)
+   integer(c_int), value :: n, na, i
+   real(c_double), intent(in) :: emo(10 * n)
+   real(c_double), value :: trelax
+   real(c_double) :: t
+   real(wp) :: e_(10 * n), tr_
+   e_ = emo
+   tr_ = trelax
+   call calctrelax(int(n), e_, int(na), int(i), tr_, t)
+end function
 
 end module qcxms_ref_bind
