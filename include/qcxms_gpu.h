@@ -103,8 +103,9 @@ typedef struct {
     int32_t max_steps;  /* nmax = tmax / tstep */
     int32_t use_graph;  /* != 0: replay the five kernels of a step as one hipGraph */
     int32_t extrapolate; /* SCC start of each step from the previous steps' converged
-                            moments: 1 linear, 2 quadratic, -1 none (previous step's
-                            moments), 0 library default (2) */
+                            moments: polynomial extrapolation of order 1 (linear) to 4
+                            (quartic), -1 none (previous step's moments), 0 library
+                            default (4) */
     double econv, pconv;  /* SCC thresholds (defaults 1e-9, 1e-8 when 0) */
     int32_t broyden_keep; /* > 0: keep up to this many Broyden secant pairs from step to
                              step (same fixed point); 0: fresh mixing history per step */

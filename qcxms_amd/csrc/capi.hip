@@ -381,6 +381,8 @@ int allocate(qcxms_batch* b)
     ALLOC(d.aesg, b->pair_total / QX_PAIR_STRIDE * QX_AES_COMP);
     ALLOC(d.xprev, b->nxtot);
     ALLOC(d.xprev2, b->nxtot);
+    ALLOC(d.xprev3, b->nxtot);
+    ALLOC(d.xprev4, b->nxtot);
     ALLOC(d.energy, b->nsys);
     ALLOC(d.eparts, 10 * (size_t)b->nsys);
     int* iters;
@@ -865,12 +867,13 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
         return nullptr;
     }
     b->opts.keep = params->broyden_keep;
-    if (params->extrapolate < -1 || params->extrapolate > 2) {
+    if (params->extrapolate < -1 || params->extrapolate > 4) {
         delete md;  // owns the batch
-        g_err = "extrapolate must be -1, 0, 1 or 2";
+        g_err = "extrapolate must lie in -1..4";
         return nullptr;
     }
-    b->opts.extrapolate = params->extrapolate == 0 ? 2 : (params->extrapolate < 0 ? 0 : params->extrapolate);
+    b->opts.extrapolate = params->extrapolate == 0 ? QX_DEFAULT_EXTRAPOLATE
+                                                   : (params->extrapolate < 0 ? 0 : params->extrapolate);
     if (params->econv > 0.0) b->opts.econv = params->econv;
     if (params->pconv > 0.0) b->opts.pconv = params->pconv;
     const int na = b->natoms;

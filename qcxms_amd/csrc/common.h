@@ -20,6 +20,10 @@
 #define QX_PAIR_STRIDE 16  // doubles per ordered atom pair in the gradient pair buffer
 #define QX_AES_COMP 10     // multipole-kernel factors per ordered atom pair
 
+#ifndef QX_DEFAULT_EXTRAPOLATE
+#define QX_DEFAULT_EXTRAPOLATE 4  // MD start of the SCC when the caller passes 0
+#endif
+
 #define QX_DEV __device__ __forceinline__
 
 struct ElemParams {
@@ -59,7 +63,9 @@ struct SysInfo {
 struct SccOpts {
     double econv, pconv, damp, jacobi_tol;
     int maxiter, mem;
-    int extrapolate;  // MD start: 1 linear 2 x(n-1) - x(n-2), 2 quadratic 3 (x(n-1) - x(n-2)) + x(n-3)
+    int extrapolate;  // MD start: 1 linear 2 x(n-1) - x(n-2), 2 quadratic 3 (x(n-1) - x(n-2)) + x(n-3),
+                      // 3 cubic 4 (x(n-1) + x(n-3)) - 6 x(n-2) - x(n-4),
+                      // 4 quartic 5 (x(n-1) - x(n-4)) - 10 (x(n-2) - x(n-3)) + x(n-5)
     int keep;         // > 0: Broyden secant pairs kept across warm-started calls (ring of keep)
 };
 
@@ -105,6 +111,8 @@ struct BatchPtrs {
     double* aesg;     // multipole-kernel factors, QX_AES_COMP planes of nat x nat per system
     double* xprev;    // mixing vector of the step before (extrapolated MD start)
     double* xprev2;   // and of the step before that
+    double* xprev3;   // and of the one before that (cubic extrapolation)
+    double* xprev4;   // and of the one before that (quartic extrapolation)
     double* energy;   // per system: total
     double* eparts;   // per system: 10 parts (band, es2, es3, aes, axc, d4, atm, rep, ts, scc)
     int* iters;       // per system

@@ -621,6 +621,8 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
     double* xg = B.x + si.x_off;
     double* xp = B.xprev + si.x_off;    // converged moments of the step before
     double* xpp = B.xprev2 + si.x_off;  // and of the one before that
+    double* xp3 = B.xprev3 + si.x_off;  // and of the one before that
+    double* xp4 = B.xprev4 + si.x_off;  // and of the one before that
     const int warm = B.warm[sys];
     double* A = bf.A;
     double* Cc = bf.C;
@@ -629,14 +631,20 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
     double* rs = bf.rs;
 
     QX_PH_INIT
-    // initial moments (linear or quadratic extrapolation over the previous
-    // steps in MD) and basis C = X U (padding column zero)
+    // initial moments (polynomial extrapolation of order 1-4 over the
+    // previous steps' converged moments in MD) and basis C = X U (padding
+    // column zero)
     for (int i = tid; i < nx; i += kNT) {
         double v = 0.0;
-        if (warm >= 3 && opt.extrapolate >= 2) v = 3.0 * (xg[i] - xp[i]) + xpp[i];
+        if (warm >= 5 && opt.extrapolate >= 4)
+            v = 5.0 * (xg[i] - xp3[i]) - 10.0 * (xp[i] - xpp[i]) + xp4[i];
+        else if (warm >= 4 && opt.extrapolate >= 3) v = 4.0 * (xg[i] + xpp[i]) - 6.0 * xp[i] - xp3[i];
+        else if (warm >= 3 && opt.extrapolate >= 2) v = 3.0 * (xg[i] - xp[i]) + xpp[i];
         else if (warm >= 2 && opt.extrapolate >= 1) v = 2.0 * xg[i] - xp[i];
         else if (warm >= 1) v = xg[i];
         xin[i] = v;
+        if (warm >= 4) xp4[i] = xp3[i];
+        if (warm >= 3) xp3[i] = xpp[i];
         if (warm >= 2) xpp[i] = xp[i];
         if (warm >= 1) xp[i] = xg[i];
     }
@@ -953,7 +961,7 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         B.iters[sys] = itn;
         B.iters_total[sys] += (unsigned long long)itn;
         B.status[sys] = conv ? 0 : -1;
-        B.warm[sys] = warm >= 3 ? 3 : warm + 1;
+        B.warm[sys] = warm >= 5 ? 5 : warm + 1;
     }
 }
 
