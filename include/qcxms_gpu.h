@@ -78,6 +78,15 @@ int qcxms_batch_info(qcxms_batch *b, int32_t *iterations, double *eparts);
 int qcxms_batch_scc_iterations_total(qcxms_batch *b, uint64_t *total);
 /* Atomic dipoles [3*natoms] and traceless quadrupoles [6*natoms, xx xy yy xz yz zz]. */
 int qcxms_batch_multipoles(qcxms_batch *b, double *dipoles, double *quadrupoles);
+/* Orbitals of system sys after the batch's last evaluation, in ascending
+ * energy, as tblite's wavefunction holds them for the MO spectrum of
+ * getspec (tblite.f90:153-166, mo_energ.f90:7-80): emo[nao] (Eh),
+ * focc[nao] (0..2), coeff[nao*nao] with orbital k in coeff[k*nao .. k*nao+nao-1]
+ * (column-major over the AOs), overlap[nao*nao], aoat[nao] (1-based atom of
+ * each AO).  cap is the capacity of the per-AO buffers; *nao receives the
+ * basis size.  Any output pointer except nao may be NULL. */
+int qcxms_batch_orbitals(qcxms_batch *b, int32_t sys, int32_t cap, int32_t *nao, double *emo,
+                         double *focc, double *coeff, double *overlap, int32_t *aoat);
 
 /* ---- timing hooks (HIP events on the library's stream) ---- */
 /* enable != 0 records events around every kernel launch of this batch. */

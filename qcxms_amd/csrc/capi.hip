@@ -756,6 +756,57 @@ int qcxms_batch_multipoles(qcxms_batch* b, double* dipoles, double* quadrupoles)
     return QCXMS_OK;
 }
 
+int qcxms_batch_orbitals(qcxms_batch* b, int32_t sys, int32_t cap, int32_t* nao, double* emo, double* focc,
+                         double* coeff, double* overlap, int32_t* aoat)
+{
+    if (!b || !nao) return fail("null argument");
+    if (sys < 0 || sys >= b->nsys) return fail("system index out of range");
+    const SysInfo& si = b->info[sys];
+    const int n = si.nao, ld = si.ld;
+    *nao = n;
+    if (cap < n) return fail("orbital buffers smaller than the basis");
+    const size_t ms = (size_t)ld * ld;
+    std::vector<double> S(ms), X(ms), U(ms), e(n), f(n);
+    HIPCHK(hipMemcpyAsync(S.data(), b->d.S + si.mat_off, sizeof(double) * ms, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(X.data(), b->d.X + si.mat_off, sizeof(double) * ms, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(U.data(), b->d.U + si.mat_off, sizeof(double) * ms, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(e.data(), b->d.eps + (size_t)sys * QX_MAX_NAO, sizeof(double) * n, hipMemcpyDeviceToHost,
+                          g_stream));
+    HIPCHK(hipMemcpyAsync(f.data(), b->d.focc + (size_t)sys * QX_MAX_NAO, sizeof(double) * n,
+                          hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    // AO coefficients C = X U (X = L^-T, U = L^T C of the last SCC), orbitals
+    // in ascending energy (the Jacobi eigensolver leaves them unordered)
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return e[a] < e[c]; });
+    for (int k = 0; k < n; ++k) {
+        const int i = ord[k];
+        if (emo) emo[k] = e[i];
+        if (focc) focc[k] = 2.0 * f[i];
+        if (coeff)
+            for (int mu = 0; mu < n; ++mu) {
+                double c = 0.0;
+                for (int j = 0; j < n; ++j) c += X[(size_t)mu * ld + j] * U[(size_t)j * ld + i];
+                coeff[(size_t)k * n + mu] = c;
+            }
+    }
+    if (overlap)
+        for (int mu = 0; mu < n; ++mu)
+            for (int nu = 0; nu < n; ++nu) overlap[(size_t)nu * n + mu] = S[(size_t)mu * ld + nu];
+    if (aoat) {
+        int k = 0;
+        for (int a = 0; a < si.nat; ++a) {
+            const ElemParams& ep = g_host_tab.e[b->elem[si.atom_off + a]];
+            for (int sh = 0; sh < ep.nsh; ++sh) {
+                const int l = (ep.nsh == 2 && sh == 1) ? 1 : 0;
+                for (int c = 0; c < 2 * l + 1; ++c) aoat[k++] = a + 1;
+            }
+        }
+    }
+    return QCXMS_OK;
+}
+
 int qcxms_egrad_batch(int32_t nsys, const int32_t* offsets, const int32_t* num, const double* xyz,
                       const int32_t* charge, const int32_t* multiplicity, int32_t method, const double* etemp,
                       double* qat, double* energy, double* gradient, int32_t* stat)
@@ -867,6 +918,9 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
         return nullptr;
     }
     b->opts.keep = params->broyden_keep;
+    // warm-started MD steps start close to the fixed point: a lighter first
+    // mixing step (0.3 instead of 0.4) takes fewer SCC cycles per step
+    b->opts.damp = 0.3;
     if (params->extrapolate < -1 || params->extrapolate > 4) {
         delete md;  // owns the batch
         g_err = "extrapolate must lie in -1..4";

@@ -277,3 +277,139 @@ def vary_energies(e_in, e_distr, rng):
     z0 = r * math.cos(2.0 * math.pi * u2)
     z1 = r * math.sin(2.0 * math.pi * u2)
     return z0 * sigma + e_in if u1 > 0.5 else z1 * sigma + e_in
+
+
+# ---------------------------------------------------------------------------
+# MO spectrum of the ground state and the randomised ensemble (main.F90:756-886)
+def mo_populations(coeff, overlap, aoat, nat):
+    """write_qmo (mo_energ.f90:7-80): Mulliken population of every MO on
+    every atom, + 1e-10, normalised per MO.  Returns qmo [nao, nat]."""
+    nao = coeff.shape[0]
+    sc = overlap @ coeff                  # (S C)[j, n]
+    per_ao = coeff * sc                   # C[j, n] sum_k C[k, n] S[j, k]
+    qmo = np.zeros((nao, nat))
+    for ia in range(1, nat + 1):
+        qmo[:, ia - 1] = per_ao[aoat == ia].sum(axis=0) + 1.0e-10
+    return qmo / qmo.sum(axis=1, keepdims=True)
+
+
+SPEC_ETEMP = 1.0e-3  # K: integer occupations for the MO spectrum (reference: etemp 0)
+
+
+def getspec(num, xyz, charge=0):
+    """getspec (mo_spec.f90:12-119) for GFN2 through the GPU: a single point
+    of the ground state, orbital energies as positive ionisation energies
+    emo (Eh, -eps), the HOMO index ihomo (1-based), ehomo = emo[ihomo-1],
+    mopop [nao, nat], and nb = nel/2 (electrons_amount, iniqm.f90:45-63)."""
+    from .gpu import Batch
+    num = np.asarray(num, dtype=np.int32)
+    b = Batch([(num, charge)])
+    try:
+        b.set_warm_start(0)
+        e, g, q, st = b.egrad(np.asarray(xyz, dtype=np.float64), SPEC_ETEMP)
+        if st[0] != 0:
+            raise RuntimeError("QC initialization error")
+        eps, focc, coeff, ovl, aoat = b.orbitals(0)
+    finally:
+        b.close()
+    nel = int(sum(int(z) - (0 if z <= 2 else 2) for z in num)) - charge
+    nb = nel // 2
+    occ = np.nonzero(focc > 1e-6)[0]
+    ihomo = max(int(occ.max()) + 1 if len(occ) else 1, 1)
+    emo = -eps
+    if emo[0] - emo[ihomo - 1] < 1.0 / AUTOEV:   # eps in eV: emo(ihomo) - emo(1) < 1
+        raise RuntimeError("weird epsilons")
+    return dict(emo=emo, ehomo=float(emo[ihomo - 1]), ihomo=ihomo, nb=nb,
+                mopop=mo_populations(coeff, ovl, aoat, len(num)), energy=float(e[0]))
+
+
+def irand(n, rng):
+    """utility.f90:502-511: integer in 1..n."""
+    r = int(n * rng.random() + 1)
+    return min(r, n)
+
+
+def momap(ihomo, emo, edum, rng):
+    """utility.f90:23-65: the ionised orbital(s) whose energies plus an
+    excitation into a virtual orbital best match edum (Eh); mo2 = 0 for a
+    single hole.  5001 random trials, as the reference."""
+    dmin, i1, i2 = np.finfo(float).max, 0, 0
+    for _ in range(5001):
+        mo1, mo2 = irand(ihomo, rng), irand(ihomo, rng)
+        vmo = irand(ihomo // 2, rng) + ihomo
+        dum = emo[mo1 - 1]
+        if mo2 > ihomo // 2:
+            mo2 = 0
+        else:
+            dum = dum + emo[mo2 - 1]
+        dum = dum + emo[vmo - 1]
+        delta = abs(dum - edum)
+        if delta < dmin:
+            dmin, i1, i2 = delta, mo1, mo2
+    return i1, i2
+
+
+def generate_ensemble(num, gs_xyz, gs_velo, ntraj, spec, rng, eimp0_ev=70.0, eimpw=0.10,
+                      ieeatm=0.6, edistri=1, iee_a=0.0, iee_b=0.0, hacc=3.0, fimp=1.0,
+                      trelax=2000.0, unity=False):
+    """The EI ensemble of main.F90:663-886 from ground-state snapshots
+    (gs_xyz, gs_velo [ndump, nat, 3]) and the MO spectrum of getspec:
+    ntraj distinct snapshots chosen at random, each with an impact energy
+    drawn by Box-Muller, an internal excess energy drawn from the IEE
+    distribution by rejection, the ionised orbitals from momap, per-atom
+    heating factors from their populations and a heating time from
+    calctrelax.  Returns trajectory dicts (num, xyz, velo, velof, mass,
+    eimp [Eh], tadd [au]) and the IEE parameters (a, b, pmax).  Random
+    numbers come from rng (numpy Generator)."""
+    num = np.asarray(num, dtype=np.int32)
+    nuc = len(num)
+    ndump = len(gs_xyz)
+    if ndump <= 2 * ntraj:
+        raise ValueError("Error: compute longer GS trajectory")
+    emo, ehomo, ihomo, nb, mopop = (spec["emo"], spec["ehomo"], spec["ihomo"], spec["nb"],
+                                    spec["mopop"])
+    eimp0 = eimp0_ev / AUTOEV
+    exc = (eimp0 - ehomo) * AUTOEV
+    ieeel = float(ihomo + nb)
+    if not (iee_a > 0 and iee_b > 0):
+        iee_a, iee_b = getieeab(ieeel, edistri, exc, nuc, ieeatm)
+    _, pmax, _ = getmaxiee(iee_a, iee_b, ieeel, edistri, exc)
+    icalc = np.zeros(ndump, dtype=bool)
+    k = 0
+    while k <= ntraj:                      # main.F90:695-701 (ntraj + 1 picks)
+        j = irand(ndump, rng)
+        if not icalc[j - 1]:
+            k += 1
+            icalc[j - 1] = True
+    dist = gauss0 if edistri == 0 else poiss0
+    out = []
+    for i in range(ndump):
+        if not icalc[i]:
+            continue
+        if len(out) == ntraj:
+            break
+        while True:
+            edum = vary_energies(eimp0, eimpw, rng)
+            if edum >= ehomo:
+                break
+        edum = edum - ehomo
+        while True:
+            x = rng.random() * edum * AUTOEV
+            if dist(iee_a, iee_b, ieeel, x) / pmax >= rng.random():
+                break
+        iee = fimp * x / AUTOEV
+        mo1, mo2 = momap(ihomo, emo, iee + ehomo, rng)
+        modum = mopop[mo1 - 1].copy()
+        if mo2 > 0:
+            modum = modum + mopop[mo2 - 1]
+        modum[num == 1] *= hacc
+        top = modum.max()
+        velof = np.ones(nuc) if (nuc > 35 or unity) else modum / top
+        t = calctrelax(emo, ihomo, mo1, trelax)
+        if mo2 > 0:
+            t = t + calctrelax(emo, ihomo, mo2, trelax)
+        t = max(t / (ihomo + nb), trelax / 10.0)
+        out.append(dict(num=num, xyz=np.array(gs_xyz[i]), velo=np.array(gs_velo[i]), velof=velof,
+                        mass=np.array([W.atomic_mass(int(z)) for z in num]), eimp=iee,
+                        tadd=t * FSTOAU))
+    return out, (iee_a, iee_b, pmax)

@@ -47,6 +47,8 @@ def library():
         L.qcxms_batch_egrad.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _ip]
         L.qcxms_batch_info.argtypes = [C.c_void_p, _ip, _dp]
         L.qcxms_batch_multipoles.argtypes = [C.c_void_p, _dp, _dp]
+        L.qcxms_batch_orbitals.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp,
+                                           _dp, _ip]
         L.qcxms_batch_scc_iterations_total.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         L.qcxms_get_xtb_egrad.argtypes = [C.c_int32, _ip, _dp, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_double, _dp, _dp, _dp, _ip]
@@ -152,6 +154,22 @@ class Batch:
         q = np.zeros((self.natoms, 6))
         library().qcxms_batch_multipoles(self._h, _p(d), _p(q))
         return d, q
+
+    def orbitals(self, sys=0):
+        """(emo [nao] Eh ascending, focc [nao], coeff [nao(AO), nao(MO)],
+        overlap [nao, nao], aoat [nao] 1-based) of system sys after the last
+        evaluation."""
+        cap = 256
+        n = C.c_int32()
+        emo, focc = np.zeros(cap), np.zeros(cap)
+        coeff, ovl = np.zeros(cap * cap), np.zeros(cap * cap)
+        aoat = np.zeros(cap, dtype=np.int32)
+        if library().qcxms_batch_orbitals(self._h, int(sys), cap, C.byref(n), _p(emo), _p(focc),
+                                          _p(coeff), _p(ovl), _p(aoat, _ip)):
+            raise RuntimeError("qcxms_batch_orbitals failed: " + last_error())
+        k = n.value
+        return (emo[:k].copy(), focc[:k].copy(), coeff[:k * k].reshape(k, k).T.copy(),
+                ovl[:k * k].reshape(k, k).T.copy(), aoat[:k].copy())
 
 
 def egrad_batch(systems, xyz_list, etemp=300.0, method=GFN2_XTB):

@@ -86,3 +86,92 @@ def test_fortran_d_edge_cases():
     assert E.fortran_d(-1.0, 16, 8) == " -0.10000000D+01"
     assert E.fortran_d(1e-120, 16, 8) == "  0.10000000-119"
     assert E.fortran_i(123456, 4) == "****"
+
+
+def test_mo_populations_single_centre_and_normalised():
+    # orbitals on single atoms with S = 1: each MO lives on its own atom
+    aoat = np.array([1, 1, 2, 3])
+    coeff = np.eye(4)
+    q = E.mo_populations(coeff, np.eye(4), aoat, 3)
+    np.testing.assert_allclose(q.sum(axis=1), 1.0, rtol=0, atol=1e-15)
+    assert q[0, 0] > 1 - 1e-9 and q[2, 1] > 1 - 1e-9 and q[3, 2] > 1 - 1e-9
+    # a bonding combination over two atoms with overlap s splits evenly
+    s = 0.3
+    S = np.array([[1.0, s], [s, 1.0]])
+    c = np.array([[1.0], [1.0]]) / np.sqrt(2 + 2 * s)
+    q = E.mo_populations(np.hstack([c, c]), S, np.array([1, 2]), 2)
+    np.testing.assert_allclose(q[0], [0.5, 0.5], atol=1e-12)
+
+
+def test_momap_finds_best_orbital_triple():
+    """With few orbitals the 5001 random trials cover every (mo1, mo2, vmo)
+    combination, so momap returns the exhaustive optimum."""
+    rng = np.random.default_rng(1)
+    emo = np.array([0.9, 0.7, 0.55, 0.5, -0.05, -0.1, -0.2])  # ihomo = 4
+    ihomo = 4
+    for target in (0.4, 1.1, 0.62):
+        best, arg = np.inf, None
+        for m1 in range(1, 5):
+            for m2 in range(1, 5):
+                for v in range(5, 5 + ihomo // 2):
+                    m2e = 0 if m2 > ihomo // 2 else m2
+                    d = emo[m1 - 1] + (emo[m2 - 1] if m2e else 0.0) + emo[v - 1]
+                    if abs(d - target) < best:
+                        best, arg = abs(d - target), (m1, m2e)
+        m1, m2 = E.momap(ihomo, emo, target, rng)
+        d = emo[m1 - 1] + (emo[m2 - 1] if m2 else 0.0)
+        dvs = [abs(d + emo[v - 1] - target) for v in range(5, 7)]
+        assert min(dvs) == pytest.approx(best, abs=1e-15)
+
+
+def _toy_spec(nat, nao):
+    rng = np.random.default_rng(4)
+    ihomo = nao // 2
+    eps = np.sort(rng.uniform(-1.0, 0.3, nao))
+    eps[:ihomo] -= 0.2
+    pop = rng.random((nao, nat))
+    return dict(emo=-eps, ehomo=float(-eps[ihomo - 1]), ihomo=ihomo, nb=ihomo,
+                mopop=pop / pop.sum(axis=1, keepdims=True))
+
+
+def test_generate_ensemble_invariants():
+    num, xyz = W.octane(noise=0.0)
+    nat = len(num)
+    rng = np.random.default_rng(11)
+    gx = xyz[None] + rng.normal(0, 0.01, (60, nat, 3))
+    gv = rng.normal(0, 1e-4, (60, nat, 3))
+    spec = _toy_spec(nat, 50)
+    trajs, (a, b, pmax) = E.generate_ensemble(num, gx, gv, 20, spec, rng)
+    assert len(trajs) == 20 and a > 0 and b > 0 and pmax > 0
+    picks = {tuple(t["xyz"][0]) for t in trajs}
+    assert len(picks) == 20  # distinct snapshots
+    exc_max = (70.0 / E.AUTOEV) * 1.6
+    for t in trajs:
+        assert t["velof"].max() == pytest.approx(1.0)  # normalised (26 atoms <= 35)
+        assert 0.0 < t["eimp"] < exc_max
+        assert t["tadd"] >= 200.0 * E.FSTOAU - 1e-9  # at least trelax / 10
+    # more than 35 atoms: uniform heating factors
+    n2, x2 = W.chain_c12h24o2n2(seed=0)
+    spec2 = _toy_spec(len(n2), 88)
+    gx2 = x2[None] + rng.normal(0, 0.01, (12, len(n2), 3))
+    t2, _ = E.generate_ensemble(n2, gx2, np.zeros_like(gx2), 5, spec2, rng)
+    assert all(np.all(t["velof"] == 1.0) for t in t2)
+
+
+def test_iee_rejection_sampling_follows_distribution():
+    """The accepted excess energies follow P(E) on [0, E_imp - E_homo]: their
+    mean equals the distribution mean of getmaxiee (to sampling error)."""
+    num, xyz = W.octane(noise=0.0)
+    nat = len(num)
+    spec = _toy_spec(nat, 50)
+    spec["ehomo"] = 0.4
+    rng = np.random.default_rng(5)
+    gx = np.repeat(xyz[None], 1201, axis=0)
+    trajs, (a, b, pmax) = E.generate_ensemble(num, gx, np.zeros_like(gx), 600, spec, rng,
+                                              eimpw=0.0)
+    exc = (70.0 / E.AUTOEV - 0.4) * E.AUTOEV
+    x = np.arange(0.001, exc, 0.01)
+    p = np.array([E.poiss0(a, b, 25.0 + 25.0, v) for v in x])
+    mean = (p * x).sum() / p.sum()
+    got = np.mean([t["eimp"] * E.AUTOEV for t in trajs])
+    assert got == pytest.approx(mean, rel=0.05)

@@ -54,3 +54,51 @@ def test_tmpqcxms_inputs_drive_md(tmp_path):
         sl = slice(md.offsets[t], md.offsets[t + 1])
         assert out["results"][t].nstep == ref["state"].nstep == 6
         assert np.abs(out["xyz"][sl] - ref["xyz"]).max() <= 1e-8
+
+
+def test_getspec_orbitals_consistent_with_scc():
+    """Orbitals read back after a GPU single point (etemp 1e-3 K): C^T S C = 1,
+    occupations sum to the electron count, orbital energies ascending, and
+    the Mulliken charges rebuilt from orbitals and occupations equal the
+    SCC's own charges; the MO populations are normalised per orbital."""
+    import qcxms_amd
+    num, xyz = W.octane(noise=0.02, seed=4)
+    b = qcxms_amd.Batch([(num, 0)])
+    b.set_warm_start(0)
+    e, g, qat, st = b.egrad(xyz, E.SPEC_ETEMP)
+    eps, focc, C, S, aoat = b.orbitals(0)
+    b.close()
+    assert st[0] == 0
+    n = len(eps)
+    assert n == 50 and np.all(np.diff(eps) >= 0)
+    np.testing.assert_allclose(C.T @ S @ C, np.eye(n), atol=1e-9)
+    assert focc.sum() == pytest.approx(50.0, abs=1e-9)
+    P = (C * focc) @ C.T
+    pop = (P * S).sum(axis=1)
+    zval = np.array([1.0 if z == 1 else 4.0 for z in num])
+    q = zval - np.array([pop[aoat == a + 1].sum() for a in range(len(num))])
+    assert np.abs(q - qat).max() <= 1e-6
+    spec = E.getspec(num, xyz)
+    assert spec["ihomo"] == 25 and spec["nb"] == 25
+    np.testing.assert_allclose(spec["mopop"].sum(axis=1), 1.0, atol=1e-12)
+    assert spec["ehomo"] == pytest.approx(-eps[24], abs=1e-8)
+
+
+def test_generated_ensemble_runs_on_gpu():
+    """Ground-state snapshots and the MO spectrum from the GPU feed
+    generate_ensemble; the trajectories run in the batched MD."""
+    import qcxms_amd
+    tr = W.md_ensemble(1, base_seed=2)[0]
+    xs, vs = E.sample_ground_state(tr, 40, dumpstep=2)
+    spec = E.getspec(tr["num"], tr["xyz"])
+    trajs, (a, b, pmax) = E.generate_ensemble(tr["num"], xs, vs, 8, spec,
+                                              np.random.default_rng(0))
+    for t in trajs:
+        t["charge"] = 1
+    md = qcxms_amd.MdEnsemble(trajs, max_steps=5, use_graph=True)
+    md.run(6)
+    res = md.read()["results"]
+    md.close()
+    print(f"IEE a={a:.3f} b={b:.3f}; eimp (eV) {[round(t['eimp'] * E.AUTOEV, 2) for t in trajs]}",
+          flush=True)
+    assert all(r.nstep == 5 and r.exit_reason == 7 for r in res)
