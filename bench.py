@@ -61,11 +61,11 @@ def traffic_from_profiles():
     return None
 
 
-def run_gpu(ntraj, steps, warmup, use_graph_check=True):
+def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1):
     import qcxms_amd
     trajs = workload(ntraj)
     nbf = 50
-    md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=False)
+    md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=False, lanes=lanes)
     md.run(warmup)
     md.set_profiling(True)
     for k in ("setup", "scc", "grad", "md_pre", "md_post"):
@@ -96,7 +96,7 @@ def run_gpu(ntraj, steps, warmup, use_graph_check=True):
                scc_flops_per_launch=flops / max(scc_launches, 1))
     # the same steps replayed as one hipGraph per step (no per-kernel events)
     if use_graph_check:
-        md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True)
+        md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True, lanes=lanes)
         md.run(warmup)
         n0 = sum(r.nstep for r in md.read()["results"])
         t0 = time.perf_counter()
@@ -272,6 +272,8 @@ def main():
                     help="skip the hipGraph replay of the same steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration)")
+    ap.add_argument("--lanes", type=int, default=1,
+                    help="split the ensemble into this many groups on separate streams")
     ap.add_argument("--config", default="md256",
                     choices=["md256", "uneven1024", "stretched64", "c20md"],
                     help="md256: the headline batched MD (default); the others print the "
@@ -293,7 +295,8 @@ def main():
         print("bench.py measures one GPU only: this build runs every trajectory on a single "
               "MI355X and has no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    g = run_gpu(args.ntraj, args.steps, args.warmup, use_graph_check=not args.no_graph_check)
+    g = run_gpu(args.ntraj, args.steps, args.warmup, use_graph_check=not args.no_graph_check,
+                lanes=args.lanes)
     line = {
         "metric": METRIC,
         "value": g["value"],
@@ -310,7 +313,7 @@ def main():
         "config": {"workload": "batched EI MD: 256 x C8H18+ (26 atoms, 50 basis functions), "
                                "GFN2-xTB FP64, etemp 5000 K, 0.5 fs leap-frog, IEE Poisson "
                                "0.6 eV/atom (<= 1.5), tadd 100 fs, 500 K Maxwell-Boltzmann",
-                   "ntraj": args.ntraj, "trajectory_steps": g["traj_steps"],
+                   "ntraj": args.ntraj, "lanes": args.lanes, "trajectory_steps": g["traj_steps"],
                    "running_after": g["running"],
                    "scc_iterations_per_trajectory_step": g["scc_iters_per_traj_step"]},
         "roofline": {"bound": "mfma", "kernel": "k_scc",

@@ -147,6 +147,13 @@ void qcxms_md_destroy(qcxms_md *md);
 /* Advance every running trajectory by up to nsteps loop iterations;
  * *running receives the number still running afterwards (may be NULL). */
 int qcxms_md_run(qcxms_md *md, int32_t nsteps, int32_t *running);
+/* Advance n ensembles together, each on its own stream: step s of every
+ * ensemble is enqueued before step s + 1 of any, so that the device overlaps
+ * their kernels and fills the gaps that an ensemble's slowest trajectory
+ * leaves in each step.  running[n] (may be NULL) receives each ensemble's
+ * count of running trajectories.  No counterpart in the reference (each
+ * trajectory is a separate process there, bin/pqcxms:98). */
+int qcxms_md_run_group(qcxms_md **mds, int32_t n, int32_t nsteps, int32_t *running);
 /* Copy state back: any pointer may be NULL.  list = fragment labels. */
 int qcxms_md_read(qcxms_md *md, double *xyz, double *velo, double *grad, double *qat,
                   int32_t *list, qcxms_md_result *results);

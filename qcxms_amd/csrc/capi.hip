@@ -211,10 +211,15 @@ struct qcxms_batch {
     BatchPtrs d{};
     SysInfo* d_info = nullptr;
     int* d_elem = nullptr;
+    // every batch (and the MD ensemble built on it) runs on a stream of its
+    // own, so that the kernels of several ensembles overlap on the device
+    hipStream_t stream = nullptr;
 
     ~qcxms_batch()
     {
+        if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -225,7 +230,7 @@ int dalloc(qcxms_batch* b, T** p, size_t n)
 {
     if (n == 0) n = 1;
     HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
-    HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), g_stream));
+    HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), b->stream));
     b->allocs.push_back((void*)*p);
     return QCXMS_OK;
 }
@@ -337,10 +342,10 @@ int allocate(qcxms_batch* b)
     ALLOC(d_items, b->items.size());
     if (!b->items.empty())
         HIPCHK(hipMemcpyAsync(d_items, b->items.data(), sizeof(int2) * b->items.size(), hipMemcpyHostToDevice,
-                              g_stream));
+                              b->stream));
     d.spitem = d_items;
-    HIPCHK(hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * b->nsys, hipMemcpyHostToDevice, g_stream));
-    HIPCHK(hipMemcpyAsync(b->d_elem, b->elem.data(), sizeof(int) * b->natoms, hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * b->nsys, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_elem, b->elem.data(), sizeof(int) * b->natoms, hipMemcpyHostToDevice, b->stream));
     d.nsys = b->nsys;
     d.info = b->d_info;
     d.elem = b->d_elem;
@@ -421,12 +426,12 @@ int timed(qcxms_batch* b, Timer& t, F&& launch)
     if (b->profiling) {
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, g_stream));
+        HIPCHK(hipEventRecord(e0, b->stream));
     }
     launch();
     HIPCHK(hipGetLastError());
     if (b->profiling) {
-        HIPCHK(hipEventRecord(e1, g_stream));
+        HIPCHK(hipEventRecord(e1, b->stream));
         g_pending.push_back({e0, e1, &t});
     }
     return QCXMS_OK;
@@ -470,16 +475,16 @@ int enqueue_egrad(qcxms_batch* b, bool with_grad)
     int st;
     const size_t l1 = lds_setup_bytes(b), l2 = lds_scc_bytes(b);
     if ((st = timed(b, b->t_setup, [&] {
-             hipLaunchKernelGGL(qx::k_setup, dim3(b->nsys), dim3(QX_BLOCK), l1, g_stream, b->d, 1);
+             hipLaunchKernelGGL(qx::k_setup, dim3(b->nsys), dim3(QX_BLOCK), l1, b->stream, b->d, 1);
          })))
         return st;
     if ((st = timed(b, b->t_scc, [&] {
-             hipLaunchKernelGGL(qx::k_scc, dim3(b->nsys), dim3(QX_SCC_BLOCK), l2, g_stream, b->d, b->opts);
+             hipLaunchKernelGGL(qx::k_scc, dim3(b->nsys), dim3(QX_SCC_BLOCK), l2, b->stream, b->d, b->opts);
          })))
         return st;
     if (with_grad &&
         (st = timed(b, b->t_grad, [&] {
-             hipLaunchKernelGGL(qx::k_grad, dim3(b->nsys), dim3(QX_BLOCK), 0, g_stream, b->d);
+             hipLaunchKernelGGL(qx::k_grad, dim3(b->nsys), dim3(QX_BLOCK), 0, b->stream, b->d);
          })))
         return st;
     return QCXMS_OK;
@@ -487,7 +492,7 @@ int enqueue_egrad(qcxms_batch* b, bool with_grad)
 
 int launch_egrad(qcxms_batch* b, bool with_grad)
 {
-    if (!b->warm_start) HIPCHK(hipMemsetAsync(b->d.warm, 0, sizeof(int) * b->nsys, g_stream));
+    if (!b->warm_start) HIPCHK(hipMemsetAsync(b->d.warm, 0, sizeof(int) * b->nsys, b->stream));
     int st = enqueue_egrad(b, with_grad);
     if (st) return st;
     return b->profiling ? collect_times() : QCXMS_OK;
@@ -506,7 +511,7 @@ int set_etemp(qcxms_batch* b, const double* etemp)
     }
     if (changed)
         HIPCHK(hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * b->nsys, hipMemcpyHostToDevice,
-                              g_stream));
+                              b->stream));
     return QCXMS_OK;
 }
 
@@ -539,8 +544,8 @@ int mdalloc(qcxms_md* md, T** p, size_t n, const T* host = nullptr)
     if (n == 0) n = 1;
     HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
     md->allocs.push_back((void*)*p);
-    if (host) HIPCHK(hipMemcpyAsync(*p, host, n * sizeof(T), hipMemcpyHostToDevice, g_stream));
-    else HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), g_stream));
+    if (host) HIPCHK(hipMemcpyAsync(*p, host, n * sizeof(T), hipMemcpyHostToDevice, md->b->stream));
+    else HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), md->b->stream));
     return QCXMS_OK;
 }
 
@@ -550,12 +555,12 @@ int enqueue_md_step(qcxms_md* md)
     qcxms_batch* b = md->b;
     int st;
     if ((st = timed(b, b->t_md_pre, [&] {
-             hipLaunchKernelGGL(qx::k_md_pre, dim3(md->ntraj), dim3(64), 0, g_stream, b->d, md->m, md->p);
+             hipLaunchKernelGGL(qx::k_md_pre, dim3(md->ntraj), dim3(64), 0, b->stream, b->d, md->m, md->p);
          })))
         return st;
     if ((st = enqueue_egrad(b, true))) return st;
     return timed(b, b->t_md_post, [&] {
-        hipLaunchKernelGGL(qx::k_md_post, dim3(md->ntraj), dim3(64), 0, g_stream, b->d, md->m, md->p);
+        hipLaunchKernelGGL(qx::k_md_post, dim3(md->ntraj), dim3(64), 0, b->stream, b->d, md->m, md->p);
     });
 }
 
@@ -630,11 +635,17 @@ qcxms_batch* qcxms_batch_create(int32_t nsys, const int32_t* offsets, const int3
         return nullptr;
     }
     qcxms_batch* b = new qcxms_batch();
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_err = "stream creation failed";
+        b->stream = nullptr;
+        delete b;
+        return nullptr;
+    }
     if (layout(b, nsys, offsets, num, charge) != QCXMS_OK || allocate(b) != QCXMS_OK) {
         delete b;
         return nullptr;
     }
-    if (hipStreamSynchronize(g_stream) != hipSuccess) {
+    if (hipStreamSynchronize(b->stream) != hipSuccess) {
         g_err = "stream synchronisation failed during batch creation";
         delete b;
         return nullptr;
@@ -645,7 +656,7 @@ qcxms_batch* qcxms_batch_create(int32_t nsys, const int32_t* offsets, const int3
 void qcxms_batch_destroy(qcxms_batch* b)
 {
     if (!b) return;
-    (void)hipStreamSynchronize(g_stream);
+    (void)hipStreamSynchronize(b->stream);
     delete b;
 }
 
@@ -700,16 +711,16 @@ int qcxms_batch_egrad(qcxms_batch* b, const double* xyz, const double* etemp, do
     if (!b || !xyz) return fail("null argument");
     int st = set_etemp(b, etemp);
     if (st) return st;
-    HIPCHK(hipMemcpyAsync(b->d.xyz, xyz, sizeof(double) * 3 * b->natoms, hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipMemcpyAsync(b->d.xyz, xyz, sizeof(double) * 3 * b->natoms, hipMemcpyHostToDevice, b->stream));
     st = launch_egrad(b, gradient != nullptr);
     if (st) return st;
-    if (energy) HIPCHK(hipMemcpyAsync(energy, b->d.energy, sizeof(double) * b->nsys, hipMemcpyDeviceToHost, g_stream));
+    if (energy) HIPCHK(hipMemcpyAsync(energy, b->d.energy, sizeof(double) * b->nsys, hipMemcpyDeviceToHost, b->stream));
     if (gradient)
-        HIPCHK(hipMemcpyAsync(gradient, b->d.grad, sizeof(double) * 3 * b->natoms, hipMemcpyDeviceToHost, g_stream));
-    if (qat) HIPCHK(hipMemcpyAsync(qat, b->d.qat, sizeof(double) * b->natoms, hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(hipMemcpyAsync(gradient, b->d.grad, sizeof(double) * 3 * b->natoms, hipMemcpyDeviceToHost, b->stream));
+    if (qat) HIPCHK(hipMemcpyAsync(qat, b->d.qat, sizeof(double) * b->natoms, hipMemcpyDeviceToHost, b->stream));
     std::vector<int> hs(b->nsys);
-    HIPCHK(hipMemcpyAsync(hs.data(), b->d.status, sizeof(int) * b->nsys, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpyAsync(hs.data(), b->d.status, sizeof(int) * b->nsys, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     int worst = QCXMS_OK;
     for (int s = 0; s < b->nsys; ++s) {
         if (stat) stat[s] = hs[s];
@@ -722,18 +733,18 @@ int qcxms_batch_info(qcxms_batch* b, int32_t* iterations, double* eparts)
 {
     if (!b) return fail("null batch");
     if (iterations)
-        HIPCHK(hipMemcpyAsync(iterations, b->d.iters, sizeof(int) * b->nsys, hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(hipMemcpyAsync(iterations, b->d.iters, sizeof(int) * b->nsys, hipMemcpyDeviceToHost, b->stream));
     if (eparts)
-        HIPCHK(hipMemcpyAsync(eparts, b->d.eparts, sizeof(double) * 10 * b->nsys, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipStreamSynchronize(g_stream));
+        HIPCHK(hipMemcpyAsync(eparts, b->d.eparts, sizeof(double) * 10 * b->nsys, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     return QCXMS_OK;
 }
 
 int qcxms_batch_scc_iterations_total(qcxms_batch* b, uint64_t* total)
 {
     if (!b || !total) return fail("null argument");
-    HIPCHK(hipMemcpyAsync(total, b->d.iters_total, sizeof(uint64_t) * b->nsys, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpyAsync(total, b->d.iters_total, sizeof(uint64_t) * b->nsys, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     return QCXMS_OK;
 }
 
@@ -741,8 +752,8 @@ int qcxms_batch_multipoles(qcxms_batch* b, double* dipoles, double* quadrupoles)
 {
     if (!b) return fail("null batch");
     std::vector<double> x(b->nxtot);
-    HIPCHK(hipMemcpyAsync(x.data(), b->d.xo, sizeof(double) * b->nxtot, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpyAsync(x.data(), b->d.xo, sizeof(double) * b->nxtot, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     for (int s = 0; s < b->nsys; ++s) {
         const SysInfo& si = b->info[s];
         for (int a = 0; a < si.nat; ++a) {
@@ -767,14 +778,14 @@ int qcxms_batch_orbitals(qcxms_batch* b, int32_t sys, int32_t cap, int32_t* nao,
     if (cap < n) return fail("orbital buffers smaller than the basis");
     const size_t ms = (size_t)ld * ld;
     std::vector<double> S(ms), X(ms), U(ms), e(n), f(n);
-    HIPCHK(hipMemcpyAsync(S.data(), b->d.S + si.mat_off, sizeof(double) * ms, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipMemcpyAsync(X.data(), b->d.X + si.mat_off, sizeof(double) * ms, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipMemcpyAsync(U.data(), b->d.U + si.mat_off, sizeof(double) * ms, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(S.data(), b->d.S + si.mat_off, sizeof(double) * ms, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(X.data(), b->d.X + si.mat_off, sizeof(double) * ms, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(U.data(), b->d.U + si.mat_off, sizeof(double) * ms, hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipMemcpyAsync(e.data(), b->d.eps + (size_t)sys * QX_MAX_NAO, sizeof(double) * n, hipMemcpyDeviceToHost,
-                          g_stream));
+                          b->stream));
     HIPCHK(hipMemcpyAsync(f.data(), b->d.focc + (size_t)sys * QX_MAX_NAO, sizeof(double) * n,
-                          hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipStreamSynchronize(g_stream));
+                          hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     // AO coefficients C = X U (X = L^-T, U = L^T C of the last SCC), orbitals
     // in ascending energy (the Jacobi eigensolver leaves them unordered)
     std::vector<int> ord(n);
@@ -969,8 +980,8 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
     md->m.imass = d_imass;
     md->m.num = d_num;
     b->d.active = md->m.active;
-    if (hipMemcpyAsync(b->d.xyz, xyz, sizeof(double) * 3 * na, hipMemcpyHostToDevice, g_stream) != hipSuccess ||
-        hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * ntraj, hipMemcpyHostToDevice, g_stream) !=
+    if (hipMemcpyAsync(b->d.xyz, xyz, sizeof(double) * 3 * na, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * ntraj, hipMemcpyHostToDevice, b->stream) !=
             hipSuccess) {
         g_err = "upload failed";
         delete md;
@@ -980,8 +991,8 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
         delete md;
         return nullptr;
     }
-    hipLaunchKernelGGL(qx::k_md_init, dim3(ntraj), dim3(64), 0, g_stream, b->d, md->m, md->p);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g_stream) != hipSuccess) {
+    hipLaunchKernelGGL(qx::k_md_init, dim3(ntraj), dim3(64), 0, b->stream, b->d, md->m, md->p);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
         g_err = "initial energy/gradient failed on the device";
         delete md;
         return nullptr;
@@ -993,43 +1004,85 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
 void qcxms_md_destroy(qcxms_md* md)
 {
     if (!md) return;
-    (void)hipStreamSynchronize(g_stream);
+    (void)hipStreamSynchronize(md->b->stream);
     delete md;
 }
 
 qcxms_batch* qcxms_md_batch(qcxms_md* md) { return md ? md->b : nullptr; }
 
+}  // extern "C"
+
+namespace {
+
+// capture one MD step of md as a graph on its own stream (first use)
+int md_graph(qcxms_md* md)
+{
+    qcxms_batch* b = md->b;
+    if (md->graph_ready) return QCXMS_OK;
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int st = enqueue_md_step(md);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (st) return st;
+    if (ec != hipSuccess) return fail(std::string("graph capture failed: ") + hipGetErrorString(ec));
+    md->graph = g;
+    HIPCHK(hipGraphInstantiate(&md->exec, md->graph, nullptr, nullptr, 0));
+    md->graph_ready = true;
+    return QCXMS_OK;
+}
+
+// enqueue one MD step of md on its stream (graph replay or eager launches)
+int md_enqueue(qcxms_md* md)
+{
+    qcxms_batch* b = md->b;
+    if (md->use_graph && !b->profiling) {
+        HIPCHK(hipGraphLaunch(md->exec, b->stream));
+        return QCXMS_OK;
+    }
+    return enqueue_md_step(md);
+}
+
+int md_running(qcxms_md* md, int32_t* running)
+{
+    std::vector<int> act(md->ntraj);
+    HIPCHK(hipMemcpyAsync(act.data(), md->m.active, sizeof(int) * md->ntraj, hipMemcpyDeviceToHost, md->b->stream));
+    HIPCHK(hipStreamSynchronize(md->b->stream));
+    int n = 0;
+    for (int v : act) n += v;
+    *running = n;
+    return QCXMS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int qcxms_md_run(qcxms_md* md, int32_t nsteps, int32_t* running)
 {
     if (!md) return fail("null ensemble");
-    qcxms_batch* b = md->b;
+    return qcxms_md_run_group(&md, 1, nsteps, running);
+}
+
+int qcxms_md_run_group(qcxms_md** mds, int32_t n, int32_t nsteps, int32_t* running)
+{
+    if (!mds || n < 1) return fail("no ensembles");
     int st;
-    if (md->use_graph && !b->profiling) {
-        if (!md->graph_ready) {
-            HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
-            st = enqueue_md_step(md);
-            hipGraph_t g = nullptr;
-            const hipError_t ec = hipStreamEndCapture(g_stream, &g);
-            if (st) return st;
-            if (ec != hipSuccess) return fail(std::string("graph capture failed: ") + hipGetErrorString(ec));
-            md->graph = g;
-            HIPCHK(hipGraphInstantiate(&md->exec, md->graph, nullptr, nullptr, 0));
-            md->graph_ready = true;
-        }
-        for (int k = 0; k < nsteps; ++k) HIPCHK(hipGraphLaunch(md->exec, g_stream));
-    } else {
-        for (int k = 0; k < nsteps; ++k)
-            if ((st = enqueue_md_step(md))) return st;
+    for (int k = 0; k < n; ++k) {
+        if (!mds[k]) return fail("null ensemble");
+        if (mds[k]->use_graph && !mds[k]->b->profiling && (st = md_graph(mds[k]))) return st;
     }
-    HIPCHK(hipStreamSynchronize(g_stream));
-    if (b->profiling) collect_times();
-    if (running) {
-        std::vector<int> act(md->ntraj);
-        HIPCHK(hipMemcpy(act.data(), md->m.active, sizeof(int) * md->ntraj, hipMemcpyDeviceToHost));
-        int n = 0;
-        for (int v : act) n += v;
-        *running = n;
-    }
+    // step s of every ensemble before step s + 1 of any: each ensemble's
+    // stream keeps its own order, the device overlaps the streams
+    for (int s = 0; s < nsteps; ++s)
+        for (int k = 0; k < n; ++k)
+            if ((st = md_enqueue(mds[k]))) return st;
+    for (int k = 0; k < n; ++k) HIPCHK(hipStreamSynchronize(mds[k]->b->stream));
+    bool prof = false;
+    for (int k = 0; k < n; ++k) prof = prof || mds[k]->b->profiling;
+    if (prof) collect_times();
+    if (running)
+        for (int k = 0; k < n; ++k)
+            if ((st = md_running(mds[k], running + k))) return st;
     return QCXMS_OK;
 }
 
@@ -1039,17 +1092,17 @@ int qcxms_md_read(qcxms_md* md, double* xyz, double* velo, double* grad, double*
     if (!md) return fail("null ensemble");
     qcxms_batch* b = md->b;
     const size_t na = b->natoms;
-    if (xyz) HIPCHK(hipMemcpyAsync(xyz, b->d.xyz, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
-    if (velo) HIPCHK(hipMemcpyAsync(velo, md->m.velo, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
-    if (grad) HIPCHK(hipMemcpyAsync(grad, b->d.grad, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
-    if (qat) HIPCHK(hipMemcpyAsync(qat, b->d.qat, sizeof(double) * na, hipMemcpyDeviceToHost, g_stream));
-    if (list) HIPCHK(hipMemcpyAsync(list, md->m.list, sizeof(int) * na, hipMemcpyDeviceToHost, g_stream));
+    if (xyz) HIPCHK(hipMemcpyAsync(xyz, b->d.xyz, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, b->stream));
+    if (velo) HIPCHK(hipMemcpyAsync(velo, md->m.velo, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, b->stream));
+    if (grad) HIPCHK(hipMemcpyAsync(grad, b->d.grad, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, b->stream));
+    if (qat) HIPCHK(hipMemcpyAsync(qat, b->d.qat, sizeof(double) * na, hipMemcpyDeviceToHost, b->stream));
+    if (list) HIPCHK(hipMemcpyAsync(list, md->m.list, sizeof(int) * na, hipMemcpyDeviceToHost, b->stream));
     std::vector<qx::MdTraj> tr(md->ntraj);
     std::vector<int> it(md->ntraj), act(md->ntraj);
-    HIPCHK(hipMemcpyAsync(tr.data(), md->m.traj, sizeof(qx::MdTraj) * md->ntraj, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipMemcpyAsync(it.data(), b->d.iters, sizeof(int) * md->ntraj, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipMemcpyAsync(act.data(), md->m.active, sizeof(int) * md->ntraj, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpyAsync(tr.data(), md->m.traj, sizeof(qx::MdTraj) * md->ntraj, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(it.data(), b->d.iters, sizeof(int) * md->ntraj, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(act.data(), md->m.active, sizeof(int) * md->ntraj, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     if (results)
         for (int s = 0; s < md->ntraj; ++s) {
             const qx::MdTraj& t = tr[s];
@@ -1087,10 +1140,10 @@ int qcxms_md_read_averages(qcxms_md* md, double* axyz, double* achrg)
     const size_t na = b->natoms;
     std::vector<double> sx(3 * na), sq(na);
     std::vector<qx::MdTraj> tr(md->ntraj);
-    HIPCHK(hipMemcpyAsync(sx.data(), md->m.avxyz, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipMemcpyAsync(sq.data(), md->m.avchrg, sizeof(double) * na, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipMemcpyAsync(tr.data(), md->m.traj, sizeof(qx::MdTraj) * md->ntraj, hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpyAsync(sx.data(), md->m.avxyz, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(sq.data(), md->m.avchrg, sizeof(double) * na, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(tr.data(), md->m.traj, sizeof(qx::MdTraj) * md->ntraj, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     // md.f90:691-698 (EI: axyz = avxyz / kdump); a trajectory that has not
     // completed a step yet has nothing averaged and gets zeros
     for (int s = 0; s < md->ntraj; ++s) {

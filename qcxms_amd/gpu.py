@@ -242,6 +242,7 @@ def _md_sigs(L):
                                   C.POINTER(MdParams)]
     L.qcxms_md_destroy.argtypes = [C.c_void_p]
     L.qcxms_md_run.argtypes = [C.c_void_p, C.c_int32, _ip]
+    L.qcxms_md_run_group.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, _ip]
     L.qcxms_md_read.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _ip, C.POINTER(MdResult)]
     L.qcxms_md_read_averages.argtypes = [C.c_void_p, _dp, _dp]
     L.qcxms_md_batch.restype = C.c_void_p
@@ -257,42 +258,56 @@ class MdEnsemble:
 
     trajectories: list of dicts with num, xyz [nat,3] (Bohr), velo [nat,3]
     (au), mass [nat] (m_e), eimp (Eh), tadd (au); optional charge, imass,
-    velof."""
+    velof.  lanes > 1 splits the trajectories into that many contiguous
+    groups, each an ensemble on its own stream, advanced together
+    (qcxms_md_run_group) so that the device overlaps their kernels."""
 
     def __init__(self, trajectories, tstep=0.5 * 41.3413733365614, max_steps=2000,
                  nfragexit=3, etempin=-1.0, ax=0.0, ieetemp=0.0, use_graph=True,
-                 econv=0.0, pconv=0.0, extrapolate=0, broyden_keep=0):
+                 econv=0.0, pconv=0.0, extrapolate=0, broyden_keep=0, lanes=1):
         L = library()
         _md_sigs(L)
         nums = [np.asarray(t["num"], dtype=np.int32) for t in trajectories]
         self.offsets = np.cumsum([0] + [len(n) for n in nums]).astype(np.int32)
         self.ntraj = len(trajectories)
         self.natoms = int(self.offsets[-1])
-        cat = lambda key, dt, default=None: np.ascontiguousarray(np.concatenate(
-            [np.asarray(t.get(key, default(t) if default else None), dtype=dt).reshape(-1)
-             for t in trajectories]))
         self.num = np.concatenate(nums).astype(np.int32)
-        charge = np.array([int(t.get("charge", 1)) for t in trajectories], dtype=np.int32)
-        mass = cat("mass", np.float64)
-        imass = cat("imass", np.int32, lambda t: np.full(len(t["num"]), -1))
-        xyz = cat("xyz", np.float64)
-        velo = cat("velo", np.float64)
-        velof = cat("velof", np.float64, lambda t: np.ones(len(t["num"])))
-        eimp = np.array([float(t["eimp"]) for t in trajectories])
-        tadd = np.array([float(t["tadd"]) for t in trajectories])
+        lanes = max(1, min(int(lanes), self.ntraj))
+        bounds = [round(k * self.ntraj / lanes) for k in range(lanes + 1)]
+        self._lanes = [(bounds[k], bounds[k + 1]) for k in range(lanes)]
         p = MdParams(tstep, etempin, ax, ieetemp, nfragexit, max_steps, int(bool(use_graph)),
                      int(extrapolate), econv, pconv, int(broyden_keep), 0)
-        h = L.qcxms_md_create(self.ntraj, _p(self.offsets, _ip), _p(self.num, _ip), _p(charge, _ip),
-                              _p(mass), _p(imass, _ip), _p(xyz), _p(velo), _p(velof), _p(eimp),
-                              _p(tadd), C.byref(p))
-        if not h:
-            raise RuntimeError("qcxms_md_create failed: " + last_error())
-        self._h = C.c_void_p(h)
+        self._hs = []
+        for t0, t1 in self._lanes:
+            part = trajectories[t0:t1]
+            cat = lambda key, dt, default=None: np.ascontiguousarray(np.concatenate(
+                [np.asarray(t.get(key, default(t) if default else None), dtype=dt).reshape(-1)
+                 for t in part]))
+            off = (self.offsets[t0:t1 + 1] - self.offsets[t0]).astype(np.int32)
+            num = np.ascontiguousarray(self.num[self.offsets[t0]:self.offsets[t1]])
+            charge = np.array([int(t.get("charge", 1)) for t in part], dtype=np.int32)
+            mass = cat("mass", np.float64)
+            imass = cat("imass", np.int32, lambda t: np.full(len(t["num"]), -1))
+            xyz = cat("xyz", np.float64)
+            velo = cat("velo", np.float64)
+            velof = cat("velof", np.float64, lambda t: np.ones(len(t["num"])))
+            eimp = np.array([float(t["eimp"]) for t in part])
+            tadd = np.array([float(t["tadd"]) for t in part])
+            h = L.qcxms_md_create(len(part), _p(off, _ip), _p(num, _ip), _p(charge, _ip), _p(mass),
+                                  _p(imass, _ip), _p(xyz), _p(velo), _p(velof), _p(eimp), _p(tadd),
+                                  C.byref(p))
+            if not h:
+                self.close()
+                raise RuntimeError("qcxms_md_create failed: " + last_error())
+            self._hs.append(C.c_void_p(h))
+        self._h = self._hs[0]
 
     def close(self):
-        if getattr(self, "_h", None):
-            library().qcxms_md_destroy(self._h)
-            self._h = None
+        for h in getattr(self, "_hs", []):
+            if h:
+                library().qcxms_md_destroy(h)
+        self._hs = []
+        self._h = None
 
     def __del__(self):
         try:
@@ -301,46 +316,76 @@ class MdEnsemble:
             pass
 
     def run(self, nsteps):
-        running = C.c_int32()
-        if library().qcxms_md_run(self._h, int(nsteps), C.byref(running)):
+        n = len(self._hs)
+        arr = (C.c_void_p * n)(*[h.value for h in self._hs])
+        running = np.zeros(n, dtype=np.int32)
+        if library().qcxms_md_run_group(arr, n, int(nsteps), _p(running, _ip)):
             raise RuntimeError("qcxms_md_run failed: " + last_error())
-        return running.value
+        return int(running.sum())
+
+    def _atoms(self, k):
+        t0, t1 = self._lanes[k]
+        return slice(int(self.offsets[t0]), int(self.offsets[t1]))
 
     def read(self):
         xyz = np.zeros((self.natoms, 3)); velo = np.zeros((self.natoms, 3))
         grad = np.zeros((self.natoms, 3)); qat = np.zeros(self.natoms)
         lst = np.zeros(self.natoms, dtype=np.int32)
-        res = (MdResult * self.ntraj)()
-        if library().qcxms_md_read(self._h, _p(xyz), _p(velo), _p(grad), _p(qat), _p(lst, _ip), res):
-            raise RuntimeError("qcxms_md_read failed: " + last_error())
-        return dict(xyz=xyz, velo=velo, grad=grad, qat=qat, list=lst, results=list(res))
+        res = []
+        for k, h in enumerate(self._hs):
+            a = self._atoms(k)
+            t0, t1 = self._lanes[k]
+            r = (MdResult * (t1 - t0))()
+            parts = [np.zeros((a.stop - a.start, 3)) for _ in range(3)]
+            q = np.zeros(a.stop - a.start)
+            l = np.zeros(a.stop - a.start, dtype=np.int32)
+            if library().qcxms_md_read(h, _p(parts[0]), _p(parts[1]), _p(parts[2]), _p(q),
+                                       _p(l, _ip), r):
+                raise RuntimeError("qcxms_md_read failed: " + last_error())
+            xyz[a], velo[a], grad[a], qat[a], lst[a] = parts[0], parts[1], parts[2], q, l
+            res.extend(list(r))
+        return dict(xyz=xyz, velo=velo, grad=grad, qat=qat, list=lst, results=res)
 
     def read_averages(self):
         """(axyz [natoms,3], achrg [natoms]): the average geometry and
         Mulliken charges over the last avdump steps (md.f90:691-698)."""
         axyz = np.zeros((self.natoms, 3)); achrg = np.zeros(self.natoms)
-        if library().qcxms_md_read_averages(self._h, _p(axyz), _p(achrg)):
-            raise RuntimeError("qcxms_md_read_averages failed: " + last_error())
+        for k, h in enumerate(self._hs):
+            a = self._atoms(k)
+            x = np.zeros((a.stop - a.start, 3)); q = np.zeros(a.stop - a.start)
+            if library().qcxms_md_read_averages(h, _p(x), _p(q)):
+                raise RuntimeError("qcxms_md_read_averages failed: " + last_error())
+            axyz[a], achrg[a] = x, q
         return axyz, achrg
 
-    def batch_handle(self):
-        return library().qcxms_md_batch(self._h)
+    def batch_handle(self, lane=0):
+        return library().qcxms_md_batch(self._hs[lane])
 
     def set_profiling(self, on):
-        library().qcxms_batch_set_profiling(C.c_void_p(self.batch_handle()), int(bool(on)))
+        for k in range(len(self._hs)):
+            library().qcxms_batch_set_profiling(C.c_void_p(self.batch_handle(k)), int(bool(on)))
 
     def kernel_time(self, name):
-        ms = C.c_double()
-        n = C.c_int32()
-        library().qcxms_batch_kernel_time(C.c_void_p(self.batch_handle()), name.encode(),
-                                          C.byref(ms), C.byref(n))
-        return ms.value, n.value
+        """(ms, launches) summed over the lanes."""
+        tot, cnt = 0.0, 0
+        for k in range(len(self._hs)):
+            ms = C.c_double()
+            n = C.c_int32()
+            library().qcxms_batch_kernel_time(C.c_void_p(self.batch_handle(k)), name.encode(),
+                                              C.byref(ms), C.byref(n))
+            tot += ms.value
+            cnt += n.value
+        return tot, cnt
 
     def scc_iterations_total(self):
-        out = np.zeros(self.ntraj, dtype=np.uint64)
-        library().qcxms_batch_scc_iterations_total(C.c_void_p(self.batch_handle()),
-                                                   out.ctypes.data_as(C.POINTER(C.c_uint64)))
-        return out
+        out = []
+        for k in range(len(self._hs)):
+            t0, t1 = self._lanes[k]
+            o = np.zeros(t1 - t0, dtype=np.uint64)
+            library().qcxms_batch_scc_iterations_total(C.c_void_p(self.batch_handle(k)),
+                                                       o.ctypes.data_as(C.POINTER(C.c_uint64)))
+            out.append(o)
+        return np.concatenate(out)
 
 
 def fragments_batch(nums, xyzs, masses):

@@ -155,3 +155,24 @@ def test_bench_workload_at_default_thresholds_meets_parity():
         print(f"bench traj {t}: dE={de:.2e} dg={dg:.2e} dq={dq:.2e} dx={dx:.2e}", flush=True)
         assert res.nstep == st.nstep == nsteps
         assert de <= 1e-8 and dg <= 1e-6 and dq <= 1e-6
+
+
+def test_md_lanes_equal_single_ensemble():
+    """Trajectories split into lanes (ensembles on separate streams advanced
+    together by qcxms_md_run_group) reproduce the single ensemble bit for
+    bit: every trajectory's arithmetic is independent of its neighbours."""
+    trajs = ensemble(5, seed0=17)
+    a = q().MdEnsemble(trajs, max_steps=30, use_graph=True)
+    b = q().MdEnsemble(trajs, max_steps=30, use_graph=True, lanes=3)
+    ra_n = a.run(6)
+    rb_n = b.run(6)
+    ra, rb = a.read(), b.read()
+    xa, qa_ = a.read_averages()
+    xb, qb_ = b.read_averages()
+    a.close()
+    b.close()
+    assert ra_n == rb_n
+    assert np.array_equal(ra["xyz"], rb["xyz"]) and np.array_equal(ra["velo"], rb["velo"])
+    assert np.array_equal(ra["qat"], rb["qat"]) and np.array_equal(xa, xb)
+    assert [r.nstep for r in ra["results"]] == [r.nstep for r in rb["results"]]
+    assert [r.epot for r in ra["results"]] == [r.epot for r in rb["results"]]
