@@ -44,15 +44,18 @@ def workload(ntraj):
                          tadd_fs=100.0)
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_scc_hbm_traffic_c8h18_256.txt")
+TRAFFIC_FILES = {False: os.path.join(ROOT, "profiles", "k_scc_hbm_traffic_c8h18_256.txt"),
+                 True: os.path.join(ROOT, "profiles", "k_md_fused_hbm_traffic_c8h18_256.txt")}
 
 
-def traffic_from_profiles():
-    """HBM bytes per SCC launch from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by a plain-copy calibration), or None."""
-    if not os.path.exists(TRAFFIC_FILE):
+def traffic_from_profiles(fused=False):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain-copy
+    calibration), or None."""
+    path = TRAFFIC_FILES[fused]
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         if line.startswith("bytes_per_launch"):
             try:
                 return float(line.split("=")[1].split()[0])
@@ -61,14 +64,15 @@ def traffic_from_profiles():
     return None
 
 
-def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1):
+def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1, fused=False):
     import qcxms_amd
     trajs = workload(ntraj)
     nbf = 50
-    md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=False, lanes=lanes)
+    md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=False, lanes=lanes, fused=fused)
     md.run(warmup)
     md.set_profiling(True)
-    for k in ("setup", "scc", "grad", "md_pre", "md_post"):
+    names = ("fused",) if fused else ("setup", "scc", "grad", "md_pre", "md_post")
+    for k in names:
         md.kernel_time(k)
     it0 = md.scc_iterations_total().astype(np.float64).sum()
     n0 = sum(r.nstep for r in md.read()["results"])
@@ -78,13 +82,14 @@ def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1):
     res = md.read()["results"]
     n1 = sum(r.nstep for r in res)
     it1 = md.scc_iterations_total().astype(np.float64).sum()
-    kt = {k: md.kernel_time(k) for k in ("setup", "scc", "grad", "md_pre", "md_post")}
+    kt = {k: md.kernel_time(k) for k in names}
     md.set_profiling(False)
     running = sum(r.running for r in res)
     md.close()
     wall = t1 - t0
     traj_steps = n1 - n0
-    scc_ms, scc_launches = kt["scc"]
+    # the dominant kernel: the SCC, or the persistent kernel that contains it
+    scc_ms, scc_launches = kt["fused" if fused else "scc"]
     flops = (it1 - it0) * scc_flops(nbf)
     achieved = flops / (scc_ms * 1e-3) / 1e12 if scc_ms > 0 else 0.0
     out = dict(wall=wall, traj_steps=traj_steps, value=traj_steps / wall,
@@ -96,7 +101,7 @@ def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1):
                scc_flops_per_launch=flops / max(scc_launches, 1))
     # the same steps replayed as one hipGraph per step (no per-kernel events)
     if use_graph_check:
-        md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True, lanes=lanes)
+        md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True, lanes=lanes, fused=fused)
         md.run(warmup)
         n0 = sum(r.nstep for r in md.read()["results"])
         t0 = time.perf_counter()
@@ -190,10 +195,10 @@ def run_c20_md(ntraj, steps, warmup):
             "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()}}
 
 
-def run_one_per_launch(steps, warmup):
+def run_one_per_launch(steps, warmup, fused=False):
     import qcxms_amd
     trajs = workload(1)
-    md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True)
+    md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True, fused=fused)
     md.run(warmup)
     n0 = md.read()["results"][0].nstep
     t0 = time.perf_counter()
@@ -272,6 +277,8 @@ def main():
                     help="skip the hipGraph replay of the same steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration)")
+    ap.add_argument("--fused", type=int, default=0,
+                    help="1: persistent kernel running whole MD iterations per trajectory")
     ap.add_argument("--lanes", type=int, default=1,
                     help="split the ensemble into this many groups on separate streams")
     ap.add_argument("--config", default="md256",
@@ -296,7 +303,7 @@ def main():
               "MI355X and has no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     g = run_gpu(args.ntraj, args.steps, args.warmup, use_graph_check=not args.no_graph_check,
-                lanes=args.lanes)
+                lanes=args.lanes, fused=bool(args.fused))
     line = {
         "metric": METRIC,
         "value": g["value"],
@@ -313,20 +320,21 @@ def main():
         "config": {"workload": "batched EI MD: 256 x C8H18+ (26 atoms, 50 basis functions), "
                                "GFN2-xTB FP64, etemp 5000 K, 0.5 fs leap-frog, IEE Poisson "
                                "0.6 eV/atom (<= 1.5), tadd 100 fs, 500 K Maxwell-Boltzmann",
-                   "ntraj": args.ntraj, "lanes": args.lanes, "trajectory_steps": g["traj_steps"],
+                   "ntraj": args.ntraj, "lanes": args.lanes, "fused": bool(args.fused),
+                   "trajectory_steps": g["traj_steps"],
                    "running_after": g["running"],
                    "scc_iterations_per_trajectory_step": g["scc_iters_per_traj_step"]},
-        "roofline": {"bound": "mfma", "kernel": "k_scc",
+        "roofline": {"bound": "mfma", "kernel": "k_md_fused" if args.fused else "k_scc",
                      "achieved": g["achieved_tflops"], "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": g["achieved_tflops"] / FP64_PEAK_TFLOPS,
-                     "traffic": traffic_from_profiles(),
+                     "traffic": traffic_from_profiles(bool(args.fused)),
                      "flops_per_launch": g["scc_flops_per_launch"],
                      "avg_launch_ms": g["scc_avg_ms"]},
         "kernels_ms_avg": {k: v["ms_avg"] for k, v in g["kernels"].items()},
         "graph_replay_value": g.get("graph_value"),
     }
     if not args.no_one_per_launch:
-        line["one_per_launch"] = {"value": run_one_per_launch(100, 10),
+        line["one_per_launch"] = {"value": run_one_per_launch(100, 10, bool(args.fused)),
                                   "unit": "trajectory-steps/s"}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = run_cpu_baseline(args.cpu_steps)

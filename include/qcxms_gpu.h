@@ -118,7 +118,9 @@ typedef struct {
     double econv, pconv;  /* SCC thresholds (defaults 1e-9, 1e-8 when 0) */
     int32_t broyden_keep; /* > 0: keep up to this many Broyden secant pairs from step to
                              step (same fixed point); 0: fresh mixing history per step */
-    int32_t pad;
+    int32_t fused;        /* != 0: one persistent launch runs whole MD iterations per
+                             trajectory (bookkeeping, integrals, SCC, gradient) back to
+                             back; 0: one launch per kernel and step */
 } qcxms_md_params;
 
 /* Final or current state of one trajectory (md.f90:684-700 outputs). */

@@ -183,6 +183,8 @@ int ensure_init(int device)
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_setup, hipFuncAttributeMaxDynamicSharedMemorySize,
                                QX_LDS_MAXM * QX_LDS_MAXM * (int)sizeof(double)));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_grad, hipFuncAttributeMaxDynamicSharedMemorySize, 0));
+    HIPCHK(hipFuncSetAttribute((const void*)qx::k_md_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (4 * qx::kFastM * (qx::kFastM + 1) + qx::kRs * qx::kFastM) * (int)sizeof(double)));
     g_device = device;
     g_ready = true;
     return QCXMS_OK;
@@ -206,7 +208,7 @@ struct qcxms_batch {
     SccOpts opts{1e-9, 1e-8, 0.4, 1e-11, 100, 100, 0, 0};
     int warm_start = 0;
     int profiling = 0;
-    Timer t_setup, t_scc, t_grad, t_md_pre, t_md_post;
+    Timer t_setup, t_scc, t_grad, t_md_pre, t_md_post, t_fused;
     std::vector<void*> allocs;
     BatchPtrs d{};
     SysInfo* d_info = nullptr;
@@ -524,6 +526,7 @@ struct qcxms_md {
     qx::MdPtrs m{};
     int ntraj = 0;
     bool use_graph = false, graph_ready = false;
+    bool fused = false;  // persistent kernel: whole MD iterations per launch
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<void*> allocs;
@@ -698,6 +701,7 @@ int qcxms_batch_kernel_time(qcxms_batch* b, const char* kernel, double* ms, int3
     else if (!std::strcmp(kernel, "grad")) t = &b->t_grad;
     else if (!std::strcmp(kernel, "md_pre")) t = &b->t_md_pre;
     else if (!std::strcmp(kernel, "md_post")) t = &b->t_md_post;
+    else if (!std::strcmp(kernel, "fused")) t = &b->t_fused;
     else return fail("unknown kernel name");
     if (ms) *ms = t->ms;
     if (launches) *launches = t->launches;
@@ -916,6 +920,7 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
     md->b = b;
     md->ntraj = ntraj;
     md->use_graph = params->use_graph != 0;
+    md->fused = params->fused != 0;
     md->p.tstep = params->tstep;
     md->p.etempin = params->etempin;
     md->p.ax = params->ax;
@@ -1042,6 +1047,17 @@ int md_enqueue(qcxms_md* md)
     return enqueue_md_step(md);
 }
 
+// up to nsteps MD iterations of every running trajectory in one persistent launch
+int md_enqueue_fused(qcxms_md* md, int nsteps)
+{
+    qcxms_batch* b = md->b;
+    const size_t lds = std::max(lds_setup_bytes(b), lds_scc_bytes(b));
+    return timed(b, b->t_fused, [&] {
+        hipLaunchKernelGGL(qx::k_md_fused, dim3(md->ntraj), dim3(QX_SCC_BLOCK), lds, b->stream, b->d, md->m,
+                           md->p, b->opts, nsteps);
+    });
+}
+
 int md_running(qcxms_md* md, int32_t* running)
 {
     std::vector<int> act(md->ntraj);
@@ -1069,13 +1085,17 @@ int qcxms_md_run_group(qcxms_md** mds, int32_t n, int32_t nsteps, int32_t* runni
     int st;
     for (int k = 0; k < n; ++k) {
         if (!mds[k]) return fail("null ensemble");
-        if (mds[k]->use_graph && !mds[k]->b->profiling && (st = md_graph(mds[k]))) return st;
+        if (!mds[k]->fused && mds[k]->use_graph && !mds[k]->b->profiling && (st = md_graph(mds[k])))
+            return st;
     }
+    // persistent ensembles: all steps in one launch each
+    for (int k = 0; k < n; ++k)
+        if (mds[k]->fused && nsteps > 0 && (st = md_enqueue_fused(mds[k], nsteps))) return st;
     // step s of every ensemble before step s + 1 of any: each ensemble's
     // stream keeps its own order, the device overlaps the streams
     for (int s = 0; s < nsteps; ++s)
         for (int k = 0; k < n; ++k)
-            if ((st = md_enqueue(mds[k]))) return st;
+            if (!mds[k]->fused && (st = md_enqueue(mds[k]))) return st;
     for (int k = 0; k < n; ++k) HIPCHK(hipStreamSynchronize(mds[k]->b->stream));
     bool prof = false;
     for (int k = 0; k < n; ++k) prof = prof || mds[k]->b->profiling;

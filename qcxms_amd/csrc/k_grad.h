@@ -43,14 +43,20 @@ QX_DEV void grad_shell_pair(const ElemParams& ea, int ka, const ElemParams& eb, 
     }
 }
 
-__global__ __launch_bounds__(QX_BLOCK) void k_grad(BatchPtrs B)
+// LDS of one system's gradient
+struct GradLds {
+    SysShared ss;
+    double dedcn[QX_MAX_NAT], dedcn4[QX_MAX_NAT], gacc[3 * QX_MAX_NAT];
+};
+
+QX_DEV void grad_system(const BatchPtrs& B, int sys, GradLds& Lg)
 {
-    const int sys = blockIdx.x;
-    if (B.active && !B.active[sys]) return;
     const SysInfo si = B.info[sys];
     const int tid = threadIdx.x, nat = si.nat, nsh = si.nsh, ld = si.ld;
-    __shared__ SysShared ss;
-    __shared__ double dedcn[QX_MAX_NAT], dedcn4[QX_MAX_NAT], gacc[3 * QX_MAX_NAT];
+    SysShared& ss = Lg.ss;
+    double* dedcn = Lg.dedcn;
+    double* dedcn4 = Lg.dedcn4;
+    double* gacc = Lg.gacc;
     load_system(si, B, ss);
     const double* P = B.P + si.mat_off;
     const double* W = B.W + si.mat_off;
@@ -389,6 +395,14 @@ __global__ __launch_bounds__(QX_BLOCK) void k_grad(BatchPtrs B)
         }
         for (int c = 0; c < 3; ++c) B.grad[3 * (size_t)(si.atom_off + a) + c] = g[c];
     }
+}
+
+__global__ __launch_bounds__(QX_BLOCK) void k_grad(BatchPtrs B)
+{
+    const int sys = blockIdx.x;
+    if (B.active && !B.active[sys]) return;
+    __shared__ GradLds Lg;
+    grad_system(B, sys, Lg);
 }
 
 }  // namespace qx

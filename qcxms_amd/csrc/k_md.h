@@ -120,45 +120,68 @@ QX_DEV double dev_leapfrog(int nat, const double* mass, double dt, const double*
     return s_ke;
 }
 
-// impactscale (impact.f90:12-55) by a 64-lane block; returns the iteration
-// count k (k >= 20000 is the reference's stop)
+// impactscale (impact.f90:12-55); returns the iteration count k (k >= 20000
+// is the reference's stop).  The reference raises scal by 0.0002 (a REAL(4)
+// literal, a multiple of 2^-36 as a double) per iteration from 0; every
+// partial sum below 4 is then exact, so iteration k uses scal = (k - 1) c
+// exactly and the iterations are independent.  Each thread evaluates one k
+// (the kinetic energy summed in atom order as ekinet does), a pass covers
+// blockDim.x consecutive k, and the first k that ends the reference's loop
+// is the answer.  Called by all threads of the block (block size a multiple
+// of 64, at most QX_BLOCK); atoms: LDS for 5 nat doubles.
 QX_DEV int dev_impactscale(int nat, double* velo, const double* mass, const double* velof,
-                           double eimp, double ff, double e0, double* term)
+                           double eimp, double ff, double e0, double* atoms)
 {
-    const int lane = threadIdx.x;
-    __shared__ int s_go, s_k;
-    __shared__ double s_scal;
-    const double esoll = eimp * ff + e0;
-    double scal = 0.0;
-    int k = 0;  // advanced by lane 0 only
-    for (;;) {
-        for (int i = lane; i < nat; i += 64) {
-            const double* v = velo + 3 * i;
-            const double f = 1.0 + velof[i] * scal;
-            const double v0 = v[0] * f, v1 = v[1] * f, v2 = v[2] * f;
-            term[i] = mass[i] * (v0 * v0 + v1 * v1 + v2 * v2);
-        }
-        __syncthreads();
-        if (lane == 0) {
-            k = k + 1;
-            const double e = ekin_terms_sum(term, nat) * 0.5;
-            s_scal = scal + (double)0.0002f;
-            s_k = k;
-            s_go = (esoll - e > (double)0.001f && k < 20000);
-        }
-        __syncthreads();
-        scal = s_scal;
-        if (!s_go) break;
-    }
-    for (int i = lane; i < nat; i += 64) {
-        double* v = velo + 3 * i;
-        const double f = 1.0 + velof[i] * scal;
-        v[0] = v[0] * f;
-        v[1] = v[1] * f;
-        v[2] = v[2] * f;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    __shared__ int s_wmin[QX_BLOCK / 64];
+    for (int i = tid; i < nat; i += nt) {
+        atoms[5 * i] = velo[3 * i];
+        atoms[5 * i + 1] = velo[3 * i + 1];
+        atoms[5 * i + 2] = velo[3 * i + 2];
+        atoms[5 * i + 3] = mass[i];
+        atoms[5 * i + 4] = velof[i];
     }
     __syncthreads();
-    return s_k;
+    const double esoll = eimp * ff + e0;
+    const double c = (double)0.0002f;
+    constexpr int kNone = 1 << 30;
+    int kstop = kNone;
+    for (int base = 1; base <= 20000; base += nt) {
+        const int k = base + tid;
+        int mine = kNone;
+        if (k <= 20000) {
+            const double scal = (double)(k - 1) * c;
+            double e = 0.0;
+            for (int i = 0; i < nat; ++i) {
+                const double* a = atoms + 5 * i;
+                const double f = 1.0 + a[4] * scal;
+                const double v0 = a[0] * f, v1 = a[1] * f, v2 = a[2] * f;
+                e = e + a[3] * (v0 * v0 + v1 * v1 + v2 * v2);
+            }
+            e = e * 0.5;
+            if (!(esoll - e > (double)0.001f && k < 20000)) mine = k;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const int other = __shfl_xor(mine, o, 64);
+            mine = other < mine ? other : mine;
+        }
+        if ((tid & 63) == 0) s_wmin[tid >> 6] = mine;
+        __syncthreads();
+        for (int w = 0; w < nt / 64; ++w) kstop = s_wmin[w] < kstop ? s_wmin[w] : kstop;
+        __syncthreads();
+        if (kstop != kNone) break;
+    }
+    // the loop leaves scal one increment past the last iteration: k c
+    const double scal = (double)kstop * c;
+    for (int i = tid; i < nat; i += nt) {
+        double* v = velo + 3 * i;
+        const double f = 1.0 + atoms[5 * i + 4] * scal;
+        v[0] = atoms[5 * i] * f;
+        v[1] = atoms[5 * i + 1] * f;
+        v[2] = atoms[5 * i + 2] * f;
+    }
+    __syncthreads();
+    return kstop;
 }
 
 // fragment_structure(rcut = 3, at1 = 1, at2 = 0) + fragmass masses by a
@@ -260,16 +283,27 @@ __global__ __launch_bounds__(64) void k_md_init(BatchPtrs B, MdPtrs M, MdParams 
     }
 }
 
-__global__ __launch_bounds__(64) void k_md_pre(BatchPtrs B, MdPtrs M, MdParams p)
+// LDS of the MD bookkeeping of one trajectory
+struct MdLds {
+    double term[3 * QX_MAX_NAT];
+    double atoms[5 * QX_MAX_NAT];
+    unsigned long long adj[QX_MAX_NAT];
+    int s_exit, s_more;
+    double s_T;
+};
+
+// one MD loop iteration up to the force call (md.f90:285-395): time,
+// averages, drift and SCF checks, running sums, leap-frog.  Any block size
+// from 64 threads; lanes from 64 on take no atoms.
+QX_DEV void md_pre_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& p, int sys, MdLds& Lm)
 {
-    const int sys = blockIdx.x;
     if (!M.active[sys]) return;
     const SysInfo si = B.info[sys];
     const int lane = threadIdx.x, nat = si.nat, off = si.atom_off;
     MdTraj& t = M.traj[sys];
-    __shared__ double term[3 * QX_MAX_NAT];
-    __shared__ int s_exit;
-    __shared__ double s_T;
+    double* term = Lm.term;
+    int& s_exit = Lm.s_exit;
+    double& s_T = Lm.s_T;
     if (lane == 0) {
         int nstep = t.nstep + 1;
         t.nstep = nstep;
@@ -322,16 +356,17 @@ __global__ __launch_bounds__(64) void k_md_pre(BatchPtrs B, MdPtrs M, MdParams p
     }
 }
 
-__global__ __launch_bounds__(64) void k_md_post(BatchPtrs B, MdPtrs M, MdParams p)
+// the rest of the iteration after the force call (md.f90:396-682): checkqc,
+// heating, connectivity, fragment temperatures, electronic temperature,
+// exits.  Any block size from 64 threads.
+QX_DEV void md_post_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& p, int sys, MdLds& Lm)
 {
-    const int sys = blockIdx.x;
     if (!M.active[sys]) return;
     const SysInfo si = B.info[sys];
     const int lane = threadIdx.x, nat = si.nat, off = si.atom_off;
     MdTraj& t = M.traj[sys];
-    __shared__ double term[QX_MAX_NAT];
-    __shared__ unsigned long long adj[QX_MAX_NAT];
-    __shared__ int s_more;
+    unsigned long long* adj = Lm.adj;
+    int& s_more = Lm.s_more;
     if (lane == 0) {
         double e = B.energy[sys];
         checkqc(nat, e, B.grad + 3 * (size_t)off, B.qat + off, si.charge);
@@ -345,7 +380,7 @@ __global__ __launch_bounds__(64) void k_md_post(BatchPtrs B, MdPtrs M, MdParams 
     // heating: impactscale (impact.f90:12-55) with ff = fadd * nstep
     if (s_more) {
         const int k = dev_impactscale(nat, M.velo + 3 * (size_t)off, M.mass + off, M.velof + off,
-                                      t.eimp, t.fadd * t.nstep, t.ekinstart, term);
+                                      t.eimp, t.fadd * t.nstep, t.ekinstart, Lm.atoms);
         if (k >= 20000) {
             if (lane == 0) {
                 t.done = 1;
@@ -416,6 +451,53 @@ __global__ __launch_bounds__(64) void k_md_post(BatchPtrs B, MdPtrs M, MdParams 
     }
 }
 
+__global__ __launch_bounds__(64) void k_md_pre(BatchPtrs B, MdPtrs M, MdParams p)
+{
+    __shared__ MdLds Lm;
+    md_pre_system(B, M, p, blockIdx.x, Lm);
+}
+
+__global__ __launch_bounds__(64) void k_md_post(BatchPtrs B, MdPtrs M, MdParams p)
+{
+    __shared__ MdLds Lm;
+    md_post_system(B, M, p, blockIdx.x, Lm);
+}
+
+// Persistent MD: one block of QX_SCC_BLOCK threads per trajectory runs up to
+// nsteps whole loop iterations (bookkeeping, setup, SCC, gradient,
+// bookkeeping) back to back, so no trajectory waits at a kernel boundary for
+// the slowest one.  The phases share one LDS allocation; the dynamic LDS
+// holds the setup's Cholesky factor and the SCC's matrices in turn.
+union FusedLds {
+    SetupLds s;
+    SccLds c;
+    GradLds g;
+    MdLds m;
+};
+
+__global__ __launch_bounds__(QX_SCC_BLOCK) void k_md_fused(BatchPtrs B, MdPtrs M, MdParams p, SccOpts opt,
+                                                           int nsteps)
+{
+    static_assert(QX_SCC_BLOCK == QX_BLOCK, "the fused phases assume one block size");
+    const int sys = blockIdx.x;
+    __shared__ FusedLds L;
+    extern __shared__ double dyn[];
+    for (int step = 0; step < nsteps; ++step) {
+        if (!M.active[sys]) break;
+        md_pre_system(B, M, p, sys, L.m);
+        __syncthreads();
+        if (!M.active[sys]) break;
+        setup_system(B, sys, 1, L.s, dyn);
+        __syncthreads();
+        scc_system(B, opt, sys, L.c, dyn);
+        __syncthreads();
+        grad_system(B, sys, L.g);
+        __syncthreads();
+        md_post_system(B, M, p, sys, L.m);
+        __syncthreads();
+    }
+}
+
 }  // namespace qx
 
 namespace qx {
@@ -452,9 +534,9 @@ __global__ __launch_bounds__(64) void k_impactscale_batch(const int* offsets, do
                                                           const double* e0, int* iters)
 {
     const int sys = blockIdx.x, off = offsets[sys], nat = offsets[sys + 1] - off;
-    __shared__ double term[QX_MAX_NAT];
+    __shared__ double atoms[5 * QX_MAX_NAT];
     const int k = dev_impactscale(nat, velo + 3 * (size_t)off, mass + off, velof + off, eimp[sys], ff[sys],
-                                  e0[sys], term);
+                                  e0[sys], atoms);
     if (threadIdx.x == 0) iters[sys] = k;
 }
 

@@ -965,34 +965,45 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
     }
 }
 
-__global__ __launch_bounds__(kNT) void k_scc(BatchPtrs B, SccOpts opt)
+// LDS of one system's SCC
+struct SccLds {
+    SysShared ss;
+    double xin[kMaxNx], xout[kMaxNx], vs[QX_MAX_NSH], vd[3 * QX_MAX_NAT], vq[6 * QX_MAX_NAT];
+    double eps[QX_MAX_NAO], focc[QX_MAX_NAO], f2[QX_MAX_NAO];
+    double2 rot[QX_MAX_NAO / 2];
+    int2 rpq[QX_MAX_NAO / 2];
+    double s_e[4];
+    int s_any[2];
+};
+
+// the whole SCC of system sys by the block (dyn: the launch's dynamic LDS)
+QX_DEV void scc_system(const BatchPtrs& B, const SccOpts& opt, int sys, SccLds& L, double* dyn)
 {
-    const int sys = blockIdx.x;
-    if (B.active && !B.active[sys]) return;
     const SysInfo si = B.info[sys];
-    __shared__ SysShared ss;
-    __shared__ double xin[kMaxNx], xout[kMaxNx], vs[QX_MAX_NSH], vd[3 * QX_MAX_NAT],
-        vq[6 * QX_MAX_NAT];
-    __shared__ double eps[QX_MAX_NAO], focc[QX_MAX_NAO], f2[QX_MAX_NAO];
-    __shared__ double2 rot[QX_MAX_NAO / 2];
-    __shared__ int2 rpq[QX_MAX_NAO / 2];
-    __shared__ double s_e[4];
-    __shared__ int s_any[2];
-    extern __shared__ double dyn[];
-    load_system(si, B, ss);
+    load_system(si, B, L.ss);
     const int m = (si.nao + 1) & ~1;
     if (m <= kFastM) {
         // odd leading dimension: column-wise LDS accesses hit distinct banks
         const size_t mm = (size_t)m * (m + 1);
         SccBuffers bf{dyn, dyn + mm, dyn + 2 * mm, dyn + 3 * mm, dyn + 4 * mm, m + 1};
-        scc_body<true>(si, B, opt, ss, bf, xin, xout, vs, vd, vq, eps, focc, f2, rot, rpq, s_e, s_any);
+        scc_body<true>(si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc, L.f2,
+                       L.rot, L.rpq, L.s_e, L.s_any);
     } else {
         // the energy-weighted density slot doubles as the row-sum scratch
         SccBuffers bf{B.T + si.mat_off, B.C + si.mat_off, B.U + si.mat_off, B.P + si.mat_off,
                       B.W + si.mat_off, si.ld};
-        scc_body<false>(si, B, opt, ss, bf, xin, xout, vs, vd, vq, eps, focc, f2, rot, rpq, s_e,
-                        s_any);
+        scc_body<false>(si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc,
+                        L.f2, L.rot, L.rpq, L.s_e, L.s_any);
     }
+}
+
+__global__ __launch_bounds__(kNT) void k_scc(BatchPtrs B, SccOpts opt)
+{
+    const int sys = blockIdx.x;
+    if (B.active && !B.active[sys]) return;
+    __shared__ SccLds L;
+    extern __shared__ double dyn[];
+    scc_system(B, opt, sys, L, dyn);
 }
 
 }  // namespace qx

@@ -141,22 +141,33 @@ QX_DEV void setup_shell_pair(const ShellDesc& sa, const ShellDesc& sb, const Ele
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(QX_BLOCK) void k_setup(BatchPtrs B, int with_chol)
+// LDS of one system's setup (the Cholesky factor uses the dynamic LDS)
+struct SetupLds {
+    ShellDesc sh[QX_MAX_NSH];
+    int ao_at[QX_MAX_NAO], ao_sh[QX_MAX_NAO];
+    double sx[3 * QX_MAX_NAT];
+    int sel[QX_MAX_NAT];
+    double scn[QX_MAX_NAT], scnd4[QX_MAX_NAT];
+    int refoff[QX_MAX_NAT + 1];
+    double red[QX_BLOCK / 64];
+    double bcast;
+};
+
+QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& Ls, double* dyn)
 {
-    const int sys = blockIdx.x;
-    if (B.active && !B.active[sys]) return;
     const SysInfo si = B.info[sys];
     const int tid = threadIdx.x;
     const int nat = si.nat, nsh = si.nsh, n = si.nao, ld = si.ld;
-    __shared__ ShellDesc sh[QX_MAX_NSH];
-    __shared__ int ao_at[QX_MAX_NAO], ao_sh[QX_MAX_NAO];
-    __shared__ double sx[3 * QX_MAX_NAT];
-    __shared__ int sel[QX_MAX_NAT];
-    __shared__ double scn[QX_MAX_NAT], scnd4[QX_MAX_NAT];
-    __shared__ int refoff[QX_MAX_NAT + 1];
-    __shared__ double red[QX_BLOCK / 64];
-    __shared__ double bcast;
-    extern __shared__ double dyn[];
+    ShellDesc* sh = Ls.sh;
+    int* ao_at = Ls.ao_at;
+    int* ao_sh = Ls.ao_sh;
+    double* sx = Ls.sx;
+    int* sel = Ls.sel;
+    double* scn = Ls.scn;
+    double* scnd4 = Ls.scnd4;
+    int* refoff = Ls.refoff;
+    double* red = Ls.red;
+    double& bcast = Ls.bcast;
 
     const double* gx = B.xyz + 3 * (size_t)si.atom_off;
     for (int i = tid; i < 3 * nat; i += QX_BLOCK) sx[i] = gx[i];
@@ -454,6 +465,15 @@ __global__ __launch_bounds__(QX_BLOCK) void k_setup(BatchPtrs B, int with_chol)
         }
     }
     // X as stored: row c holds column c of L^-1, i.e. Xg = (L^-1)^T row-major
+}
+
+__global__ __launch_bounds__(QX_BLOCK) void k_setup(BatchPtrs B, int with_chol)
+{
+    const int sys = blockIdx.x;
+    if (B.active && !B.active[sys]) return;
+    __shared__ SetupLds Ls;
+    extern __shared__ double dyn[];
+    setup_system(B, sys, with_chol, Ls, dyn);
 }
 
 }  // namespace qx

@@ -153,6 +153,38 @@ inline T shfl_xor(T v, int mask, int width)
     return out;
 }
 
+// v_readlane: every lane of the wave receives lane l's value; l must be the
+// same in all lanes of the wave.
+inline int readlane(int v, int l)
+{
+    const int me = threadIdx.x & 63;
+    Wave& w = wave();
+    w.a[me] = (double)v;
+    w.b[me] = (double)l;
+    w.bar.wait("lane read");
+    if (l < 0 || l > 63 || w.b[0] != (double)l) {
+        std::fprintf(stderr, "cpu emulation: lane read of lane %d not uniform (thread %u)\n", l, threadIdx.x);
+        std::abort();
+    }
+    const int out = (int)w.a[l];
+    w.bar.wait("lane read");
+    return out;
+}
+
+inline long long double_as_longlong(double d)
+{
+    long long b;
+    std::memcpy(&b, &d, sizeof b);
+    return b;
+}
+
+inline double longlong_as_double(long long b)
+{
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+}
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 // D[i][j] = C + sum_k A[i][k] B[k][j] over a 16x16x4 tile: lane l holds
@@ -238,6 +270,9 @@ inline void run_or_record(std::function<void()> f)
 #define __shfl_xor(v, m, w) qx_emu::shfl_xor((v), (m), (w))
 #define __builtin_amdgcn_mfma_f64_16x16x4f64 qx_emu::mfma_f64_16x16x4
 #define __ffsll(x) __builtin_ffsll(x)
+#define __builtin_amdgcn_readlane(v, l) qx_emu::readlane((v), (l))
+#define __double_as_longlong(d) qx_emu::double_as_longlong(d)
+#define __longlong_as_double(b) qx_emu::longlong_as_double(b)
 
 template <class... P, class... A>
 inline void hipLaunchKernelGGL(void (*k)(P...), dim3 grid, dim3 block, size_t shmem, hipStream_t, A... args)
@@ -263,6 +298,11 @@ inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned)
     return hipSuccess;
 }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t s)
+{
+    delete s;
+    return hipSuccess;
+}
 inline hipError_t hipMalloc(void** p, size_t n)
 {
     *p = std::malloc(n ? n : 1);

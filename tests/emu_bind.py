@@ -51,3 +51,45 @@ def egrad_batch(systems, xyz_list, etemp, method=2, timeout=1800):
     grads = [g[off[i]:off[i + 1]] for i in range(nsys)]
     qs = [q[off[i]:off[i + 1]] for i in range(nsys)]
     return rc, e, grads, qs, st, p.stderr
+
+
+EMU_MD = os.path.join(EMU_DIR, "_build", "md_emu")
+
+
+def md_run(trajs, nsteps, fused, use_graph=1, timeout=1800):
+    """MD ensemble through the emulated library: (rc, running, xyz, velo, qat,
+    [(nstep, exit_reason, epot, ekin)])."""
+    nt = len(trajs)
+    off = np.cumsum([0] + [len(t["num"]) for t in trajs]).astype(np.int32)
+    cat = lambda k, dt: np.concatenate([np.asarray(t[k], dtype=dt).reshape(-1) for t in trajs])
+    nat = int(off[-1])
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([nt, nsteps, int(fused), int(use_graph)], dtype=np.int32).tofile(f)
+            off.tofile(f)
+            cat("num", np.int32).tofile(f)
+            np.array([t.get("charge", 1) for t in trajs], dtype=np.int32).tofile(f)
+            np.full(nat, -1, dtype=np.int32).tofile(f)
+            cat("mass", np.float64).tofile(f)
+            cat("xyz", np.float64).tofile(f)
+            cat("velo", np.float64).tofile(f)
+            np.ones(nat).tofile(f)
+            np.array([t["eimp"] for t in trajs], dtype=np.float64).tofile(f)
+            np.array([t["tadd"] for t in trajs], dtype=np.float64).tofile(f)
+        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+        p = subprocess.run([EMU_MD, fin, fout], capture_output=True, text=True, timeout=timeout, env=env)
+        if p.returncode != 0:
+            raise RuntimeError(f"emulated MD failed ({p.returncode}):\n{p.stderr[-6000:]}")
+        raw = open(fout, "rb").read()
+    rc, running = np.frombuffer(raw, np.int32, 2, 0)
+    o = 8
+    xyz = np.frombuffer(raw, np.float64, 3 * nat, o).reshape(nat, 3).copy(); o += 24 * nat
+    velo = np.frombuffer(raw, np.float64, 3 * nat, o).reshape(nat, 3).copy(); o += 24 * nat
+    q = np.frombuffer(raw, np.float64, nat, o).copy(); o += 8 * nat
+    res = []
+    for _ in range(nt):
+        ns, ex = np.frombuffer(raw, np.int32, 2, o); o += 8
+        ep, ek = np.frombuffer(raw, np.float64, 2, o); o += 16
+        res.append((int(ns), int(ex), float(ep), float(ek)))
+    return int(rc), int(running), xyz, velo, q, res

@@ -38,3 +38,34 @@ def test_emulated_kernels_bounds_and_parity():
     dq = max(np.abs(q[i] - rq[i]).max() for i in range(len(systems)))
     print(f"emulated kernels: dE={de:.2e} dg={dg:.2e} dq={dq:.2e}")
     assert de <= 1e-8 and dg <= 1e-6 and dq <= 1e-6
+
+
+@pytest.mark.timeout(900)
+def test_emulated_md_heating_fused_and_stepwise():
+    """Three MD steps inside the heating window (impactscale every step) of a
+    5-atom and a 3-atom trajectory through the emulated library: one
+    persistent launch (k_md_fused) and the per-step kernels give identical
+    bits, and both follow the oracle's trajectory."""
+    eb.build()
+    trajs = []
+    for s in (5004, 5026):
+        num, xyz, charge = W.random_chain(s)
+        mass = np.array([W.atomic_mass(int(z)) for z in num])
+        trajs.append(dict(num=num, xyz=xyz, velo=W.maxwell_boltzmann(mass, 800.0, s), mass=mass,
+                          eimp=0.05, tadd=100.0 * W.FSTOAU, charge=charge))
+    steps = 3
+    a = eb.md_run(trajs, steps, fused=0)
+    b = eb.md_run(trajs, steps, fused=1)
+    assert a[0] == 0 and b[0] == 0
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+    assert [r[:2] for r in a[5]] == [r[:2] for r in b[5]]
+    off = np.cumsum([0] + [len(t["num"]) for t in trajs])
+    for t, tr in enumerate(trajs):
+        r = ob.md_run(tr, steps_limit=0, max_steps=steps, charge=tr["charge"],
+                      opts=ob.default_opts(econv=1e-12, pconv=1e-10))
+        assert r["state"].nstep == a[5][t][0] == steps
+        sl = slice(off[t], off[t + 1])
+        dx = np.abs(a[2][sl] - r["xyz"]).max()
+        dv = np.abs(a[3][sl] - r["velo"]).max()
+        print(f"emulated md traj {t}: dx={dx:.2e} dv={dv:.2e}")
+        assert dx < 1e-8 and dv < 1e-8

@@ -176,3 +176,50 @@ def test_md_lanes_equal_single_ensemble():
     assert np.array_equal(ra["qat"], rb["qat"]) and np.array_equal(xa, xb)
     assert [r.nstep for r in ra["results"]] == [r.nstep for r in rb["results"]]
     assert [r.epot for r in ra["results"]] == [r.epot for r in rb["results"]]
+
+
+def test_fused_md_matches_oracle_md():
+    """The persistent MD kernel (whole iterations per launch) against the
+    oracle MD over 30 steps of heating trajectories, as
+    test_md_ensemble_matches_oracle_md does for the per-kernel path."""
+    trajs = ensemble(4, seed0=41)
+    nsteps = 30
+    md = q().MdEnsemble(trajs, max_steps=nsteps, fused=True, econv=1e-11, pconv=1e-9)
+    running = md.run(nsteps + 5)
+    out = md.read()
+    axyz, achrg = md.read_averages()
+    md.close()
+    assert running == 0
+    opts = ob.default_opts(econv=1e-12, pconv=1e-10)
+    for t, tr in enumerate(trajs):
+        ref = ob.md_run(tr, steps_limit=0, max_steps=nsteps, opts=opts)
+        st, res = ref["state"], out["results"][t]
+        sl = slice(md.offsets[t], md.offsets[t + 1])
+        print(f"fused traj {t}: nstep {res.nstep}/{st.nstep} dEpot {abs(res.epot - st.epot):.2e} "
+              f"dx {np.abs(out['xyz'][sl] - ref['xyz']).max():.2e}", flush=True)
+        assert res.nstep == st.nstep and res.exit_reason == st.exit_reason
+        assert np.array_equal(out["list"][sl], ref["list"])
+        assert abs(res.epot - st.epot) < 1e-8
+        assert np.abs(out["xyz"][sl] - ref["xyz"]).max() < 1e-8
+        assert np.abs(out["velo"][sl] - ref["velo"]).max() < 1e-10
+        assert np.abs(axyz[sl] - ref["axyz"]).max() < 1e-8
+
+
+def test_fused_md_chunks_equal_one_launch():
+    """Steps split over several persistent launches give the state of one
+    launch over all of them, and match the per-kernel path to the parity bar."""
+    trajs = ensemble(3, seed0=43)
+    a = q().MdEnsemble(trajs, max_steps=40, fused=True)
+    b = q().MdEnsemble(trajs, max_steps=40, fused=True)
+    c = q().MdEnsemble(trajs, max_steps=40, use_graph=True)
+    a.run(12)
+    for _ in range(3):
+        b.run(4)
+    c.run(12)
+    ra, rb, rc = a.read(), b.read(), c.read()
+    for m in (a, b, c):
+        m.close()
+    assert np.array_equal(ra["xyz"], rb["xyz"]) and np.array_equal(ra["velo"], rb["velo"])
+    assert [r.nstep for r in ra["results"]] == [r.nstep for r in rc["results"]] == [12, 12, 12]
+    assert np.abs(ra["xyz"] - rc["xyz"]).max() < 1e-8
+    assert max(abs(x.epot - y.epot) for x, y in zip(ra["results"], rc["results"])) < 1e-8

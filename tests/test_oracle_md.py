@@ -190,3 +190,16 @@ def test_md_averages_window_closed_form():
     xs = [x0 + k * dt * v0 + dt * dt * a * k * (k + 1) / 2 for k in range(50, 56)]
     np.testing.assert_allclose(axyz, np.mean(xs, axis=0), rtol=0, atol=1e-12)
     np.testing.assert_allclose(achrg, q, rtol=0, atol=1e-15)
+
+
+def test_impactscale_increments_are_exact():
+    """impactscale (impact.f90:12-55) adds the REAL(4) literal 0.0002 to a
+    REAL(8) scal once per iteration from 0.  The increment is a multiple of
+    2^-36, so every partial sum below 4 (20000 iterations) is exact and equal
+    to (k - 1) * 0.0002f: the property the batched search evaluates the
+    iterations independently by."""
+    c = float(np.float32(0.0002))
+    s = 0.0
+    for k in range(1, 20001):
+        assert s == (k - 1) * c
+        s = s + c
