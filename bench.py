@@ -64,11 +64,12 @@ def traffic_from_profiles(fused=False):
     return None
 
 
-def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1, fused=False):
+def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1, fused=False, eigensolver=0):
     import qcxms_amd
     trajs = workload(ntraj)
     nbf = 50
-    md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=False, lanes=lanes, fused=fused)
+    md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=False, lanes=lanes, fused=fused,
+                              eigensolver=eigensolver)
     md.run(warmup)
     md.set_profiling(True)
     names = ("fused",) if fused else ("setup", "scc", "grad", "md_pre", "md_post")
@@ -101,7 +102,8 @@ def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1, fused=False):
                scc_flops_per_launch=flops / max(scc_launches, 1))
     # the same steps replayed as one hipGraph per step (no per-kernel events)
     if use_graph_check:
-        md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True, lanes=lanes, fused=fused)
+        md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True, lanes=lanes, fused=fused,
+                                  eigensolver=eigensolver)
         md.run(warmup)
         n0 = sum(r.nstep for r in md.read()["results"])
         t0 = time.perf_counter()
@@ -279,6 +281,8 @@ def main():
                     help="only run a 256 MiB device copy (counter calibration)")
     ap.add_argument("--fused", type=int, default=0,
                     help="1: persistent kernel running whole MD iterations per trajectory")
+    ap.add_argument("--eigensolver", type=int, default=0, choices=[0, 1],
+                    help="SCC eigenvectors: 0 Jacobi, 1 MFMA simultaneous rotations then Jacobi")
     ap.add_argument("--lanes", type=int, default=1,
                     help="split the ensemble into this many groups on separate streams")
     ap.add_argument("--config", default="md256",
@@ -303,7 +307,7 @@ def main():
               "MI355X and has no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     g = run_gpu(args.ntraj, args.steps, args.warmup, use_graph_check=not args.no_graph_check,
-                lanes=args.lanes, fused=bool(args.fused))
+                lanes=args.lanes, fused=bool(args.fused), eigensolver=args.eigensolver)
     line = {
         "metric": METRIC,
         "value": g["value"],
@@ -321,6 +325,7 @@ def main():
                                "GFN2-xTB FP64, etemp 5000 K, 0.5 fs leap-frog, IEE Poisson "
                                "0.6 eV/atom (<= 1.5), tadd 100 fs, 500 K Maxwell-Boltzmann",
                    "ntraj": args.ntraj, "lanes": args.lanes, "fused": bool(args.fused),
+                   "eigensolver": "mfma+jacobi" if args.eigensolver else "jacobi",
                    "trajectory_steps": g["traj_steps"],
                    "running_after": g["running"],
                    "scc_iterations_per_trajectory_step": g["scc_iters_per_traj_step"]},

@@ -29,6 +29,7 @@ std::mutex g_mu;
 bool g_ready = false;
 int g_device = -1;
 hipStream_t g_stream = nullptr;
+size_t g_scc_dyn_max = 0;  // dynamic LDS the SCC kernels may take (set at init)
 DevTables g_host_tab;
 
 int fail(const std::string& msg)
@@ -178,13 +179,30 @@ int ensure_init(int device)
     for (int z = 0; z < 118; ++z) rad[z] = gfn2::kAatoAu * kQcxmsRadAa[z];
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_rad), rad, sizeof(rad)));
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-    HIPCHK(hipFuncSetAttribute((const void*)qx::k_scc, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (4 * qx::kFastM * (qx::kFastM + 1) + qx::kRs * qx::kFastM) * (int)sizeof(double)));
+    // dynamic LDS of the SCC kernels: whatever the workgroup's LDS leaves
+    // beside the kernels' static LDS (at least the LDS-resident matrices of
+    // kFastM basis functions)
+    int lds_block = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    hipFuncAttributes fa;
+    HIPCHK(hipFuncGetAttributes(&fa, (const void*)qx::k_scc<false>));
+    size_t lds_static = fa.sharedSizeBytes;
+    HIPCHK(hipFuncGetAttributes(&fa, (const void*)qx::k_scc<true>));
+    lds_static = std::max(lds_static, fa.sharedSizeBytes);
+    HIPCHK(hipFuncGetAttributes(&fa, (const void*)qx::k_md_fused));
+    lds_static = std::max(lds_static, fa.sharedSizeBytes);
+    const size_t base_dyn = (4 * qx::kFastM * (qx::kFastM + 1) + qx::kRs * qx::kFastM) * sizeof(double);
+    if ((size_t)lds_block < lds_static + base_dyn) return fail("the device's LDS per workgroup is too small");
+    g_scc_dyn_max = (size_t)lds_block - lds_static;
+    HIPCHK(hipFuncSetAttribute((const void*)qx::k_scc<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)g_scc_dyn_max));
+    HIPCHK(hipFuncSetAttribute((const void*)qx::k_scc<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)g_scc_dyn_max));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_setup, hipFuncAttributeMaxDynamicSharedMemorySize,
                                QX_LDS_MAXM * QX_LDS_MAXM * (int)sizeof(double)));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_grad, hipFuncAttributeMaxDynamicSharedMemorySize, 0));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_md_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (4 * qx::kFastM * (qx::kFastM + 1) + qx::kRs * qx::kFastM) * (int)sizeof(double)));
+                               (int)g_scc_dyn_max));
     g_device = device;
     g_ready = true;
     return QCXMS_OK;
@@ -205,7 +223,7 @@ struct qcxms_batch {
     int natoms = 0, nshells = 0, nrefs = 0, nxtot = 0;
     size_t mat_total = 0, gam_total = 0, d4_total = 0, hist_total = 0, pair_total = 0;
     int max_ld = 0, max_m = 0;
-    SccOpts opts{1e-9, 1e-8, 0.4, 1e-11, 100, 100, 0, 0};
+    SccOpts opts{1e-9, 1e-8, 0.4, 1e-11, 100, 100, 0, 0, 0, 0};
     int warm_start = 0;
     int profiling = 0;
     Timer t_setup, t_scc, t_grad, t_md_pre, t_md_post, t_fused;
@@ -468,7 +486,19 @@ size_t lds_scc_bytes(const qcxms_batch* b)
         const size_t m = (size_t)((si.nao + 1) & ~1);
         if (m <= (size_t)qx::kFastM) lm = std::max(lm, m);
     }
-    return sizeof(double) * (4 * lm * (lm + 1) + qx::kRs * lm);
+    const size_t base = sizeof(double) * (4 * lm * (lm + 1) + qx::kRs * lm);
+    // a fifth matrix for the MFMA eigen refinement when it is on and the LDS
+    // holds it
+    const size_t ext = base + sizeof(double) * lm * (lm + 1);
+    return b->opts.eig_mfma && ext <= g_scc_dyn_max ? ext : base;
+}
+
+// the SCC options of one launch with its dynamic LDS size
+SccOpts launch_opts(const qcxms_batch* b, size_t lds)
+{
+    SccOpts o = b->opts;
+    o.lds_doubles = (int)(lds / sizeof(double));
+    return o;
 }
 
 // setup -> SCC -> gradient for every system of the batch (masked by d.active)
@@ -481,7 +511,8 @@ int enqueue_egrad(qcxms_batch* b, bool with_grad)
          })))
         return st;
     if ((st = timed(b, b->t_scc, [&] {
-             hipLaunchKernelGGL(qx::k_scc, dim3(b->nsys), dim3(QX_SCC_BLOCK), l2, b->stream, b->d, b->opts);
+             hipLaunchKernelGGL(b->opts.eig_mfma ? qx::k_scc<true> : qx::k_scc<false>, dim3(b->nsys),
+                                dim3(QX_SCC_BLOCK), l2, b->stream, b->d, launch_opts(b, l2));
          })))
         return st;
     if (with_grad &&
@@ -612,7 +643,7 @@ int qcxms_debug_phase_read(unsigned long long* out, int nsys, int reset)
 {
     if (!g_ready) return fail("library not initialised");
     const int n = std::min(nsys, qx::kPhaseSys) * qx::kPhaseSlots;
-    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpyFromSymbol(out, HIP_SYMBOL(qx::g_phase), sizeof(unsigned long long) * n));
     if (reset) {
         std::vector<unsigned long long> z(qx::kPhaseSys * qx::kPhaseSlots, 0ull);
@@ -921,6 +952,17 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
     md->ntraj = ntraj;
     md->use_graph = params->use_graph != 0;
     md->fused = params->fused != 0;
+    if (params->eigensolver < 0 || params->eigensolver > 1) {
+        delete md;
+        g_err = "eigensolver must be 0 (Jacobi) or 1 (MFMA rotations with Jacobi)";
+        return nullptr;
+    }
+    if (params->eigensolver == 1 && md->fused) {
+        delete md;
+        g_err = "the persistent MD kernel (fused) runs the Jacobi eigensolver only";
+        return nullptr;
+    }
+    b->opts.eig_mfma = params->eigensolver;
     md->p.tstep = params->tstep;
     md->p.etempin = params->etempin;
     md->p.ax = params->ax;
@@ -1054,7 +1096,7 @@ int md_enqueue_fused(qcxms_md* md, int nsteps)
     const size_t lds = std::max(lds_setup_bytes(b), lds_scc_bytes(b));
     return timed(b, b->t_fused, [&] {
         hipLaunchKernelGGL(qx::k_md_fused, dim3(md->ntraj), dim3(QX_SCC_BLOCK), lds, b->stream, b->d, md->m,
-                           md->p, b->opts, nsteps);
+                           md->p, launch_opts(b, lds), nsteps);
     });
 }
 

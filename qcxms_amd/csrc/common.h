@@ -67,6 +67,8 @@ struct SccOpts {
                       // 3 cubic 4 (x(n-1) + x(n-3)) - 6 x(n-2) - x(n-4),
                       // 4 quartic 5 (x(n-1) - x(n-4)) - 10 (x(n-2) - x(n-3)) + x(n-5)
     int keep;         // > 0: Broyden secant pairs kept across warm-started calls (ring of keep)
+    int lds_doubles;  // dynamic LDS of the launch in doubles (set per launch by the host)
+    int eig_mfma;     // 1: MFMA simultaneous-rotation eigen refinement before Jacobi
 };
 
 // Device-resident batch state.  Every pointer is a flat array over all systems.

@@ -489,7 +489,7 @@ __global__ __launch_bounds__(QX_SCC_BLOCK) void k_md_fused(BatchPtrs B, MdPtrs M
         if (!M.active[sys]) break;
         setup_system(B, sys, 1, L.s, dyn);
         __syncthreads();
-        scc_system(B, opt, sys, L.c, dyn);
+        scc_system<false>(B, opt, sys, L.c, dyn);
         __syncthreads();
         grad_system(B, sys, L.g);
         __syncthreads();

@@ -97,21 +97,45 @@ QX_DEV void mfma_gemm(int M, int N, int K, const double* A, int lda, const doubl
     for (int t = wave; t < tm * tc; t += kNT / 64) {
         const int i0 = (t / tc) * 16, jc = (t % tc) * 16 * NU;
         const int i = i0 + li;
+        const bool iin = i < M;
+        const int ic = iin ? i : M - 1;
+        int jcl[NU];
+        bool jin[NU];
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int j = jc + 16 * u + li;
+            jin[u] = j < N;
+            jcl[u] = jin[u] ? j : N - 1;
+        }
         qx_d4 acc[NU];
 #pragma unroll
         for (int u = 0; u < NU; ++u) acc[u] = qx_d4{0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < K; k0 += 4) {
-            const int k = k0 + lk;
-            const bool kin = k < K;
-            double a = 0.0;
-            if (i < M && kin) a = (TA ? A[(size_t)k * lda + i] : A[(size_t)i * lda + k]) * (d ? d[k] : 1.0);
+        // four k-steps per pass: all operand loads first, then the MFMAs
+        // (k-steps past K multiply zeros).  Loads are clamped into the
+        // matrices and out-of-range lanes zeroed afterwards, so no branch
+        // surrounds a load.
+        constexpr int KS = 4;
+        for (int k0 = 0; k0 < K; k0 += 4 * KS) {
+            double a[KS], b[KS][NU];
 #pragma unroll
-            for (int u = 0; u < NU; ++u) {
-                const int j = jc + 16 * u + li;
-                double b = 0.0;
-                if (j < N && kin) b = TB ? Bm[(size_t)j * ldb + k] : Bm[(size_t)k * ldb + j];
-                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
+            for (int q = 0; q < KS; ++q) {
+                const int k = k0 + 4 * q + lk;
+                const bool kin = k < K;
+                const int kc = kin ? k : K - 1;
+                double av = TA ? A[(size_t)kc * lda + ic] : A[(size_t)ic * lda + kc];
+                if (d) av *= d[kc];
+                a[q] = (iin && kin) ? av : 0.0;
+#pragma unroll
+                for (int u = 0; u < NU; ++u) {
+                    const double bv = TB ? Bm[(size_t)jcl[u] * ldb + kc] : Bm[(size_t)kc * ldb + jcl[u]];
+                    b[q][u] = (jin[u] && kin) ? bv : 0.0;
+                }
             }
+#pragma unroll
+            for (int q = 0; q < KS; ++q)
+#pragma unroll
+                for (int u = 0; u < NU; ++u)
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], b[q][u], acc[u], 0, 0, 0);
         }
 #pragma unroll
         for (int u = 0; u < NU; ++u)
@@ -377,6 +401,86 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
     return sweep;
 }
 
+// Simultaneous-rotation refinement of a nearly diagonal symmetric A (m x m,
+// leading dimension lda) by FP64 MFMA products.  Every pair (i, j) gets the
+// angle of the Jacobi rotation that would zero a_ij alone; with K the
+// antisymmetric matrix of the angles, Y = I + K (+ K^2 / 2 when max |K| >
+// 1e-4) is orthogonal up to O(K^2) (O(K^4)), and one or two Newton-Schulz
+// steps R <- R (3 I - R^T R) / 2 make it orthogonal to rounding.  Then
+// A <- R^T A R and Cv <- Cv R (nrow x m).  The off-diagonal decreases
+// quadratically (errors of the simultaneous angles are second order in the
+// off-diagonal over the eigenvalue gaps).  Returns true when the largest
+// off-diagonal magnitude is below tol after at most maxit rounds; false,
+// possibly after some rounds, when an angle exceeds 0.1 (near-degenerate
+// pairs with a relatively large coupling: the series would need more terms)
+// or the rounds did not suffice, and the caller finishes with Jacobi.  Y, Z,
+// T: m x m scratch (leading dimension lda).  Called by all threads.
+QX_DEV bool eig_rotate(double* A, double* Cv, int m, int nrow, int lda, double tol, int maxit, double* Y,
+                       double* Z, double* T, double* red)
+{
+    const int tid = threadIdx.x;
+    if (offdiag_max(A, m, lda, red) < tol) return true;
+    for (int it = 0; it < maxit; ++it) {
+        double kmax = 0.0;
+        for (int p = tid; p < m * m; p += kNT) {
+            const int i = p / m, j = p - i * m;
+            double k = 0.0;
+            if (i != j) {
+                const int lo = i < j ? i : j, hi = i < j ? j : i;
+                const double a = A[(size_t)lo * lda + hi];
+                const double d = A[(size_t)hi * lda + hi] - A[(size_t)lo * lda + lo];
+                const double r = sqrt(d * d + 4.0 * a * a);
+                const double t = r > 0.0 ? 2.0 * (d >= 0.0 ? a : -a) / (fabs(d) + r) : 0.0;
+                const double th = atan(t);
+                k = i < j ? th : -th;
+            }
+            Y[(size_t)i * lda + j] = k;
+            kmax = fmax(kmax, fabs(k));
+        }
+        kmax = block_max(kmax, red);
+        if (kmax > 0.1) return false;
+        if (kmax > 1e-4) {
+            mfma_gemm<false, false>(m, m, m, Y, lda, Y, lda, Z, lda, nullptr);
+            __syncthreads();
+        }
+        for (int p = tid; p < m * m; p += kNT) {
+            const int i = p / m, j = p - i * m;
+            const size_t o = (size_t)i * lda + j;
+            Y[o] = (i == j ? 1.0 : 0.0) + Y[o] + (kmax > 1e-4 ? 0.5 * Z[o] : 0.0);
+        }
+        __syncthreads();
+        double* R = Y;
+        const int nns = kmax > 0.005 ? 2 : 1;
+        for (int s = 0; s < nns; ++s) {
+            mfma_gemm<true, false>(m, m, m, R, lda, R, lda, Z, lda, nullptr);
+            __syncthreads();
+            for (int p = tid; p < m * m; p += kNT) {
+                const int i = p / m, j = p - i * m;
+                const size_t o = (size_t)i * lda + j;
+                Z[o] = (i == j ? 1.5 : 0.0) - 0.5 * Z[o];
+            }
+            __syncthreads();
+            double* out = R == Y ? T : Y;
+            mfma_gemm<false, false>(m, m, m, R, lda, Z, lda, out, lda, nullptr);
+            __syncthreads();
+            R = out;
+        }
+        double* W = R == Y ? T : Y;
+        mfma_gemm<false, false>(m, m, m, A, lda, R, lda, W, lda, nullptr);
+        __syncthreads();
+        mfma_gemm<true, false>(m, m, m, R, lda, W, lda, A, lda, nullptr);
+        mfma_gemm<false, false>(nrow, m, m, Cv, lda, R, lda, Z, lda, nullptr);
+        __syncthreads();
+        for (int p = tid; p < nrow * m; p += kNT) {
+            const int i = p / m, j = p - i * m;
+            Cv[(size_t)i * lda + j] = Z[(size_t)i * lda + j];
+        }
+        __syncthreads();
+        if (offdiag_max(A, m, lda, red) < tol) return true;
+    }
+    return false;
+}
+
 // Fermi occupations per spin on wavefront 0 for nalpha electrons; returns -TS.
 // The chemical potential is found by Newton steps safeguarded by a bracket
 // (bisection whenever a step leaves it), starting from mu_io when it lies
@@ -600,9 +704,10 @@ struct SccBuffers {
     double *A, *C, *F, *P;  // m x m working matrices (LDS or global), leading dimension ld
     double* rs;             // kRs row sums per AO
     int ld;
+    double* E;              // fifth m x m matrix for the MFMA eigen refinement, or null
 };
 
-template <bool FAST>
+template <bool FAST, bool EIG>
 QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, SysShared& ss,
                      SccBuffers bf, double* xin, double* xout, double* vs, double* vd, double* vq,
                      double* eps, double* focc, double* f2, double2* rot, int2* rpq, double* s_e,
@@ -730,7 +835,12 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         QX_PH(2);
         double jtol = fmin(1e-6, 1e-3 * errp);
         if (jtol < opt.jacobi_tol) jtol = opt.jacobi_tol;
-        const int nsweep = jacobi(A, Cc, m, n, lb, jtol, 50, rot, rpq, ss.red, s_any);
+        int nsweep = 0;
+        if (EIG && bf.E && eig_rotate(A, Cc, m, n, lb, jtol, 3, Ft, P, bf.E, ss.red)) {
+            QX_PH_ADD(15, 1);
+        } else {
+            nsweep = jacobi(A, Cc, m, n, lb, jtol, 50, rot, rpq, ss.red, s_any);
+        }
         QX_PH(3);
         QX_PH_ADD(12, nsweep);
         (void)nsweep;
@@ -754,14 +864,41 @@ QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, 
         mfma_gemm<false, true>(n, n, n, Cc, lb, Cc, lb, P, lb, f2);
         __syncthreads();
         QX_PH(5);
-        // row sums of P o M for M = S, D (3), Q (6), H0 as coalesced column sweeps
-        for (int t = tid; t < kRs * n; t += kNT) {
-            const int c = t / n, mu = t - c * n;
-            const double* M = c == 0 ? S : c < 4 ? Dm + (size_t)(c - 1) * ms : c < 10 ? Qm + (size_t)(c - 4) * ms : H;
-            double acc = 0.0;
+        // row sums of P o M for M = S, D (3), Q (6), H0 as coalesced column
+        // sweeps: wavefront w takes the matrices w, w + 4, w + 8 for all AOs,
+        // so every lane keeps three independent sums (in nu order) and three
+        // times the loads in flight of one sum per thread
+        {
+            static_assert(kNT == 256 && kRs <= 12, "three row-sum matrices per wavefront");
+            const int c0 = wave, c1 = wave + 4, c2 = wave + 8;
+            const bool has2 = c2 < kRs;
+            const double* M0 = c0 == 0 ? S : Dm + (size_t)(c0 - 1) * ms;
+            const double* M1 = Qm + (size_t)(c1 - 4) * ms;
+            const double* M2 = !has2 ? M1 : c2 < 10 ? Qm + (size_t)(c2 - 4) * ms : H;
+            for (int mu = lane; mu < n; mu += 64) {
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+                if (has2) {
 #pragma unroll 8
-            for (int nu = 0; nu < n; ++nu) acc += P[(size_t)nu * lb + mu] * M[(size_t)nu * ld + mu];
-            rs[(size_t)mu * kRs + c] = acc;
+                    for (int nu = 0; nu < n; ++nu) {
+                        const double p = P[(size_t)nu * lb + mu];
+                        const size_t o = (size_t)nu * ld + mu;
+                        a0 += p * M0[o];
+                        a1 += p * M1[o];
+                        a2 += p * M2[o];
+                    }
+                } else {
+#pragma unroll 8
+                    for (int nu = 0; nu < n; ++nu) {
+                        const double p = P[(size_t)nu * lb + mu];
+                        const size_t o = (size_t)nu * ld + mu;
+                        a0 += p * M0[o];
+                        a1 += p * M1[o];
+                    }
+                }
+                rs[(size_t)mu * kRs + c0] = a0;
+                rs[(size_t)mu * kRs + c1] = a1;
+                if (has2) rs[(size_t)mu * kRs + c2] = a2;
+            }
         }
         __syncthreads();
         for (int s = tid; s < nsh; s += kNT) {
@@ -977,6 +1114,10 @@ struct SccLds {
 };
 
 // the whole SCC of system sys by the block (dyn: the launch's dynamic LDS)
+//   EIG: with the MFMA eigen refinement (a separate instantiation: the code
+//   alone, even when not taken, costs the Jacobi-only kernel a quarter of
+//   its speed through the scheduling of the shared SCC body)
+template <bool EIG>
 QX_DEV void scc_system(const BatchPtrs& B, const SccOpts& opt, int sys, SccLds& L, double* dyn)
 {
     const SysInfo si = B.info[sys];
@@ -985,25 +1126,29 @@ QX_DEV void scc_system(const BatchPtrs& B, const SccOpts& opt, int sys, SccLds& 
     if (m <= kFastM) {
         // odd leading dimension: column-wise LDS accesses hit distinct banks
         const size_t mm = (size_t)m * (m + 1);
-        SccBuffers bf{dyn, dyn + mm, dyn + 2 * mm, dyn + 3 * mm, dyn + 4 * mm, m + 1};
-        scc_body<true>(si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc, L.f2,
+        // a fifth matrix after the row sums when the launch's LDS holds it
+        double* E = EIG && 5 * mm + kRs * (size_t)m <= (size_t)opt.lds_doubles ? dyn + 4 * mm + kRs * m
+                                                                            : nullptr;
+        SccBuffers bf{dyn, dyn + mm, dyn + 2 * mm, dyn + 3 * mm, dyn + 4 * mm, m + 1, E};
+        scc_body<true, EIG>(si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc, L.f2,
                        L.rot, L.rpq, L.s_e, L.s_any);
     } else {
         // the energy-weighted density slot doubles as the row-sum scratch
         SccBuffers bf{B.T + si.mat_off, B.C + si.mat_off, B.U + si.mat_off, B.P + si.mat_off,
-                      B.W + si.mat_off, si.ld};
-        scc_body<false>(si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc,
+                      B.W + si.mat_off, si.ld, nullptr};
+        scc_body<false, false>(si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc,
                         L.f2, L.rot, L.rpq, L.s_e, L.s_any);
     }
 }
 
+template <bool EIG>
 __global__ __launch_bounds__(kNT) void k_scc(BatchPtrs B, SccOpts opt)
 {
     const int sys = blockIdx.x;
     if (B.active && !B.active[sys]) return;
     __shared__ SccLds L;
     extern __shared__ double dyn[];
-    scc_system(B, opt, sys, L, dyn);
+    scc_system<EIG>(B, opt, sys, L, dyn);
 }
 
 }  // namespace qx

@@ -218,7 +218,8 @@ class MdParams(C.Structure):
     _fields_ = [("tstep", C.c_double), ("etempin", C.c_double), ("ax", C.c_double),
                 ("ieetemp", C.c_double), ("nfragexit", C.c_int32), ("max_steps", C.c_int32),
                 ("use_graph", C.c_int32), ("extrapolate", C.c_int32), ("econv", C.c_double),
-                ("pconv", C.c_double), ("broyden_keep", C.c_int32), ("fused", C.c_int32)]
+                ("pconv", C.c_double), ("broyden_keep", C.c_int32), ("fused", C.c_int32),
+                ("eigensolver", C.c_int32)]
 
 
 class MdResult(C.Structure):
@@ -261,11 +262,14 @@ class MdEnsemble:
     velof.  lanes > 1 splits the trajectories into that many contiguous
     groups, each an ensemble on its own stream, advanced together
     (qcxms_md_run_group) so that the device overlaps their kernels.  fused
-    runs whole MD iterations per trajectory in one persistent launch."""
+    runs whole MD iterations per trajectory in one persistent launch.
+    eigensolver 1 refines the SCC eigenvectors by simultaneous rotations on
+    the FP64 matrix cores before Jacobi (0: Jacobi only, the default)."""
 
     def __init__(self, trajectories, tstep=0.5 * 41.3413733365614, max_steps=2000,
                  nfragexit=3, etempin=-1.0, ax=0.0, ieetemp=0.0, use_graph=True,
-                 econv=0.0, pconv=0.0, extrapolate=0, broyden_keep=0, lanes=1, fused=False):
+                 econv=0.0, pconv=0.0, extrapolate=0, broyden_keep=0, lanes=1, fused=False,
+                 eigensolver=0):
         L = library()
         _md_sigs(L)
         nums = [np.asarray(t["num"], dtype=np.int32) for t in trajectories]
@@ -277,7 +281,8 @@ class MdEnsemble:
         bounds = [round(k * self.ntraj / lanes) for k in range(lanes + 1)]
         self._lanes = [(bounds[k], bounds[k + 1]) for k in range(lanes)]
         p = MdParams(tstep, etempin, ax, ieetemp, nfragexit, max_steps, int(bool(use_graph)),
-                     int(extrapolate), econv, pconv, int(broyden_keep), int(bool(fused)))
+                     int(extrapolate), econv, pconv, int(broyden_keep), int(bool(fused)),
+                     int(eigensolver))
         self._hs = []
         for t0, t1 in self._lanes:
             part = trajectories[t0:t1]

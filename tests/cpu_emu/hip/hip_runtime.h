@@ -292,6 +292,24 @@ inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int v)
 {
     return v <= (int)qx_emu::kMaxLds ? hipSuccess : hipErrorEmu;
 }
+enum hipDeviceAttribute_t { hipDeviceAttributeMaxSharedMemoryPerBlock = 1 };
+inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int)
+{
+    *v = (int)qx_emu::kMaxLds;
+    return hipSuccess;
+}
+// Static LDS per kernel: the emulation keeps every kernel's __shared__
+// objects in one section, so it reports the MI355X build's figure for the
+// largest kernels (46,480 bytes for the SCC and persistent MD kernels),
+// rounded up.
+struct hipFuncAttributes {
+    size_t sharedSizeBytes;
+};
+inline hipError_t hipFuncGetAttributes(hipFuncAttributes* a, const void*)
+{
+    a->sharedSizeBytes = 47 * 1024;
+    return hipSuccess;
+}
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned)
 {
     *s = new qx_emu_stream;

@@ -223,3 +223,31 @@ def test_fused_md_chunks_equal_one_launch():
     assert [r.nstep for r in ra["results"]] == [r.nstep for r in rc["results"]] == [12, 12, 12]
     assert np.abs(ra["xyz"] - rc["xyz"]).max() < 1e-8
     assert max(abs(x.epot - y.epot) for x, y in zip(ra["results"], rc["results"])) < 1e-8
+
+
+def test_mfma_eigensolver_md_matches_oracle_md():
+    """MD with the SCC eigenvectors refined by simultaneous rotations on the
+    FP64 matrix cores (eigensolver 1, Jacobi where an angle exceeds 0.1)
+    against the oracle MD over 30 heating steps; an unknown eigensolver is
+    rejected."""
+    trajs = ensemble(4, seed0=57)
+    nsteps = 30
+    with pytest.raises(RuntimeError):
+        q().MdEnsemble(trajs, max_steps=nsteps, eigensolver=2)
+    md = q().MdEnsemble(trajs, max_steps=nsteps, eigensolver=1, use_graph=False, econv=1e-11, pconv=1e-9)
+    running = md.run(nsteps + 5)
+    out = md.read()
+    md.close()
+    assert running == 0
+    opts = ob.default_opts(econv=1e-12, pconv=1e-10)
+    for t, tr in enumerate(trajs):
+        ref = ob.md_run(tr, steps_limit=0, max_steps=nsteps, opts=opts)
+        st, res = ref["state"], out["results"][t]
+        sl = slice(md.offsets[t], md.offsets[t + 1])
+        print(f"mfma-eigen traj {t}: nstep {res.nstep}/{st.nstep} dEpot {abs(res.epot - st.epot):.2e} "
+              f"dx {np.abs(out['xyz'][sl] - ref['xyz']).max():.2e}", flush=True)
+        assert res.nstep == st.nstep and res.exit_reason == st.exit_reason
+        assert np.array_equal(out["list"][sl], ref["list"])
+        assert abs(res.epot - st.epot) < 1e-8
+        assert np.abs(out["xyz"][sl] - ref["xyz"]).max() < 1e-8
+        assert np.abs(out["velo"][sl] - ref["velo"]).max() < 1e-10
