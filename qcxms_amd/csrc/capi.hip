@@ -199,7 +199,8 @@ int ensure_init(int device)
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_scc<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)g_scc_dyn_max));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_setup, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               QX_LDS_MAXM * QX_LDS_MAXM * (int)sizeof(double)));
+                               (QX_LDS_MAXM * QX_LDS_MAXM + QX_LDS_MAXM * (QX_LDS_MAXM | 1)) *
+                                   (int)sizeof(double)));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_grad, hipFuncAttributeMaxDynamicSharedMemorySize, 0));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_md_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)g_scc_dyn_max));
@@ -474,8 +475,9 @@ int collect_times()
 
 size_t lds_setup_bytes(const qcxms_batch* b)
 {
+    // Cholesky factor and the columns of its inverse
     const size_t ls = std::min(b->max_ld, QX_LDS_MAXM);
-    return sizeof(double) * ls * ls;
+    return sizeof(double) * (ls * ls + ls * (ls | 1));
 }
 // four m x m working matrices and the AO row sums for the systems that keep
 // them in LDS

@@ -150,7 +150,6 @@ struct SetupLds {
     double scn[QX_MAX_NAT], scnd4[QX_MAX_NAT];
     int refoff[QX_MAX_NAT + 1];
     double red[QX_BLOCK / 64];
-    double bcast;
 };
 
 QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& Ls, double* dyn)
@@ -167,7 +166,6 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
     double* scnd4 = Ls.scnd4;
     int* refoff = Ls.refoff;
     double* red = Ls.red;
-    double& bcast = Ls.bcast;
 
     const double* gx = B.xyz + 3 * (size_t)si.atom_off;
     for (int i = tid; i < 3 * nat; i += QX_BLOCK) sx[i] = gx[i];
@@ -429,15 +427,12 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
     for (size_t i = tid; i < ms; i += QX_BLOCK) A[i] = S[i];
     __syncthreads();
     for (int j = 0; j < n; ++j) {
-        if (tid == 0) {
-            const double d = sqrt(A[(size_t)j * ld + j]);
-            A[(size_t)j * ld + j] = d;
-            bcast = d;
-        }
-        __syncthreads();
-        const double d = bcast;
+        // every thread takes the pivot's root itself; thread 0 stores it
+        // while the others update the trailing block (two barriers a column)
+        const double d = sqrt(A[(size_t)j * ld + j]);
         for (int i = j + 1 + tid; i < n; i += QX_BLOCK) A[(size_t)i * ld + j] /= d;
         __syncthreads();
+        if (tid == 0) A[(size_t)j * ld + j] = d;
         const int m = n - j - 1;
         for (int p = tid; p < m * m; p += QX_BLOCK) {
             const int i = j + 1 + p / m, k = j + 1 + p % m;
@@ -453,15 +448,29 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
         Lg[p] = v;
     }
     __syncthreads();
-    // X = (L^-1)^T: column c of L^-1 by forward substitution, written as row c of X
-    for (size_t p = tid; p < ms; p += QX_BLOCK) Xg[p] = 0.0;
-    __syncthreads();
+    // X = (L^-1)^T: column c of L^-1 by forward substitution (thread c),
+    // written as row c of X.  With the factor in LDS the columns are built in
+    // LDS too (rows of odd stride) and copied out once.
+    const bool lx = ld <= QX_LDS_MAXM;
+    const int xs = n | 1;
+    double* xl = dyn + (size_t)ld * ld;
+    if (!lx) {
+        for (size_t p = tid; p < ms; p += QX_BLOCK) Xg[p] = 0.0;
+        __syncthreads();
+    }
     for (int c = tid; c < n; c += QX_BLOCK) {
-        double* xr = Xg + (size_t)c * ld;  // X[c][i] = Linv[i][c]
+        double* xr = lx ? xl + (size_t)c * xs : Xg + (size_t)c * ld;  // X[c][i] = Linv[i][c]
         for (int i = c; i < n; ++i) {
             double v = (i == c) ? 1.0 : 0.0;
             for (int k = c; k < i; ++k) v -= A[(size_t)i * ld + k] * xr[k];
             xr[i] = v / A[(size_t)i * ld + i];
+        }
+    }
+    if (lx) {
+        __syncthreads();
+        for (size_t p = tid; p < ms; p += QX_BLOCK) {
+            const int c = (int)(p / ld), i = (int)(p % ld);
+            Xg[p] = (c < n && i >= c && i < n) ? xl[(size_t)c * xs + i] : 0.0;
         }
     }
     // X as stored: row c holds column c of L^-1, i.e. Xg = (L^-1)^T row-major
