@@ -150,6 +150,9 @@ struct SetupLds {
     double scn[QX_MAX_NAT], scnd4[QX_MAX_NAT];
     int refoff[QX_MAX_NAT + 1];
     double red[QX_BLOCK / 64];
+    // per D4 reference: atom, index within the atom, zeta(q = 0) * gw
+    int refat[QX_MAX_REF * QX_MAX_NAT], refii[QX_MAX_REF * QX_MAX_NAT];
+    double zg[QX_MAX_REF * QX_MAX_NAT];
 };
 
 QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& Ls, double* dyn)
@@ -300,11 +303,26 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
         const double eta = 0.5 * (g_tab.e[sh[a].el].hard[sh[a].k] + g_tab.e[sh[b].el].hard[sh[b].k]);
         G[p] = 1.0 / sqrt(dx * dx + dy * dy + dz * dz + 1.0 / (eta * eta));
     }
-    // 6. D4 charge-response matrix wd4[k][l] = gw_k gw_l C6ref_kl disp_ij (i != j)
+    // 6. D4 charge-response matrix wd4[k][l] = gw_k gw_l C6ref_kl disp_ij (i != j):
+    //    the pair damping disp_ij into LDS (the Cholesky slot is free here),
+    //    then one element per thread, coalesced along l
     const int nd = si.ndim;
     double* WD = B.wd4 + si.d4_off;
     const double* gw = B.gw + si.ref_off;
+    double* disp = dyn;  // nat x nat
+    int* refat = Ls.refat;
+    int* refii = Ls.refii;
+    double* zg = Ls.zg;
     __syncthreads();
+    for (int k = tid; k < nd; k += QX_BLOCK) {
+        int a = 0;
+        while (refoff[a + 1] <= k) ++a;
+        const ElemParams& e = g_tab.e[sel[a]];
+        const int r = k - refoff[a];
+        refat[k] = a;
+        refii[k] = r;
+        zg[k] = d4_zeta(gfn2::kD4Ga, e.gam * gfn2::kD4Gc, e.refq[r] + e.zeff, e.zeff) * gw[k];
+    }
     for (int p = tid; p < nat * nat; p += QX_BLOCK) {
         const int i = p / nat, j = p % nat;
         const ElemParams& ei = g_tab.e[sel[i]];
@@ -312,13 +330,13 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
         const double dx = sx[3 * i] - sx[3 * j], dy = sx[3 * i + 1] - sx[3 * j + 1],
                      dz = sx[3 * i + 2] - sx[3 * j + 2];
         const double r2 = dx * dx + dy * dy + dz * dz;
-        const double disp = (i != j && r2 <= gfn2::kD4RThr) ? d4_disp(ei, ej, r2, nullptr) : 0.0;
-        for (int ii = 0; ii < ei.nref; ++ii)
-            for (int jj = 0; jj < ej.nref; ++jj) {
-                const int k = refoff[i] + ii, l = refoff[j] + jj;
-                WD[(size_t)k * nd + l] =
-                    disp * gw[k] * gw[l] * g_tab.c6ref[sel[i]][ii][sel[j]][jj];
-            }
+        disp[p] = (i != j && r2 <= gfn2::kD4RThr) ? d4_disp(ei, ej, r2, nullptr) : 0.0;
+    }
+    __syncthreads();
+    for (int p = tid; p < nd * nd; p += QX_BLOCK) {
+        const int k = p / nd, l = p - k * nd;
+        const int i = refat[k], j = refat[l];
+        WD[p] = disp[i * nat + j] * gw[k] * gw[l] * g_tab.c6ref[sel[i]][refii[k]][sel[j]][refii[l]];
     }
     // 6b. damped multipole kernels (AES2) per ordered pair (i, j), rv = R_i - R_j,
     //     stored as planes c * nat^2 + j * nat + i:
@@ -374,13 +392,8 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
         const ElemParams& ej = g_tab.e[sel[j]];
         double c = 0.0;
         for (int ii = 0; ii < ei.nref; ++ii) {
-            const double zi = d4_zeta(gfn2::kD4Ga, ei.gam * gfn2::kD4Gc, ei.refq[ii] + ei.zeff, ei.zeff) *
-                              gw[refoff[i] + ii];
-            for (int jj = 0; jj < ej.nref; ++jj) {
-                const double zj = d4_zeta(gfn2::kD4Ga, ej.gam * gfn2::kD4Gc, ej.refq[jj] + ej.zeff,
-                                          ej.zeff) * gw[refoff[j] + jj];
-                c += zi * zj * g_tab.c6ref[sel[i]][ii][sel[j]][jj];
-            }
+            const double zi = zg[refoff[i] + ii];
+            for (int jj = 0; jj < ej.nref; ++jj) c += zi * zg[refoff[j] + jj] * g_tab.c6ref[sel[i]][ii][sel[j]][jj];
         }
         c60[p] = c;
     }
