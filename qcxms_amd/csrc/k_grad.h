@@ -180,9 +180,16 @@ QX_DEV void grad_system(const BatchPtrs& B, int sys, GradLds& Lg)
         o[4] = psum * scal * (-eb.kcn[sb.k]);
     }
 
-    // Phase B: explicit pair terms, contribution to atom a from ordered pair (a, b)
-    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
-        const int a = p / nat, b = p % nat;
+    // Phase B: explicit pair terms, contribution to atom a from ordered pair
+    // (a, b).  The three-body loop runs over c > b, so pairs are taken in
+    // order of b (heaviest first) and dealt out in snake order over the
+    // rounds of the block.
+    const int npair = nat * nat;
+    for (int r0 = 0; r0 < npair; r0 += QX_BLOCK) {
+        const int rank = r0 + (((r0 / QX_BLOCK) & 1) ? QX_BLOCK - 1 - tid : tid);
+        if (rank >= npair) continue;
+        const int b = rank / nat, a = rank - b * nat;
+        const int p = a * nat + b;
         double* o = pb + (size_t)p * kPairStride;
         if (a == b) {
             o[5] = o[6] = o[7] = o[8] = o[9] = 0.0;
@@ -317,43 +324,62 @@ QX_DEV void grad_system(const BatchPtrs& B, int sys, GradLds& Lg)
         o[9] = dcn4_a;
     }
     __syncthreads();
-    // Phase C: per-atom reduction of forces and CN derivatives
-    for (int a = tid; a < nat; a += QX_BLOCK) {
-        double g[3] = {0, 0, 0}, dc = 0.0, dc4 = 0.0;
-        for (int b = 0; b < nat; ++b) {
-            if (b == a) continue;
-            const double* o = pb + (size_t)(a * nat + b) * kPairStride;
-            for (int c = 0; c < 3; ++c) g[c] += o[5 + c];
-            dc += o[8];
-            dc4 += o[9];
-            if (b < a) {
-                for (int ia = ss.shoff[a]; ia < ss.shoff[a + 1]; ++ia)
-                    for (int ib = ss.shoff[b]; ib < ss.shoff[b + 1]; ++ib) {
-                        const double* sp = spb + (size_t)(ia * nsh + ib) * 5;
-                        for (int c = 0; c < 3; ++c) g[c] += sp[c];
-                        dc += sp[3];
+    // Phase C: per-atom reduction of forces and CN derivatives: a wavefront
+    // per atom, a lane per partner, tree sums over the lanes
+    {
+        const int lane = tid & 63;
+        for (int a = tid >> 6; a < nat; a += QX_BLOCK / 64) {
+            double g0 = 0.0, g1 = 0.0, g2 = 0.0, dc = 0.0, dc4 = 0.0;
+            const int b = lane;
+            if (b < nat && b != a) {
+                const double* o = pb + (size_t)(a * nat + b) * kPairStride;
+                g0 = o[5];
+                g1 = o[6];
+                g2 = o[7];
+                dc = o[8];
+                dc4 = o[9];
+                if (b < a) {
+                    for (int ia = ss.shoff[a]; ia < ss.shoff[a + 1]; ++ia)
+                        for (int ib = ss.shoff[b]; ib < ss.shoff[b + 1]; ++ib) {
+                            const double* sp = spb + (size_t)(ia * nsh + ib) * 5;
+                            g0 += sp[0];
+                            g1 += sp[1];
+                            g2 += sp[2];
+                            dc += sp[3];
+                        }
+                } else {
+                    for (int ia = ss.shoff[b]; ia < ss.shoff[b + 1]; ++ia)
+                        for (int ib = ss.shoff[a]; ib < ss.shoff[a + 1]; ++ib) {
+                            const double* sp = spb + (size_t)(ia * nsh + ib) * 5;
+                            g0 -= sp[0];
+                            g1 -= sp[1];
+                            g2 -= sp[2];
+                            dc += sp[4];
+                        }
+                }
+            }
+            g0 = wave_sum(g0);
+            g1 = wave_sum(g1);
+            g2 = wave_sum(g2);
+            dc = wave_sum(dc);
+            dc4 = wave_sum(dc4);
+            if (lane == 0) {
+                // on-site self-energies
+                for (int s = ss.shoff[a]; s < ss.shoff[a + 1]; ++s) {
+                    double pd = 0.0;
+                    for (int k = 0; k < ss.sh[s].nao; ++k) {
+                        const int mu = ss.sh[s].ao0 + k;
+                        pd += P[(size_t)mu * ld + mu];
                     }
-            } else {
-                for (int ia = ss.shoff[b]; ia < ss.shoff[b + 1]; ++ia)
-                    for (int ib = ss.shoff[a]; ib < ss.shoff[a + 1]; ++ib) {
-                        const double* sp = spb + (size_t)(ia * nsh + ib) * 5;
-                        for (int c = 0; c < 3; ++c) g[c] -= sp[c];
-                        dc += sp[4];
-                    }
+                    dc += pd * (-g_tab.e[ss.sh[s].el].kcn[ss.sh[s].k]);
+                }
+                gacc[3 * a] = g0;
+                gacc[3 * a + 1] = g1;
+                gacc[3 * a + 2] = g2;
+                dedcn[a] = dc;
+                dedcn4[a] = dc4;
             }
         }
-        // on-site self-energies
-        for (int s = ss.shoff[a]; s < ss.shoff[a + 1]; ++s) {
-            double pd = 0.0;
-            for (int k = 0; k < ss.sh[s].nao; ++k) {
-                const int mu = ss.sh[s].ao0 + k;
-                pd += P[(size_t)mu * ld + mu];
-            }
-            dc += pd * (-g_tab.e[ss.sh[s].el].kcn[ss.sh[s].k]);
-        }
-        for (int c = 0; c < 3; ++c) gacc[3 * a + c] = g[c];
-        dedcn[a] = dc;
-        dedcn4[a] = dc4;
     }
     __syncthreads();
     // Phase D: CN chain rules, ordered pairs
