@@ -398,32 +398,53 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
         c60[p] = c;
     }
     __syncthreads();
+    // triples i > j > k in contiguous equal chunks per thread (the pair-wise
+    // loop left half the threads idle and the rest unevenly loaded)
     double eatm = 0.0;
-    for (int p = tid; p < nat * nat; p += QX_BLOCK) {
-        const int i = p / nat, j = p % nat;
-        if (j >= i) continue;
-        const double dij[3] = {sx[3 * i] - sx[3 * j], sx[3 * i + 1] - sx[3 * j + 1], sx[3 * i + 2] - sx[3 * j + 2]};
-        const double u = dij[0] * dij[0] + dij[1] * dij[1] + dij[2] * dij[2];
-        if (u > gfn2::kD4AtmThr) continue;
-        const ElemParams& ei = g_tab.e[sel[i]];
-        const ElemParams& ej = g_tab.e[sel[j]];
-        const double cij = gfn2::kD4A1 * sqrt(3.0 * ei.r4r2 * ej.r4r2) + gfn2::kD4A2;
-        for (int k = 0; k < j; ++k) {
+    {
+        const long long ntri = (long long)nat * (nat - 1) * (nat - 2) / 6;
+        const long long chunk = (ntri + QX_BLOCK - 1) / QX_BLOCK;
+        long long t = (long long)tid * chunk;
+        const long long tend = t + chunk < ntri ? t + chunk : ntri;
+        // unrank t: i with C(i,3) <= t < C(i+1,3), then j with C(j,2) <= rest
+        int i = 2, j = 1, k = 0;
+        if (t < tend) {
+            while ((long long)(i + 1) * i * (i - 1) / 6 <= t) ++i;
+            long long rest = t - (long long)i * (i - 1) * (i - 2) / 6;
+            j = 1;
+            while ((long long)(j + 1) * j / 2 <= rest) ++j;
+            k = (int)(rest - (long long)j * (j - 1) / 2);
+        }
+        for (; t < tend; ++t) {
+            const double dij[3] = {sx[3 * i] - sx[3 * j], sx[3 * i + 1] - sx[3 * j + 1], sx[3 * i + 2] - sx[3 * j + 2]};
+            const double u = dij[0] * dij[0] + dij[1] * dij[1] + dij[2] * dij[2];
             const double dik[3] = {sx[3 * i] - sx[3 * k], sx[3 * i + 1] - sx[3 * k + 1], sx[3 * i + 2] - sx[3 * k + 2]};
             const double djk[3] = {sx[3 * j] - sx[3 * k], sx[3 * j + 1] - sx[3 * k + 1], sx[3 * j + 2] - sx[3 * k + 2]};
             const double w = dik[0] * dik[0] + dik[1] * dik[1] + dik[2] * dik[2];
             const double v = djk[0] * djk[0] + djk[1] * djk[1] + djk[2] * djk[2];
-            if (w > gfn2::kD4AtmThr || v > gfn2::kD4AtmThr) continue;
-            const ElemParams& ek = g_tab.e[sel[k]];
-            const double cik = gfn2::kD4A1 * sqrt(3.0 * ei.r4r2 * ek.r4r2) + gfn2::kD4A2;
-            const double cjk = gfn2::kD4A1 * sqrt(3.0 * ej.r4r2 * ek.r4r2) + gfn2::kD4A2;
-            const double uvw = u * v * w, rijk = sqrt(uvw), cijk = cij * cjk * cik;
-            const double aa = u + v - w, bb = u + w - v, cc = w + v - u;
-            const double atm = (0.375 * aa * bb * cc / uvw + 1.0) / (rijk * rijk * rijk);
-            const double x = pow(cijk / rijk, gfn2::kD4Alp / 3.0);
-            const double fd = 1.0 / (1.0 + 6.0 * x);
-            const double c9 = gfn2::kD4S9 * sqrt(c60[i * nat + j] * c60[j * nat + k] * c60[i * nat + k]);
-            eatm += c9 * atm * fd;
+            if (u <= gfn2::kD4AtmThr && w <= gfn2::kD4AtmThr && v <= gfn2::kD4AtmThr) {
+                const ElemParams& ei = g_tab.e[sel[i]];
+                const ElemParams& ej = g_tab.e[sel[j]];
+                const ElemParams& ek = g_tab.e[sel[k]];
+                const double cij = gfn2::kD4A1 * sqrt(3.0 * ei.r4r2 * ej.r4r2) + gfn2::kD4A2;
+                const double cik = gfn2::kD4A1 * sqrt(3.0 * ei.r4r2 * ek.r4r2) + gfn2::kD4A2;
+                const double cjk = gfn2::kD4A1 * sqrt(3.0 * ej.r4r2 * ek.r4r2) + gfn2::kD4A2;
+                const double uvw = u * v * w, rijk = sqrt(uvw), cijk = cij * cjk * cik;
+                const double aa = u + v - w, bb = u + w - v, cc = w + v - u;
+                const double atm = (0.375 * aa * bb * cc / uvw + 1.0) / (rijk * rijk * rijk);
+                const double x = pow(cijk / rijk, gfn2::kD4Alp / 3.0);
+                const double fd = 1.0 / (1.0 + 6.0 * x);
+                const double c9 = gfn2::kD4S9 * sqrt(c60[i * nat + j] * c60[j * nat + k] * c60[i * nat + k]);
+                eatm += c9 * atm * fd;
+            }
+            // next triple in (i, j, k) lexicographic order with k < j < i
+            if (++k == j) {
+                k = 0;
+                if (++j == i) {
+                    j = 1;
+                    ++i;
+                }
+            }
         }
     }
     eatm = block_sum(eatm, red);
@@ -446,10 +467,16 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
         for (int i = j + 1 + tid; i < n; i += QX_BLOCK) A[(size_t)i * ld + j] /= d;
         __syncthreads();
         if (tid == 0) A[(size_t)j * ld + j] = d;
+        // trailing lower triangle: row j + 1 + r by lane r, its columns
+        // j + 1 + c (c <= r) by the wavefronts in turn
         const int m = n - j - 1;
-        for (int p = tid; p < m * m; p += QX_BLOCK) {
-            const int i = j + 1 + p / m, k = j + 1 + p % m;
-            if (k <= i) A[(size_t)i * ld + k] -= A[(size_t)i * ld + j] * A[(size_t)k * ld + j];
+        for (int r = tid & 63; r < m; r += 64) {
+            const int i = j + 1 + r;
+            const double lij = A[(size_t)i * ld + j];
+            for (int c = tid >> 6; c <= r; c += QX_BLOCK / 64) {
+                const int k = j + 1 + c;
+                A[(size_t)i * ld + k] -= lij * A[(size_t)k * ld + j];
+            }
         }
         __syncthreads();
     }
@@ -475,6 +502,7 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
         double* xr = lx ? xl + (size_t)c * xs : Xg + (size_t)c * ld;  // X[c][i] = Linv[i][c]
         for (int i = c; i < n; ++i) {
             double v = (i == c) ? 1.0 : 0.0;
+#pragma unroll 8
             for (int k = c; k < i; ++k) v -= A[(size_t)i * ld + k] * xr[k];
             xr[i] = v / A[(size_t)i * ld + i];
         }
