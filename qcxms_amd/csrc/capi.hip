@@ -596,7 +596,7 @@ int enqueue_md_step(qcxms_md* md)
         return st;
     if ((st = enqueue_egrad(b, true))) return st;
     return timed(b, b->t_md_post, [&] {
-        hipLaunchKernelGGL(qx::k_md_post, dim3(md->ntraj), dim3(64), 0, b->stream, b->d, md->m, md->p);
+        hipLaunchKernelGGL(qx::k_md_post, dim3(md->ntraj), dim3(QX_BLOCK), 0, b->stream, b->d, md->m, md->p);
     });
 }
 

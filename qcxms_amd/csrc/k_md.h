@@ -457,7 +457,8 @@ __global__ __launch_bounds__(64) void k_md_pre(BatchPtrs B, MdPtrs M, MdParams p
     md_pre_system(B, M, p, blockIdx.x, Lm);
 }
 
-__global__ __launch_bounds__(64) void k_md_post(BatchPtrs B, MdPtrs M, MdParams p)
+// 256 threads: the heating search evaluates 256 increments per pass
+__global__ __launch_bounds__(QX_BLOCK) void k_md_post(BatchPtrs B, MdPtrs M, MdParams p)
 {
     __shared__ MdLds Lm;
     md_post_system(B, M, p, blockIdx.x, Lm);
