@@ -281,8 +281,9 @@ def main():
                     help="only run a 256 MiB device copy (counter calibration)")
     ap.add_argument("--fused", type=int, default=0,
                     help="1: persistent kernel running whole MD iterations per trajectory")
-    ap.add_argument("--eigensolver", type=int, default=0, choices=[0, 1],
-                    help="SCC eigenvectors: 0 Jacobi, 1 MFMA simultaneous rotations then Jacobi")
+    ap.add_argument("--eigensolver", type=int, default=0, choices=[0, 1, 2],
+                    help="SCC eigenvectors: 1 MFMA simultaneous rotations then Jacobi, 2 Jacobi "
+                         "only, 0 library default (1)")
     ap.add_argument("--lanes", type=int, default=1,
                     help="split the ensemble into this many groups on separate streams")
     ap.add_argument("--config", default="md256",
@@ -325,7 +326,7 @@ def main():
                                "GFN2-xTB FP64, etemp 5000 K, 0.5 fs leap-frog, IEE Poisson "
                                "0.6 eV/atom (<= 1.5), tadd 100 fs, 500 K Maxwell-Boltzmann",
                    "ntraj": args.ntraj, "lanes": args.lanes, "fused": bool(args.fused),
-                   "eigensolver": "mfma+jacobi" if args.eigensolver else "jacobi",
+                   "eigensolver": "jacobi" if args.eigensolver == 2 else "mfma+jacobi",
                    "trajectory_steps": g["traj_steps"],
                    "running_after": g["running"],
                    "scc_iterations_per_trajectory_step": g["scc_iters_per_traj_step"]},

@@ -122,9 +122,10 @@ typedef struct {
                              trajectory (bookkeeping, integrals, SCC, gradient) back to
                              back; 0: one launch per kernel and step */
     int32_t eigensolver;  /* SCC eigenvectors of systems with at most 52 basis functions:
-                             0 parallel cyclic Jacobi (default); 1 simultaneous rotations
-                             by FP64 MFMA products while every angle is below 0.1, then
-                             Jacobi.  Same fixed point and tolerances either way. */
+                             1 simultaneous rotations by FP64 MFMA products while every
+                             angle is below 0.1, then parallel cyclic Jacobi; 2 Jacobi
+                             only; 0 library default (1).  Same fixed point and
+                             tolerances either way. */
 } qcxms_md_params;
 
 /* Final or current state of one trajectory (md.f90:684-700 outputs). */

@@ -189,7 +189,9 @@ int ensure_init(int device)
     size_t lds_static = fa.sharedSizeBytes;
     HIPCHK(hipFuncGetAttributes(&fa, (const void*)qx::k_scc<true>));
     lds_static = std::max(lds_static, fa.sharedSizeBytes);
-    HIPCHK(hipFuncGetAttributes(&fa, (const void*)qx::k_md_fused));
+    HIPCHK(hipFuncGetAttributes(&fa, (const void*)qx::k_md_fused<false>));
+    lds_static = std::max(lds_static, fa.sharedSizeBytes);
+    HIPCHK(hipFuncGetAttributes(&fa, (const void*)qx::k_md_fused<true>));
     lds_static = std::max(lds_static, fa.sharedSizeBytes);
     const size_t base_dyn = (4 * qx::kFastM * (qx::kFastM + 1) + qx::kRs * qx::kFastM) * sizeof(double);
     if ((size_t)lds_block < lds_static + base_dyn) return fail("the device's LDS per workgroup is too small");
@@ -202,7 +204,9 @@ int ensure_init(int device)
                                (QX_LDS_MAXM * QX_LDS_MAXM + QX_LDS_MAXM * (QX_LDS_MAXM | 1)) *
                                    (int)sizeof(double)));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_grad, hipFuncAttributeMaxDynamicSharedMemorySize, 0));
-    HIPCHK(hipFuncSetAttribute((const void*)qx::k_md_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIPCHK(hipFuncSetAttribute((const void*)qx::k_md_fused<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)g_scc_dyn_max));
+    HIPCHK(hipFuncSetAttribute((const void*)qx::k_md_fused<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)g_scc_dyn_max));
     g_device = device;
     g_ready = true;
@@ -954,17 +958,12 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
     md->ntraj = ntraj;
     md->use_graph = params->use_graph != 0;
     md->fused = params->fused != 0;
-    if (params->eigensolver < 0 || params->eigensolver > 1) {
+    if (params->eigensolver < 0 || params->eigensolver > 2) {
         delete md;
-        g_err = "eigensolver must be 0 (Jacobi) or 1 (MFMA rotations with Jacobi)";
+        g_err = "eigensolver must be 0 (default), 1 (MFMA rotations, then Jacobi) or 2 (Jacobi)";
         return nullptr;
     }
-    if (params->eigensolver == 1 && md->fused) {
-        delete md;
-        g_err = "the persistent MD kernel (fused) runs the Jacobi eigensolver only";
-        return nullptr;
-    }
-    b->opts.eig_mfma = params->eigensolver;
+    b->opts.eig_mfma = params->eigensolver == 2 ? 0 : 1;
     md->p.tstep = params->tstep;
     md->p.etempin = params->etempin;
     md->p.ax = params->ax;
@@ -1097,8 +1096,8 @@ int md_enqueue_fused(qcxms_md* md, int nsteps)
     qcxms_batch* b = md->b;
     const size_t lds = std::max(lds_setup_bytes(b), lds_scc_bytes(b));
     return timed(b, b->t_fused, [&] {
-        hipLaunchKernelGGL(qx::k_md_fused, dim3(md->ntraj), dim3(QX_SCC_BLOCK), lds, b->stream, b->d, md->m,
-                           md->p, launch_opts(b, lds), nsteps);
+        hipLaunchKernelGGL(b->opts.eig_mfma ? qx::k_md_fused<true> : qx::k_md_fused<false>, dim3(md->ntraj),
+                           dim3(QX_SCC_BLOCK), lds, b->stream, b->d, md->m, md->p, launch_opts(b, lds), nsteps);
     });
 }
 

@@ -476,6 +476,7 @@ union FusedLds {
     MdLds m;
 };
 
+template <bool EIG>
 __global__ __launch_bounds__(QX_SCC_BLOCK) void k_md_fused(BatchPtrs B, MdPtrs M, MdParams p, SccOpts opt,
                                                            int nsteps)
 {
@@ -490,7 +491,7 @@ __global__ __launch_bounds__(QX_SCC_BLOCK) void k_md_fused(BatchPtrs B, MdPtrs M
         if (!M.active[sys]) break;
         setup_system(B, sys, 1, L.s, dyn);
         __syncthreads();
-        scc_system<false>(B, opt, sys, L.c, dyn);
+        scc_system<EIG>(B, opt, sys, L.c, dyn);
         __syncthreads();
         grad_system(B, sys, L.g);
         __syncthreads();

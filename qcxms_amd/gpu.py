@@ -263,8 +263,9 @@ class MdEnsemble:
     groups, each an ensemble on its own stream, advanced together
     (qcxms_md_run_group) so that the device overlaps their kernels.  fused
     runs whole MD iterations per trajectory in one persistent launch.
-    eigensolver 1 refines the SCC eigenvectors by simultaneous rotations on
-    the FP64 matrix cores before Jacobi (0: Jacobi only, the default)."""
+    eigensolver: 1 refines the SCC eigenvectors by simultaneous rotations on
+    the FP64 matrix cores before Jacobi, 2 Jacobi only, 0 the library
+    default (1)."""
 
     def __init__(self, trajectories, tstep=0.5 * 41.3413733365614, max_steps=2000,
                  nfragexit=3, etempin=-1.0, ax=0.0, ieetemp=0.0, use_graph=True,

@@ -225,16 +225,16 @@ def test_fused_md_chunks_equal_one_launch():
     assert max(abs(x.epot - y.epot) for x, y in zip(ra["results"], rc["results"])) < 1e-8
 
 
-def test_mfma_eigensolver_md_matches_oracle_md():
-    """MD with the SCC eigenvectors refined by simultaneous rotations on the
-    FP64 matrix cores (eigensolver 1, Jacobi where an angle exceeds 0.1)
-    against the oracle MD over 30 heating steps; an unknown eigensolver is
-    rejected."""
+def test_jacobi_eigensolver_md_matches_oracle_md():
+    """MD with the SCC eigenvectors from parallel cyclic Jacobi alone
+    (eigensolver 2; the default refines them by simultaneous rotations on the
+    FP64 matrix cores first, which every other MD test exercises) against the
+    oracle MD over 30 heating steps; an unknown eigensolver is rejected."""
     trajs = ensemble(4, seed0=57)
     nsteps = 30
     with pytest.raises(RuntimeError):
-        q().MdEnsemble(trajs, max_steps=nsteps, eigensolver=2)
-    md = q().MdEnsemble(trajs, max_steps=nsteps, eigensolver=1, use_graph=False, econv=1e-11, pconv=1e-9)
+        q().MdEnsemble(trajs, max_steps=nsteps, eigensolver=3)
+    md = q().MdEnsemble(trajs, max_steps=nsteps, eigensolver=2, use_graph=False, econv=1e-11, pconv=1e-9)
     running = md.run(nsteps + 5)
     out = md.read()
     md.close()
@@ -244,7 +244,7 @@ def test_mfma_eigensolver_md_matches_oracle_md():
         ref = ob.md_run(tr, steps_limit=0, max_steps=nsteps, opts=opts)
         st, res = ref["state"], out["results"][t]
         sl = slice(md.offsets[t], md.offsets[t + 1])
-        print(f"mfma-eigen traj {t}: nstep {res.nstep}/{st.nstep} dEpot {abs(res.epot - st.epot):.2e} "
+        print(f"jacobi traj {t}: nstep {res.nstep}/{st.nstep} dEpot {abs(res.epot - st.epot):.2e} "
               f"dx {np.abs(out['xyz'][sl] - ref['xyz']).max():.2e}", flush=True)
         assert res.nstep == st.nstep and res.exit_reason == st.exit_reason
         assert np.array_equal(out["list"][sl], ref["list"])
