@@ -48,19 +48,21 @@ TRAFFIC_FILES = {False: os.path.join(ROOT, "profiles", "k_scc_hbm_traffic_c8h18_
                  True: os.path.join(ROOT, "profiles", "k_md_fused_hbm_traffic_c8h18_256.txt")}
 
 
-def traffic_from_profiles(fused=False):
+def traffic_from_profiles(fused=False, steps_per_launch=1.0):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain-copy
-    calibration), or None."""
+    calibration), or None.  The persistent kernel's summary gives bytes per
+    MD step, scaled here by the steps one launch runs."""
     path = TRAFFIC_FILES[fused]
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("bytes_per_launch"):
-            try:
-                return float(line.split("=")[1].split()[0])
-            except (IndexError, ValueError):
-                return None
+        for key, scale in (("bytes_per_launch", 1.0), ("bytes_per_step", steps_per_launch)):
+            if line.startswith(key):
+                try:
+                    return float(line.split("=")[1].split()[0]) * scale
+                except (IndexError, ValueError):
+                    return None
     return None
 
 
@@ -96,6 +98,7 @@ def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1, fused=False, ei
     out = dict(wall=wall, traj_steps=traj_steps, value=traj_steps / wall,
                ms_per_step=1e3 * wall / steps, running=running,
                scc_iters_per_traj_step=(it1 - it0) / max(traj_steps, 1),
+               dominant_launches=scc_launches,
                kernels={k: dict(ms_total=v[0], launches=v[1],
                                 ms_avg=(v[0] / v[1] if v[1] else 0.0)) for k, v in kt.items()},
                achieved_tflops=achieved, scc_avg_ms=scc_ms / max(scc_launches, 1),
@@ -279,8 +282,9 @@ def main():
                     help="skip the hipGraph replay of the same steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration)")
-    ap.add_argument("--fused", type=int, default=0,
-                    help="1: persistent kernel running whole MD iterations per trajectory")
+    ap.add_argument("--fused", type=int, default=1, choices=[0, 1],
+                    help="1 (default): persistent kernel running whole MD iterations per "
+                         "trajectory; 0: one launch per kernel and step")
     ap.add_argument("--eigensolver", type=int, default=0, choices=[0, 1, 2],
                     help="SCC eigenvectors: 1 MFMA simultaneous rotations then Jacobi, 2 Jacobi "
                          "only, 0 library default (1)")
@@ -333,7 +337,8 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "k_md_fused" if args.fused else "k_scc",
                      "achieved": g["achieved_tflops"], "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": g["achieved_tflops"] / FP64_PEAK_TFLOPS,
-                     "traffic": traffic_from_profiles(bool(args.fused)),
+                     "traffic": traffic_from_profiles(
+                         bool(args.fused), args.steps / max(g["dominant_launches"], 1)),
                      "flops_per_launch": g["scc_flops_per_launch"],
                      "avg_launch_ms": g["scc_avg_ms"]},
         "kernels_ms_avg": {k: v["ms_avg"] for k, v in g["kernels"].items()},

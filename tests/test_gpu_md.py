@@ -132,29 +132,33 @@ def test_md_fragment_exit_matches_oracle():
 
 
 def test_bench_workload_at_default_thresholds_meets_parity():
-    """The batched MD exactly as bench.py runs it (default SCC thresholds,
-    warm-started and extrapolated SCC, hipGraph replay) stays within the
-    parity bar of the tightly converged oracle MD: per-step energies to
-    1e-8 Eh, forces to 1e-6 Eh/bohr, charges to 1e-6 e."""
+    """The batched MD exactly as bench.py runs it by default (persistent MD
+    kernel, default SCC thresholds and eigensolver, warm-started and
+    extrapolated SCC) and as the per-step kernels replayed as hipGraphs run
+    it, both within the parity bar of the tightly converged oracle MD:
+    per-step energies to 1e-8 Eh, forces to 1e-6 Eh/bohr, charges to 1e-6 e."""
     import bench
     trajs = bench.workload(4)
     nsteps = 12
-    md = q().MdEnsemble(trajs, max_steps=nsteps, use_graph=True)
-    md.run(nsteps)
-    out = md.read()
-    md.close()
+    outs = []
+    for kw in (dict(fused=True), dict(use_graph=True)):
+        md = q().MdEnsemble(trajs, max_steps=nsteps, **kw)
+        md.run(nsteps)
+        outs.append(md.read())
+        md.close()
     opts = ob.default_opts(econv=1e-12, pconv=1e-10)
     for t, tr in enumerate(trajs):
         ref = ob.md_run(tr, steps_limit=0, max_steps=nsteps, opts=opts)
-        st, res = ref["state"], out["results"][t]
-        sl = slice(md.offsets[t], md.offsets[t + 1])
-        de = abs(res.epot - st.epot)
-        dg = np.abs(out["grad"][sl] - ref["grad"]).max()
-        dq = np.abs(out["qat"][sl] - ref["qat"]).max()
-        dx = np.abs(out["xyz"][sl] - ref["xyz"]).max()
-        print(f"bench traj {t}: dE={de:.2e} dg={dg:.2e} dq={dq:.2e} dx={dx:.2e}", flush=True)
-        assert res.nstep == st.nstep == nsteps
-        assert de <= 1e-8 and dg <= 1e-6 and dq <= 1e-6
+        for mode, out in zip(("fused", "graph"), outs):
+            st, res = ref["state"], out["results"][t]
+            sl = slice(md.offsets[t], md.offsets[t + 1])
+            de = abs(res.epot - st.epot)
+            dg = np.abs(out["grad"][sl] - ref["grad"]).max()
+            dq = np.abs(out["qat"][sl] - ref["qat"]).max()
+            dx = np.abs(out["xyz"][sl] - ref["xyz"]).max()
+            print(f"bench traj {t} {mode}: dE={de:.2e} dg={dg:.2e} dq={dq:.2e} dx={dx:.2e}", flush=True)
+            assert res.nstep == st.nstep == nsteps
+            assert de <= 1e-8 and dg <= 1e-6 and dq <= 1e-6
 
 
 def test_md_lanes_equal_single_ensemble():
