@@ -121,7 +121,7 @@ typedef struct {
     int32_t fused;        /* != 0: one persistent launch runs whole MD iterations per
                              trajectory (bookkeeping, integrals, SCC, gradient) back to
                              back; 0: one launch per kernel and step */
-    int32_t eigensolver;  /* SCC eigenvectors of systems with at most 52 basis functions:
+    int32_t eigensolver;  /* SCC eigenvectors of systems with at most 50 basis functions:
                              1 simultaneous rotations by FP64 MFMA products while every
                              angle is below 0.1, then parallel cyclic Jacobi; 2 Jacobi
                              only; 0 library default (1).  Same fixed point and

@@ -30,6 +30,7 @@ bool g_ready = false;
 int g_device = -1;
 hipStream_t g_stream = nullptr;
 size_t g_scc_dyn_max = 0;  // dynamic LDS the SCC kernels may take (set at init)
+size_t g_eig_dyn_max = 0;  // the same with the MFMA eigen refinement's fifth matrix
 DevTables g_host_tab;
 
 int fail(const std::string& msg)
@@ -196,6 +197,11 @@ int ensure_init(int device)
     const size_t base_dyn = (4 * qx::kFastM * (qx::kFastM + 1) + qx::kRs * qx::kFastM) * sizeof(double);
     if ((size_t)lds_block < lds_static + base_dyn) return fail("the device's LDS per workgroup is too small");
     g_scc_dyn_max = (size_t)lds_block - lds_static;
+    // the optional fifth matrix (MFMA eigen refinement) only where the
+    // workgroup keeps 8 KiB of its LDS free: workgroups of up to 153 KB are
+    // what the MI355X runs have exercised (m <= 50)
+    constexpr size_t kLdsReserve = 8192;
+    g_eig_dyn_max = g_scc_dyn_max > kLdsReserve ? g_scc_dyn_max - kLdsReserve : 0;
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_scc<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)g_scc_dyn_max));
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_scc<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -496,7 +502,7 @@ size_t lds_scc_bytes(const qcxms_batch* b)
     // a fifth matrix for the MFMA eigen refinement when it is on and the LDS
     // holds it
     const size_t ext = base + sizeof(double) * lm * (lm + 1);
-    return b->opts.eig_mfma && ext <= g_scc_dyn_max ? ext : base;
+    return b->opts.eig_mfma && ext <= g_eig_dyn_max ? ext : base;
 }
 
 // the SCC options of one launch with its dynamic LDS size
