@@ -182,11 +182,14 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
         }
         refoff[nat] = o;
     }
-    // 1. coordination numbers (each thread owns an atom: no atomics)
-    for (int a = tid; a < nat; a += QX_BLOCK) {
+    // 1. coordination numbers: a wavefront per atom, a lane per partner,
+    //    tree sums over the lanes
+    __syncthreads();
+    for (int a = tid >> 6; a < nat; a += QX_BLOCK / 64) {
+        const int lane = tid & 63;
         const ElemParams& ea = g_tab.e[sel[a]];
         double cn = 0.0, cn4 = 0.0;
-        for (int b = 0; b < nat; ++b) {
+        for (int b = lane; b < nat; b += 64) {
             if (b == a) continue;
             const ElemParams& eb = g_tab.e[sel[b]];
             const double dx = sx[3 * a] - sx[3 * b], dy = sx[3 * a + 1] - sx[3 * b + 1],
@@ -203,6 +206,9 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
                 cn4 += f;
             }
         }
+        cn = wave_sum(cn);
+        cn4 = wave_sum(cn4);
+        if (lane != 0) continue;
         scn[a] = cn;
         scnd4[a] = cn4;
         B.cn[si.atom_off + a] = cn;
