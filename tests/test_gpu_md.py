@@ -255,3 +255,38 @@ def test_jacobi_eigensolver_md_matches_oracle_md():
         assert abs(res.epot - st.epot) < 1e-8
         assert np.abs(out["xyz"][sl] - ref["xyz"]).max() < 1e-8
         assert np.abs(out["velo"][sl] - ref["velo"]).max() < 1e-10
+
+
+def test_md_mixed_sizes_lds_fallback_matches_oracle():
+    """One ensemble of C8H18+ (50 basis functions: LDS matrices with the MFMA
+    eigen refinement) and two C/H/N/O chain cations of 53 and 56 basis
+    functions (LDS matrices without room for the refinement's fifth matrix:
+    Jacobi), default MD settings, 10 heating steps against the oracle; the
+    per-step kernels and the persistent kernel give the same bits."""
+    trajs = ensemble(1, seed0=73)
+    for seed in (5155, 5180):
+        num, xyz, charge = W.random_chain(seed)
+        mass = np.array([W.atomic_mass(int(z)) for z in num])
+        trajs.append(dict(num=num, xyz=xyz, velo=W.maxwell_boltzmann(mass, 500.0, seed + 7919), mass=mass,
+                          eimp=W.iee_poisson(len(num), 0.6, 1.5, seed + 104729), tadd=5.0 * W.FSTOAU,
+                          charge=charge))
+    nsteps = 10
+    outs = []
+    for fused in (False, True):
+        md = q().MdEnsemble(trajs, max_steps=nsteps, use_graph=False, fused=fused, econv=1e-11, pconv=1e-9)
+        md.run(nsteps + 2)
+        outs.append(md.read())
+        md.close()
+    # the persistent kernel runs the same code paths per trajectory
+    assert np.array_equal(outs[0]["xyz"], outs[1]["xyz"])
+    out = outs[0]
+    opts = ob.default_opts(econv=1e-12, pconv=1e-10)
+    for t, tr in enumerate(trajs):
+        ref = ob.md_run(tr, steps_limit=0, max_steps=nsteps, opts=opts, charge=tr.get("charge", 1))
+        st, res = ref["state"], out["results"][t]
+        sl = slice(md.offsets[t], md.offsets[t + 1])
+        print(f"mixed traj {t} ({len(tr['num'])} atoms): nstep {res.nstep}/{st.nstep} "
+              f"dEpot {abs(res.epot - st.epot):.2e} dx {np.abs(out['xyz'][sl] - ref['xyz']).max():.2e}", flush=True)
+        assert res.nstep == st.nstep and res.exit_reason == st.exit_reason
+        assert abs(res.epot - st.epot) < 1e-8
+        assert np.abs(out["xyz"][sl] - ref["xyz"]).max() < 1e-8
