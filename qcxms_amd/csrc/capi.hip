@@ -333,6 +333,18 @@ int layout(qcxms_batch* b, int nsys, const int32_t* offsets, const int32_t* num,
                 for (int ib = 0; ib <= ia; ++ib)
                     if (sh_at[ia] == sh_at[ib]) b->items.push_back(make_int2(ia, ib));
             si.nitem = (int)b->items.size() - si.item_off;
+            // the same pairs as rows of the first shell (a p shell has three),
+            // row index in the high half of x; rows over a p second shell
+            // (three AO pairs each) first
+            si.rowitem_off = (int)b->items.size();
+            for (int pass = 0; pass < 2; ++pass)
+                for (int it = si.item_off; it < si.item_off + si.nitem; ++it) {
+                    const int2 pr = b->items[it];
+                    if ((sh_l[pr.y] != 0) != (pass == 0)) continue;
+                    const int nrow = sh_l[pr.x] != 0 ? 3 : 1;
+                    for (int r = 0; r < nrow; ++r) b->items.push_back(make_int2(pr.x | (r << 16), pr.y));
+                }
+            si.nrowitem = (int)b->items.size() - si.rowitem_off;
         }
         si.charge = charge ? charge[s] : 0;
         si.nel = nel - si.charge;

@@ -56,6 +56,8 @@ struct SysInfo {
     int item_off;        // shell-pair work items: off-atom pairs first (grouped by
     int nitem_off;       // angular-momentum class), then same-atom pairs
     int nitem;
+    int rowitem_off;     // the same pairs split into rows of the first shell's AOs
+    int nrowitem;        // (rows of three AO pairs first), for the setup integrals
     int charge;
     double nel, kt;
 };

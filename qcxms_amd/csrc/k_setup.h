@@ -128,18 +128,6 @@ QX_DEV void setup_shell_row(const ShellDesc& sa, const ShellDesc& sb, const Elem
     }
 }
 
-template <int LA, int LB>
-QX_DEV void setup_shell_pair(const ShellDesc& sa, const ShellDesc& sb, const ElemParams& ea,
-                             const ElemParams& eb, const double* ra, const double* rb,
-                             const double t[3], double hs, bool off, const IntegralOut& io)
-{
-    setup_shell_row<LA, LB, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io);
-    if constexpr (LA == 1) {
-        setup_shell_row<LA, LB, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, io);
-        setup_shell_row<LA, LB, 2>(sa, sb, ea, eb, ra, rb, t, hs, off, io);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // LDS of one system's setup (the Cholesky factor uses the dynamic LDS)
 struct SetupLds {
@@ -267,11 +255,12 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
     for (size_t i = tid; i < 3 * ms; i += QX_BLOCK) D[i] = 0.0;
     for (size_t i = tid; i < 6 * ms; i += QX_BLOCK) Q[i] = 0.0;
     __syncthreads();
-    // 4. integrals over shell pairs, one work item per pair (off-atom pairs
-    //    grouped by angular-momentum class, then same-atom pairs)
-    for (int it = tid; it < si.nitem; it += QX_BLOCK) {
-        const int2 pr = B.spitem[si.item_off + it];
-        const ShellDesc sa = sh[pr.x], sb = sh[pr.y];
+    // 4. integrals, one work item per row of a shell pair (the rows of a p
+    //    shell are three items; rows of three AO pairs come first)
+    for (int it = tid; it < si.nrowitem; it += QX_BLOCK) {
+        const int2 pr = B.spitem[si.rowitem_off + it];
+        const int ia = pr.x & 0xffff, row = pr.x >> 16, ib = pr.y;
+        const ShellDesc sa = sh[ia], sb = sh[ib];
         const ElemParams& ea = g_tab.e[sa.el];
         const ElemParams& eb = g_tab.e[sb.el];
         const double* ra = sx + 3 * sa.at;
@@ -282,15 +271,19 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
         if (off) {
             const double r = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
             double pir, dpir, kxy;
-            hs = 0.5 * (B.hself[si.shell_off + pr.x] + B.hself[si.shell_off + pr.y]) *
+            hs = 0.5 * (B.hself[si.shell_off + ia] + B.hself[si.shell_off + ib]) *
                  h0_scale(ea, sa.k, eb, sb.k, r, pir, dpir, kxy);
         }
         const IntegralOut io{S, H, D, Q, ld, ms};
-        switch (sa.l * 2 + sb.l) {
-        case 3: setup_shell_pair<1, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
-        case 2: setup_shell_pair<1, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
-        case 1: setup_shell_pair<0, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
-        default: setup_shell_pair<0, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        switch ((sa.l * 2 + sb.l) * 3 + row) {
+        case 9: setup_shell_row<1, 1, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        case 10: setup_shell_row<1, 1, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        case 11: setup_shell_row<1, 1, 2>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        case 6: setup_shell_row<1, 0, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        case 7: setup_shell_row<1, 0, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        case 8: setup_shell_row<1, 0, 2>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        case 3: setup_shell_row<0, 1, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        default: setup_shell_row<0, 0, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
         }
     }
     __syncthreads();
