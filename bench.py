@@ -10,10 +10,13 @@ leap-frog steps, trajectory length 5000 steps.  A step is one pass of the MD
 loop (src/md.f90:285-682) over every running trajectory.
 
 value = trajectory-steps executed / wall time of the timed steps, with all
-state resident in HBM.  The roofline object describes the dominant kernel
-(the SCC kernel) with the analytic FP64 flop count per SCC iteration
-15 n^3 + 30 n^2 (n = basis functions), timed by HIP events on the library's
-stream.  cpu_baseline: the C oracle's restatement of the same MD loop with the
+state resident in HBM.  By default the steps run in the persistent MD kernel
+(k_md_fused: whole MD iterations per trajectory, many steps per launch) with
+the SCC eigenvectors refined by FP64 MFMA rotations; --fused 0 runs one launch
+per kernel and step.  The roofline object describes the dominant kernel
+(k_md_fused, or the SCC kernel k_scc with --fused 0) with the analytic FP64
+flop count per SCC iteration 15 n^3 + 30 n^2 (n = basis functions), timed by
+HIP events on the library's stream.  cpu_baseline: the C oracle's restatement of the same MD loop with the
 same forces (not QCxMS+tblite, which cannot be built here), one trajectory per
 host thread as bin/pqcxms runs it, on a bounded sample.
 """
