@@ -106,9 +106,10 @@ def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1, fused=False, ei
                                 ms_avg=(v[0] / v[1] if v[1] else 0.0)) for k, v in kt.items()},
                achieved_tflops=achieved, scc_avg_ms=scc_ms / max(scc_launches, 1),
                scc_flops_per_launch=flops / max(scc_launches, 1))
-    # the same steps replayed as one hipGraph per step (no per-kernel events)
+    # the same steps through the per-step kernels, each step replayed as one
+    # captured hipGraph (no per-kernel events)
     if use_graph_check:
-        md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True, lanes=lanes, fused=fused,
+        md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=True, lanes=lanes, fused=False,
                                   eigensolver=eigensolver)
         md.run(warmup)
         n0 = sum(r.nstep for r in md.read()["results"])
@@ -117,7 +118,7 @@ def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1, fused=False, ei
         t1 = time.perf_counter()
         n1 = sum(r.nstep for r in md.read()["results"])
         md.close()
-        out["graph_value"] = (n1 - n0) / (t1 - t0)
+        out["per_step_hipgraph_value"] = (n1 - n0) / (t1 - t0)
     return out
 
 
@@ -217,14 +218,28 @@ def run_one_per_launch(steps, warmup, fused=False):
     return (n1 - n0) / (t1 - t0)
 
 
-def run_cpu_baseline(steps):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
+def host_cores():
+    """(usable, nproc, quota): CPUs this process may run on (sched affinity,
+    what nproc prints), os.cpu_count(), and the cgroup CPU quota in CPUs
+    (None when unlimited)."""
+    usable = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    return usable, os.cpu_count() or usable, quota
+
+
+def _cpu_md(trajs, steps, threads):
+    """Oracle C MD + GFN2 of trajs on `threads` host threads; returns
+    (trajectory-steps done, wall seconds)."""
     import ctypes as C
 
     import oracle_bind as ob
-    threads = max(1, min(os.cpu_count() or 1, 16))
-    ntraj = threads
-    trajs = workload(ntraj)
+    ntraj = len(trajs)
     L = ob.oracle()
     L.orc_md_run_batch.argtypes = [C.c_int, ob.ip, ob.ip, ob.ip, ob.dp, ob.dp, ob.dp, ob.dp,
                                    ob.dp, ob.dp, ob.ip, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
@@ -241,7 +256,7 @@ def run_cpu_baseline(steps):
     states = (ob.MdState * ntraj)()
     for s, t in enumerate(trajs):
         st = states[s]
-        st.nat, st.chrg, st.nfragexit, st.max_steps = len(t["num"]), 1, 3, 5000
+        st.nat, st.chrg, st.nfragexit, st.max_steps, st.isec = len(t["num"]), 1, 3, 5000, 1
         st.tstep, st.tadd, st.eimp = 0.5 * 41.3413733365614, t["tadd"], t["eimp"]
         st.ax, st.ieetemp, st.etempin = 0.0, 0.0, -1.0
     opts = ob.default_opts()
@@ -251,11 +266,69 @@ def run_cpu_baseline(steps):
                        ob.ptr(qat), ob.ptr(lst, ob.ip), C.cast(states, C.c_void_p),
                        C.cast(C.pointer(opts), C.c_void_p), steps, threads)
     wall = time.perf_counter() - t0
-    done = sum(states[s].nstep for s in range(ntraj))
+    return sum(states[s].nstep for s in range(ntraj)), wall
+
+
+def eigensolver_vs_lapack(n=50, reps=200):
+    """Per-core speed of the oracle's generalized eigensolver (Cholesky +
+    Householder tridiagonalisation + implicit QL, as tred2/tql2) against
+    LAPACK's (scipy.linalg.eigh(F, S), dsygvd-class) on one thread, for an
+    n x n problem shaped like a GFN2 Fock/overlap pair."""
+    import ctypes as C
+
+    import oracle_bind as ob
+    from scipy import linalg
+    rng = np.random.default_rng(0)
+    a = rng.normal(size=(n, n))
+    s = a @ a.T / n + np.eye(n)
+    f = rng.normal(size=(n, n))
+    f = 0.5 * (f + f.T)
+    L = ob.oracle()
+    L.orc_sygv.argtypes = [C.c_int, ob.dp, ob.dp, ob.dp, ob.dp]
+    ev, work = np.zeros(n), np.zeros(4 * n * n + 8 * n)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fc, sc = f.copy(), s.copy()
+        L.orc_sygv(n, ob.ptr(fc), ob.ptr(sc), ob.ptr(ev), ob.ptr(work))
+    t_orc = (time.perf_counter() - t0) / reps
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            linalg.eigh(f, s, driver="gvd")
+        t_lap = (time.perf_counter() - t0) / reps
+    return dict(n=n, oracle_ms=1e3 * t_orc, lapack_ms=1e3 * t_lap, ratio=t_orc / t_lap)
+
+
+def run_cpu_baseline(steps=None, target_s=20.0):
+    """One trajectory per usable host CPU, each on its own thread (bin/pqcxms
+    runs one single-threaded qcxms per core), for a bounded number of steps so
+    that the CPU work lasts about target_s seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    usable, nproc, quota = host_cores()
+    threads = usable
+    eff = min(usable, quota) if quota else usable
+    if steps is None:
+        # about 42 ms per cold-started trajectory-step and thread
+        steps = max(10, int(target_s / 0.042 * eff / threads))
+    done, wall = _cpu_md(workload(threads), steps, threads)
+    eig = eigensolver_vs_lapack()
     return dict(value=done / wall, unit="trajectory-steps/s", cores=threads, kind="port",
-                sample=f"{ntraj} trajectories x {steps} MD steps of the same C8H18+ workload "
+                nproc=nproc, cgroup_cpu_quota=quota,
+                sample=f"{threads} trajectories x {steps} MD steps of the same C8H18+ workload "
                        f"(oracle C restatement, SCC cold-started every step as the reference, "
-                       f"one trajectory per thread; not QCxMS+tblite), wall {wall:.1f} s")
+                       f"one trajectory per thread on every usable host CPU; not "
+                       f"QCxMS+tblite), wall {wall:.1f} s",
+                eigensolver_vs_lapack=eig)
+
+
+def run_cpu_single(steps=200):
+    """Config-0 latency on the CPU: one C8H18+ trajectory on one thread, a
+    bounded number of steps of the 2000-step run."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    done, wall = _cpu_md(workload(1), steps, 1)
+    return dict(ms_per_step=1e3 * wall / max(done, 1), steps=done,
+                est_2000_steps_s=2000 * wall / max(done, 1), cores=1, kind="port")
 
 
 def copy_calibration(mib=256, reps=5):
@@ -278,8 +351,13 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--ntraj", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=250)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="MD steps per trajectory of the CPU baseline (default: about 20 s "
+                         "of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--steady-steps", type=int, default=400,
+                    help="also time this many steps after --steady-warmup steps (0: skip)")
+    ap.add_argument("--steady-warmup", type=int, default=220)
     ap.add_argument("--no-one-per-launch", action="store_true")
     ap.add_argument("--no-graph-check", action="store_true",
                     help="skip the hipGraph replay of the same steps")
@@ -345,13 +423,27 @@ def main():
                      "flops_per_launch": g["scc_flops_per_launch"],
                      "avg_launch_ms": g["scc_avg_ms"]},
         "kernels_ms_avg": {k: v["ms_avg"] for k, v in g["kernels"].items()},
-        "graph_replay_value": g.get("graph_value"),
+        "per_step_hipgraph": {"value": g.get("per_step_hipgraph_value"),
+                              "unit": "trajectory-steps/s",
+                              "form": "per-step kernels, each MD step replayed as one hipGraph"},
     }
+    if args.steady_steps > 0:
+        # the same workload past heating (tadd = 100 fs = 200 steps), so that a
+        # short timed window inside the heating phase is shown beside the rate
+        # the trajectories keep for the rest of their run
+        w = run_gpu(args.ntraj, args.steady_steps, args.steady_warmup, use_graph_check=False,
+                    lanes=args.lanes, fused=bool(args.fused), eigensolver=args.eigensolver)
+        line["steady_state"] = {"value": w["value"], "unit": "trajectory-steps/s",
+                                "steps": args.steady_steps, "warmup": args.steady_warmup,
+                                "ms_per_step": w["ms_per_step"],
+                                "scc_iterations_per_trajectory_step": w["scc_iters_per_traj_step"],
+                                "roofline_frac": w["achieved_tflops"] / FP64_PEAK_TFLOPS}
     if not args.no_one_per_launch:
         line["one_per_launch"] = {"value": run_one_per_launch(100, 10, bool(args.fused)),
                                   "unit": "trajectory-steps/s"}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = run_cpu_baseline(args.cpu_steps)
+        line["cpu_single_trajectory"] = run_cpu_single()
     print(json.dumps(line), flush=True)
 
 

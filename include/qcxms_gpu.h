@@ -171,6 +171,12 @@ int qcxms_md_read(qcxms_md *md, double *xyz, double *velo, double *grad, double 
 int qcxms_md_read_averages(qcxms_md *md, double *axyz, double *achrg);
 /* Timing hooks of the ensemble's underlying batch (see qcxms_batch_kernel_time). */
 qcxms_batch *qcxms_md_batch(qcxms_md *md);
+/* Fragmentation generation isec[ntraj] (>= 1) of each trajectory, the isec
+ * argument of md (src/md.f90:34, src/main.F90:2228): from the second
+ * generation on a trajectory exits at more than two fragments
+ * (src/main.F90:2255) and an SCF or drift failure keeps the run
+ * (src/md.f90:329).  Default 1. */
+int qcxms_md_set_generation(qcxms_md *md, const int32_t *isec);
 
 /* ---- batched drop-ins for single reference routines ---- */
 /* fragment_structure(nat,iat,xyz,3.0,1,0,list) + fragmass masses

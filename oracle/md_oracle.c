@@ -313,6 +313,18 @@ void orc_intenergy(int nuc, const int32_t *list, const double *mass, const doubl
         t[i] = eint[i] / ((double)(1.5f * (float)n[i]) * ORC_KB);
 }
 
+/* Optional per-step record for the parity tests (per thread): after the
+ * connectivity of every loop iteration, ORC_MD_TRACE doubles: nstep, the
+ * electronic temperature set for the next egrad, epot, ekin, nfrag,
+ * fragT(1), fragT(2), eerror.  cap bounds the number of records. */
+static _Thread_local double *md_trace;
+static _Thread_local int md_trace_cap;
+void orc_md_set_trace(double *buf, int cap)
+{
+    md_trace = buf;
+    md_trace_cap = buf ? cap : 0;
+}
+
 /* md.f90:34-708 restricted to the EI production branch (it>0, method<3,
  * icoll=0, no restart).  egrad follows iniqm.f90:457-679 for prog=8: the
  * callback returns E, grad, qat; checkqc is applied here.  steps_limit
@@ -335,6 +347,8 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
 {
     const int nuc = st->nat;
     const int more = 250, avdump = 50;
+    /* later generations exit at two fragments (main.F90:2255) */
+    const int nfragexit = st->isec > 1 ? 2 : st->nfragexit;
     double ekin, t, epot;
     double *avchrg = (double *)calloc((size_t)nuc, sizeof(double));
     double *avxyz = (double *)calloc(3 * (size_t)nuc, sizeof(double));
@@ -385,7 +399,7 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
         if (nfrag == 1) fragT[0] = tav / nstep;
         int err1 = (epot == 0.0), err2 = (fabs(eerror) > (double)0.1f);
         if (err1 || err2) {
-            st->mdok = (nfrag > 1 && nfrag <= 4);
+            st->mdok = (nfrag > 1 && nfrag <= 4) || st->isec > 1;
             st->exit_reason = err1 ? ORC_MD_ERR_SCF : ORC_MD_ERR_DRIFT;
             break;
         }
@@ -425,11 +439,22 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
         orc_fragmass(nuc, iat, list, mass, imass, &nf, fragm, fragat);
         nfrag = nf;
         if (nfrag > 1) orc_intenergy(nuc, list, mass, velo, nfrag, fragT);
+        if (md_trace && nstep <= md_trace_cap) {
+            double *r = md_trace + ORC_MD_TRACE * (size_t)(nstep - 1);
+            r[0] = nstep;
+            r[1] = etemp;
+            r[2] = epot;
+            r[3] = ekin;
+            r[4] = nfrag;
+            r[5] = fragT[0];
+            r[6] = fragT[1];
+            r[7] = eerror;
+        }
         if (nfrag > 6) {
             st->exit_reason = ORC_MD_NFRAG6;
             break;
         }
-        if (nfrag > st->nfragexit) {
+        if (nfrag > nfragexit) {
             st->fragstate = 1;
             st->mdok = 1;
             st->exit_reason = ORC_MD_NFRAGEXIT;
@@ -443,7 +468,7 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
             st->exit_reason = ORC_MD_FCONST;
             break;
         }
-        if (nfrag >= st->nfragexit) {
+        if (nfrag >= nfragexit) {
             morestep = morestep + 1;
             if (morestep > more) {
                 st->fragstate = 1;

@@ -55,8 +55,8 @@ typedef int (*orc_egrad_fn)(void *ctx, int nat, const int32_t *iat, const double
                             double *qat);
 
 typedef struct {
-    /* inputs */
-    int nat, chrg, nfragexit, max_steps;
+    /* inputs; isec: fragmentation generation (main.F90:2228), 0 or 1 for the first */
+    int nat, chrg, nfragexit, max_steps, isec;
     double tstep, tadd, eimp, ax, ieetemp, etempin;
     /* outputs (md.f90:684-700) */
     int nstep, nfrag, fragstate, mdok, exit_reason;
@@ -79,6 +79,10 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass,
                    const double *mass, const double *velof, double *xyz, double *velo,
                    double *grad, double *achrg, int32_t *list, orc_egrad_fn egrad, void *ctx,
                    int steps_limit, double *axyz, double *avg_chrg);
+
+/* per-step records of orc_md_run on this thread (buf NULL: off) */
+#define ORC_MD_TRACE 8
+void orc_md_set_trace(double *buf, int cap);
 
 /* ---- GFN2-xTB single point with analytic gradient ---- */
 typedef struct {

@@ -1015,6 +1015,7 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
         tr[s].eimp = eimp[s];
         tr[s].tadd = tadd[s];
         tr[s].nfrag = 1;
+        tr[s].isec = 1;
         const double et = (params->etempin < 0.0)
                               ? host_setetemp(1, eimp[s], params->ax, params->ieetemp)
                               : params->etempin;
@@ -1075,6 +1076,21 @@ void qcxms_md_destroy(qcxms_md* md)
 }
 
 qcxms_batch* qcxms_md_batch(qcxms_md* md) { return md ? md->b : nullptr; }
+
+int qcxms_md_set_generation(qcxms_md* md, const int32_t* isec)
+{
+    if (!md || !isec) return fail("null argument");
+    for (int s = 0; s < md->ntraj; ++s)
+        if (isec[s] < 1) return fail("generation numbers start at 1");
+    qcxms_batch* b = md->b;
+    std::vector<qx::MdTraj> tr(md->ntraj);
+    HIPCHK(hipMemcpyAsync(tr.data(), md->m.traj, sizeof(qx::MdTraj) * md->ntraj, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int s = 0; s < md->ntraj; ++s) tr[s].isec = isec[s];
+    HIPCHK(hipMemcpyAsync(md->m.traj, tr.data(), sizeof(qx::MdTraj) * md->ntraj, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return QCXMS_OK;
+}
 
 }  // extern "C"
 

@@ -27,6 +27,7 @@ enum MdExit { kRunning = 0, kErrScf = 1, kErrDrift = 2, kNfrag6 = 3, kNfragExit 
 
 struct MdTraj {
     int nstep, nfrag, fconst, morestep, kdump, nadd, done, mdok, fragstate, exit_reason;
+    int isec;  // fragmentation generation (main.F90:2228): 1 for the first MD of a trajectory
     double ekin, epot, tav, epav, ekav, edum, eerror, ttime, dtime, atlast, ekinstart, etemp,
         eimp, fadd, tadd;
     double fragm[10], fragT[10];
@@ -323,10 +324,9 @@ QX_DEV void md_pre_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& p
         const int err1 = (t.epot == 0.0), err2 = (fabs(t.eerror) > (double)0.1f);
         s_exit = 0;
         if (err1 || err2) {
-            t.mdok = (t.nfrag > 1 && t.nfrag <= 4);
+            t.mdok = (t.nfrag > 1 && t.nfrag <= 4) || t.isec > 1;
             t.exit_reason = err1 ? kErrScf : kErrDrift;
             t.done = 1;
-            M.active[sys] = 0;
             s_exit = 1;
         } else if (t.kdump > 49) {
             t.kdump = 0;
@@ -335,7 +335,12 @@ QX_DEV void md_pre_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& p
         s_T = T;
     }
     __syncthreads();
-    if (s_exit) return;
+    // the flag is cleared only after every wavefront has passed the entry
+    // test above, so that all of them take the same barriers
+    if (s_exit) {
+        if (lane == 0) M.active[sys] = 0;
+        return;
+    }
     const bool reset = (t.kdump == 0);
     for (int i = lane; i < nat; i += 64) {
         const double prev = reset ? 0.0 : M.avchrg[off + i];
@@ -395,12 +400,13 @@ QX_DEV void md_post_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& 
     const int nfrag = dev_fragments(nat, M.num + off, B.xyz + 3 * (size_t)off, M.mass + off,
                                     M.list + off, fragm, adj);
     if (lane != 0) return;
-    for (int f = 0; f < 10; ++f) t.fragm[f] = fragm[f];
-    t.nfrag = nfrag;
-    // electronic temperature for the next egrad (md.f90:443-445)
+    // electronic temperature for the next egrad (md.f90:443-445): setetemp
+    // runs before fragment_structure, so it sees the previous step's nfrag
     const double dum = t.eimp - t.eimp * (double)(float)t.nstep / (double)(float)t.nadd;
     t.etemp = setetemp(t.nfrag, dum, p.ax, p.ieetemp);
     ((SysInfo*)B.info)[sys].kt = t.etemp * gfn2::kKtoAu;
+    for (int f = 0; f < 10; ++f) t.fragm[f] = fragm[f];
+    t.nfrag = nfrag;
     // fragment temperatures (md.f90:715-741)
     if (nfrag > 1) {
         double eint[10];
@@ -419,11 +425,13 @@ QX_DEV void md_post_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& 
         for (int f = 0; f < nfrag && f < 10; ++f)
             t.fragT[f] = eint[f] / ((double)(1.5f * (float)cnt[f]) * kKB);
     }
-    // exits (md.f90:458-462, 630-679)
+    // exits (md.f90:458-462, 630-679); later generations exit at two
+    // fragments (main.F90:2255)
+    const int nfragexit = t.isec > 1 ? 2 : p.nfragexit;
     int reason = kRunning;
     if (nfrag > 6) {
         reason = kNfrag6;
-    } else if (nfrag > p.nfragexit) {
+    } else if (nfrag > nfragexit) {
         t.fragstate = 1;
         t.mdok = 1;
         reason = kNfragExit;
@@ -434,7 +442,7 @@ QX_DEV void md_post_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& 
             t.fragstate = 2;
             t.mdok = 1;
             reason = kFconst;
-        } else if (nfrag >= p.nfragexit && (t.morestep = t.morestep + 1) > 250) {
+        } else if (nfrag >= nfragexit && (t.morestep = t.morestep + 1) > 250) {
             t.fragstate = 1;
             t.mdok = 1;
             reason = kMorestep;

@@ -248,6 +248,7 @@ def _md_sigs(L):
     L.qcxms_md_read_averages.argtypes = [C.c_void_p, _dp, _dp]
     L.qcxms_md_batch.restype = C.c_void_p
     L.qcxms_md_batch.argtypes = [C.c_void_p]
+    L.qcxms_md_set_generation.argtypes = [C.c_void_p, _ip]
     L.qcxms_fragments_batch.argtypes = [C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
     L.qcxms_leapfrog_batch.argtypes = [C.c_int32, _ip, _dp, _dp, C.c_double, _dp, _dp, _dp]
     L.qcxms_impactscale_batch.argtypes = [C.c_int32, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
@@ -259,7 +260,7 @@ class MdEnsemble:
 
     trajectories: list of dicts with num, xyz [nat,3] (Bohr), velo [nat,3]
     (au), mass [nat] (m_e), eimp (Eh), tadd (au); optional charge, imass,
-    velof.  lanes > 1 splits the trajectories into that many contiguous
+    velof, isec (fragmentation generation, default 1).  lanes > 1 splits the trajectories into that many contiguous
     groups, each an ensemble on its own stream, advanced together
     (qcxms_md_run_group) so that the device overlaps their kernels.  fused
     runs whole MD iterations per trajectory in one persistent launch.
@@ -307,6 +308,10 @@ class MdEnsemble:
                 self.close()
                 raise RuntimeError("qcxms_md_create failed: " + last_error())
             self._hs.append(C.c_void_p(h))
+            isec = np.array([int(t.get("isec", 1)) for t in part], dtype=np.int32)
+            if np.any(isec != 1) and L.qcxms_md_set_generation(self._hs[-1], _p(isec, _ip)):
+                self.close()
+                raise RuntimeError("qcxms_md_set_generation failed: " + last_error())
         self._h = self._hs[0]
 
     def close(self):

@@ -31,7 +31,7 @@ class Gfn2Info(C.Structure):
 
 
 class MdState(C.Structure):
-    _fields_ = ([(n, C.c_int) for n in ("nat", "chrg", "nfragexit", "max_steps")]
+    _fields_ = ([(n, C.c_int) for n in ("nat", "chrg", "nfragexit", "max_steps", "isec")]
                 + [(n, C.c_double) for n in ("tstep", "tadd", "eimp", "ax", "ieetemp", "etempin")]
                 + [(n, C.c_int) for n in ("nstep", "nfrag", "fragstate", "mdok", "exit_reason")]
                 + [(n, C.c_double) for n in ("tav", "epav", "ekav", "ttime", "dtime", "atlast",
@@ -67,6 +67,7 @@ def oracle():
                                  C.c_void_p, C.c_int]
         L.orc_md_run_avg.argtypes = [C.c_void_p, ip, ip, dp, dp, dp, dp, dp, dp, ip, EGRAD_FN,
                                      C.c_void_p, C.c_int, dp, dp]
+        L.orc_md_set_trace.argtypes = [dp, C.c_int]
         _oracle = L
     return _oracle
 
@@ -175,7 +176,8 @@ def impactscale(lib, prefix, velo, mass, velof, eimp, ff, e0):
 
 
 def md_run(traj, steps_limit, tstep=0.5 * 41.3413733365614, max_steps=2000, nfragexit=3,
-           etempin=-1.0, opts=None, charge=1):
+           etempin=-1.0, opts=None, charge=1, ieetemp=0.0, ax=0.0, isec=1, egrad=None,
+           trace_steps=0):
     """Oracle EI production MD (md.f90:285-682) with GFN2 forces for one
     trajectory dict (num, xyz, velo, mass, eimp, tadd[, velof, imass])."""
     L = oracle()
@@ -191,17 +193,27 @@ def md_run(traj, steps_limit, tstep=0.5 * 41.3413733365614, max_steps=2000, nfra
     lst = np.zeros(nat, dtype=np.int32)
     st = MdState()
     st.nat, st.chrg, st.nfragexit, st.max_steps = nat, int(charge), nfragexit, max_steps
+    st.isec = int(isec)
     st.tstep, st.tadd, st.eimp = tstep, float(traj["tadd"]), float(traj["eimp"])
-    st.ax, st.ieetemp, st.etempin = 0.0, 0.0, etempin
+    st.ax, st.ieetemp, st.etempin = float(ax), float(ieetemp), etempin
     o = opts if opts is not None else default_opts()
-    fn = C.cast(L.orc_gfn2_md_egrad, EGRAD_FN)
+    # egrad: optional Python callback (ctx, nat, iat, xyz, chrg, etemp, e, g, q)
+    # wrapped as EGRAD_FN, for tests that force an exit through the energy
+    fn = egrad if egrad is not None else C.cast(L.orc_gfn2_md_egrad, EGRAD_FN)
     axyz = np.zeros((nat, 3))
     achrg = np.zeros(nat)
+    # per-step records: nstep, etemp for the next egrad, epot, ekin, nfrag,
+    # fragT(1), fragT(2), eerror
+    trace = np.full((max(int(trace_steps), 1), 8), np.nan)
+    if trace_steps:
+        L.orc_md_set_trace(ptr(trace), int(trace_steps))
     L.orc_md_run_avg(C.byref(st), ptr(num, ip), ptr(imass, ip), ptr(mass), ptr(velof), ptr(xyz),
                      ptr(velo), ptr(grad), ptr(qat), ptr(lst, ip), fn,
                      C.cast(C.pointer(o), C.c_void_p), int(steps_limit), ptr(axyz), ptr(achrg))
+    if trace_steps:
+        L.orc_md_set_trace(None, 0)
     return dict(state=st, xyz=xyz, velo=velo, grad=grad, qat=qat, list=lst, axyz=axyz,
-                achrg=achrg)
+                achrg=achrg, trace=trace[:st.nstep] if trace_steps else None)
 
 
 def gfn2_egrad_batch(systems, xyz_list, etemp, opts=None, nthreads=None):
