@@ -1740,6 +1740,55 @@ int orc_gfn2_matrices(int nat, const int32_t *num, const double *xyz, double *s,
     return 0;
 }
 
+/* unit pinning: the ES2 kernel gamma (nsh x nsh, row-major) of a structure */
+int orc_gfn2_gamma(int nat, const int32_t *num, const double *xyz, double *gam, int *nsh_out)
+{
+    sys_t y;
+    int st = sys_init(&y, nat, num, xyz, 0, 300.0);
+    if (st) return st;
+    *nsh_out = y.nsh;
+    if (gam) {
+        build_geometry(&y);
+        memcpy(gam, y.gam, sizeof(double) * y.nsh * y.nsh);
+    }
+    sys_free(&y);
+    return 0;
+}
+
+/* unit pinning: the interaction energies of the SCC at given moments
+ * x = [q_sh (nsh), atomic dipoles (3 nat), traceless quadrupoles (6 nat)]:
+ * e[0..4] = ES2, ES3, AES, AXC, D4 two-body, and their derivatives with
+ * respect to x (vs, vd, vq; any may be NULL) */
+int orc_gfn2_interaction(int nat, const int32_t *num, const double *xyz, const double *x,
+                         double *e, double *vs, double *vd, double *vq)
+{
+    sys_t y;
+    int st = sys_init(&y, nat, num, xyz, 0, 300.0);
+    if (st) return st;
+    build_geometry(&y);
+    double *ws = (double *)malloc(sizeof(double) * (y.nsh + 9 * (size_t)nat));
+    eparts_t ep;
+    interaction(&y, x, ws, ws + y.nsh, ws + y.nsh + 3 * nat, &ep);
+    e[0] = ep.es2;
+    e[1] = ep.es3;
+    e[2] = ep.aes;
+    e[3] = ep.axc;
+    e[4] = ep.d4;
+    if (vs) memcpy(vs, ws, sizeof(double) * y.nsh);
+    if (vd) memcpy(vd, ws + y.nsh, sizeof(double) * 3 * nat);
+    if (vq) memcpy(vq, ws + y.nsh + 3 * nat, sizeof(double) * 6 * nat);
+    free(ws);
+    sys_free(&y);
+    return 0;
+}
+
+/* unit pinning: Fermi occupations f[n] (per spin) of orbital energies e[n]
+ * (ascending) for nalpha electrons per spin at kt; returns -T S of both spins */
+double orc_fermi_fill(int n, const double *e, double kt, double nalpha, double *f)
+{
+    return fermi(n, e, kt, nalpha, f);
+}
+
 /* ------------------------------------------------------------ batch driver */
 typedef struct {
     int nsys;

@@ -118,6 +118,10 @@ void orc_overlap_shellpair(int la, int ga, const double *aa, const double *ca,
 int  orc_gfn2_matrices(int nat, const int32_t *num, const double *xyz, double *s,
                        double *h0, double *dpint, double *qpint, int *nao_out);
 void orc_sygv(int n, double *f, double *s, double *eval, double *work);
+int  orc_gfn2_gamma(int nat, const int32_t *num, const double *xyz, double *gam, int *nsh);
+int  orc_gfn2_interaction(int nat, const int32_t *num, const double *xyz, const double *x,
+                          double *e, double *vs, double *vd, double *vq);
+double orc_fermi_fill(int n, const double *e, double kt, double nalpha, double *f);
 double orc_d4_twobody(int nat, const int32_t *num, const double *xyz, const double *q,
                       double *grad, double *dedq);
 double orc_d4_atm(int nat, const int32_t *num, const double *xyz, double *grad);

@@ -8,15 +8,18 @@ optimal coefficients solve a linear system; the exponents are optimised in
 log space.  Exponents scale with zeta^2 for other zeta.
 
 tblite builds its basis from Stewart's tabulated fits (tblite v0.2.1,
-slater_to_gauss); those tables are not in this container, so this script
-regenerates them from the published criterion.  tests/test_sto_ng.py checks
-the 1s results against Stewart's published 1s values.
+slater_to_gauss).  Wherever Stewart's printed values are held in
+tests/golden/stewart_1970.json they replace the fit, digit for digit (the
+shells GFN2 uses for H, C, N, O are among them: 1s with three terms, 2s and 2p
+with four); the fit fills the remaining entries.  tests/test_sto_ng.py pins the
+printed values: their coefficients are the least-squares coefficients of their
+exponents to the printed digits, and the overlap is stationary there.
 
-Writes the fitted table as data into:
-  tests/golden/sto_ng_fit.json
+Writes the table (published values, fits elsewhere) as data into
+(tests/golden/sto_ng_fit.json holds the fits; --refit rewrites it):
   qcxms_amd/csrc/sto_ng_table.h   (product)
   oracle/sto_ng_oracle.h          (CPU oracle)
-Run: python tests/golden/gen_sto_ng.py
+Run: python tests/golden/gen_sto_ng.py [--refit]   (--refit redoes the fits, slow)
 """
 import json
 import math
@@ -91,7 +94,16 @@ def fit(n, l, ng, start=None):
     return alpha, d, math.sqrt(ov2)
 
 
-def main():
+def published():
+    """Stewart's printed expansions, key 'nl_ng' -> (alpha, coeff)."""
+    with open(os.path.join(HERE, "stewart_1970.json")) as fh:
+        data = json.load(fh)
+    return {k: (v[0], v[1]) for k, v in data.items() if not k.startswith("_")}
+
+
+def refit():
+    """All least-squares fits (slow: minutes per entry set); writes
+    sto_ng_fit.json."""
     table = {}
     prev = {}
     for (n, l) in CASES:
@@ -108,10 +120,22 @@ def main():
     with open(os.path.join(HERE, "sto_ng_fit.json"), "w") as fh:
         json.dump(table, fh, indent=1)
 
+
+def main(do_refit=False):
+    if do_refit:
+        refit()
+    with open(os.path.join(HERE, "sto_ng_fit.json")) as fh:
+        table = json.load(fh)
+    for key, (alpha, coeff) in published().items():
+        table[key] = dict(table[key], alpha=list(alpha), coeff=list(coeff))
+        print(f"{key}: Stewart 1970")
+
     def header(guard, prefix, comment):
         lines = [f"/* {comment}",
-                 " * Least-squares STO-nG fits of Slater functions with zeta = 1",
-                 " * (Stewart 1970 criterion), generated by tests/golden/gen_sto_ng.py.",
+                 " * STO-nG expansions of Slater functions with zeta = 1: Stewart's printed",
+                 " * values (J. Chem. Phys. 52, 431 (1970)) where held in",
+                 " * tests/golden/stewart_1970.json, least-squares fits by his criterion",
+                 " * elsewhere; generated by tests/golden/gen_sto_ng.py.",
                  " * Index [n-1][l][ng-1]; exponents scale with zeta^2.  Coefficients",
                  " * multiply normalised primitives. */",
                  f"#ifndef {guard}", f"#define {guard}",
@@ -151,4 +175,5 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    main(do_refit="--refit" in sys.argv)
