@@ -79,5 +79,6 @@ def test_split_during_heating_per_step():
         assert r.etemp == row[1], (k, r.etemp, row[1])
         assert abs(r.epot - row[2]) < 1e-8, (k, r.epot, row[2])
         assert abs(r.ekin - row[3]) < 1e-9
-        np.testing.assert_allclose([r.fragT[0], r.fragT[1]], row[5:7], rtol=1e-8, atol=1e-9)
+        # 1e-5 K on a 4-atom fragment is 2e-10 Eh of kinetic energy
+        np.testing.assert_allclose([r.fragT[0], r.fragT[1]], row[5:7], rtol=1e-8, atol=1e-5)
     md.close()
