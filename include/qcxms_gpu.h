@@ -50,6 +50,23 @@ int qcxms_egrad_batch(int32_t nsys, const int32_t *offsets, const int32_t *num,
                       int32_t method, const double *etemp, double *qat, double *energy,
                       double *gradient, int32_t *stat);
 
+/* Allocation-free variant of qcxms_egrad_batch: same arguments, results and
+ * status codes.  No counterpart in the reference; it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Every device array of the call
+ * lives in one static device array of the library, 64 MiB per GPU, held
+ * until the process ends (about 1.5 MB per system of 26 atoms, 5.8 MB per
+ * system of 60 atoms: 44 and 11 systems);
+ * a batch that needs more is rejected with QCXMS_INVALID and a message.  It
+ * calls no hipMalloc, hipFree or stream creation and runs on the library's
+ * stream.  Calls in one process are serialised by the library (they share
+ * the array, which keeps the last call's contents); calls that overlapped on
+ * one GPU would race. */
+int qcxms_egrad_batch_noalloc(int32_t nsys, const int32_t *offsets, const int32_t *num,
+                              const double *xyz, const int32_t *charge,
+                              const int32_t *multiplicity, int32_t method, const double *etemp,
+                              double *qat, double *energy, double *gradient, int32_t *stat);
+
 /* ---- persistent batches: device-resident state for repeated evaluations --- */
 typedef struct qcxms_batch qcxms_batch;
 

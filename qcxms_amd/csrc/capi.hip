@@ -19,6 +19,22 @@
 __constant__ DevTables g_tab;
 __constant__ double g_rad[118];
 
+// Device memory of the allocation-free entry point qcxms_egrad_batch_noalloc:
+// the whole batch layout of one call is carved from this array.  Every GPU
+// the library runs on holds its own copy until the process ends.  Host code
+// never uses this name as an address: it takes the device address of the
+// array from hipGetSymbolAddress once per device (g_scratch_dev).
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// zero n doubles (a kernel rather than a memset node, so that callers may
+// capture the launch)
+__global__ __launch_bounds__(256) void k_fill_zero(double* p, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0;
+}
+
 #include "k_grad.h"
 #include "k_md.h"
 
@@ -30,6 +46,8 @@ bool g_ready = false;
 int g_device = -1;
 hipStream_t g_stream = nullptr;
 size_t g_scc_dyn_max = 0;  // dynamic LDS the SCC kernels may take (set at init)
+unsigned char* g_scratch_dev = nullptr;  // device address of scratch on g_device
+std::mutex g_scratch_mu;                 // one allocation-free call at a time
 size_t g_eig_dyn_max = 0;  // the same with the MFMA eigen refinement's fifth matrix
 DevTables g_host_tab;
 
@@ -180,6 +198,11 @@ int ensure_init(int device)
     for (int z = 0; z < 118; ++z) rad[z] = gfn2::kAatoAu * kQcxmsRadAa[z];
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_rad), rad, sizeof(rad)));
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    // the allocation-free entry point's array on this device
+    void* sp = nullptr;
+    HIPCHK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    if (!sp) return fail("no device address for the scratch array");
+    g_scratch_dev = (unsigned char*)sp;
     // dynamic LDS of the SCC kernels: whatever the workgroup's LDS leaves
     // beside the kernels' static LDS (at least the LDS-resident matrices of
     // kFastM basis functions)
@@ -245,12 +268,17 @@ struct qcxms_batch {
     // every batch (and the MD ensemble built on it) runs on a stream of its
     // own, so that the kernels of several ensembles overlap on the device
     hipStream_t stream = nullptr;
+    bool owns_stream = true;
+    // arena != null: device arrays are carved from [arena, arena + arena_cap)
+    // instead of allocated (the allocation-free entry point)
+    unsigned char* arena = nullptr;
+    size_t arena_used = 0, arena_cap = 0;
 
     ~qcxms_batch()
     {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : allocs) (void)hipFree(p);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && owns_stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -260,6 +288,16 @@ template <class T>
 int dalloc(qcxms_batch* b, T** p, size_t n)
 {
     if (n == 0) n = 1;
+    if (b->arena) {
+        // 256-byte aligned slices of the arena, zeroed together afterwards
+        const size_t off = (b->arena_used + 255) & ~(size_t)255, bytes = n * sizeof(T);
+        if (off + bytes > b->arena_cap)
+            return fail("the batch needs more than the " + std::to_string(b->arena_cap >> 20) +
+                        " MiB scratch of the allocation-free entry point; use qcxms_egrad_batch");
+        *p = (T*)(b->arena + off);
+        b->arena_used = off + bytes;
+        return QCXMS_OK;
+    }
     HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
     HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), b->stream));
     b->allocs.push_back((void*)*p);
@@ -383,12 +421,7 @@ int allocate(qcxms_batch* b)
     ALLOC(b->d_elem, b->natoms);
     int2* d_items = nullptr;
     ALLOC(d_items, b->items.size());
-    if (!b->items.empty())
-        HIPCHK(hipMemcpyAsync(d_items, b->items.data(), sizeof(int2) * b->items.size(), hipMemcpyHostToDevice,
-                              b->stream));
     d.spitem = d_items;
-    HIPCHK(hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * b->nsys, hipMemcpyHostToDevice, b->stream));
-    HIPCHK(hipMemcpyAsync(b->d_elem, b->elem.data(), sizeof(int) * b->natoms, hipMemcpyHostToDevice, b->stream));
     d.nsys = b->nsys;
     d.info = b->d_info;
     d.elem = b->d_elem;
@@ -447,6 +480,17 @@ int allocate(qcxms_batch* b)
     d.warm = warm;
     d.active = nullptr;
 #undef ALLOC
+    if (b->arena) {
+        const size_t nd = (b->arena_used + 7) / 8;
+        hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<size_t>((nd + 255) / 256, 4096)), dim3(256), 0,
+                           b->stream, (double*)b->arena, nd);
+        HIPCHK(hipGetLastError());
+    }
+    if (!b->items.empty())
+        HIPCHK(hipMemcpyAsync(d_items, b->items.data(), sizeof(int2) * b->items.size(), hipMemcpyHostToDevice,
+                              b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * b->nsys, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_elem, b->elem.data(), sizeof(int) * b->natoms, hipMemcpyHostToDevice, b->stream));
     return QCXMS_OK;
 }
 
@@ -549,7 +593,8 @@ int enqueue_egrad(qcxms_batch* b, bool with_grad)
 
 int launch_egrad(qcxms_batch* b, bool with_grad)
 {
-    if (!b->warm_start) HIPCHK(hipMemsetAsync(b->d.warm, 0, sizeof(int) * b->nsys, b->stream));
+    // (an arena batch lives for one call and starts zeroed: no warm flags)
+    if (!b->warm_start && !b->arena) HIPCHK(hipMemsetAsync(b->d.warm, 0, sizeof(int) * b->nsys, b->stream));
     int st = enqueue_egrad(b, with_grad);
     if (st) return st;
     return b->profiling ? collect_times() : QCXMS_OK;
@@ -877,9 +922,33 @@ int qcxms_batch_orbitals(qcxms_batch* b, int32_t sys, int32_t cap, int32_t* nao,
     return QCXMS_OK;
 }
 
-int qcxms_egrad_batch(int32_t nsys, const int32_t* offsets, const int32_t* num, const double* xyz,
-                      const int32_t* charge, const int32_t* multiplicity, int32_t method, const double* etemp,
-                      double* qat, double* energy, double* gradient, int32_t* stat)
+}  // extern "C"
+
+namespace {
+
+// a batch of one call whose device arrays live in the scratch array, run on
+// the library stream (no device allocation, no stream creation)
+qcxms_batch* arena_batch_create(int32_t nsys, const int32_t* offsets, const int32_t* num, const int32_t* charge)
+{
+    g_err.clear();
+    if (ensure_init(0) != QCXMS_OK) return nullptr;
+    qcxms_batch* b = new qcxms_batch();
+    b->stream = g_stream;
+    b->owns_stream = false;
+    b->arena = g_scratch_dev;
+    b->arena_cap = kScratchBytes;
+    if (layout(b, nsys, offsets, num, charge) != QCXMS_OK || allocate(b) != QCXMS_OK) {
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+// qcxms_egrad_batch and its allocation-free variant (arena != 0: every device
+// array of the call is carved from the scratch array)
+int egrad_batch_impl(bool arena, int32_t nsys, const int32_t* offsets, const int32_t* num, const double* xyz,
+                     const int32_t* charge, const int32_t* multiplicity, int32_t method, const double* etemp,
+                     double* qat, double* energy, double* gradient, int32_t* stat)
 {
     (void)multiplicity;  // uhf = min(mult-1, 0) = 0 for every multiplicity (tblite.f90:111)
     if (method != QCXMS_GFN2_XTB) {
@@ -917,7 +986,8 @@ int qcxms_egrad_batch(int32_t nsys, const int32_t* offsets, const int32_t* num, 
         pc[k] = charge ? charge[s] : 0;
         pe[k] = etemp ? etemp[s] : 300.0;
     }
-    qcxms_batch* b = qcxms_batch_create(nsys, po.data(), pn.data(), pc.data(), method);
+    qcxms_batch* b = arena ? arena_batch_create(nsys, po.data(), pn.data(), pc.data())
+                           : qcxms_batch_create(nsys, po.data(), pn.data(), pc.data(), method);
     if (!b) {
         const bool elem = g_err.find("no GFN2 parameters") != std::string::npos;
         if (stat)
@@ -940,6 +1010,28 @@ int qcxms_egrad_batch(int32_t nsys, const int32_t* offsets, const int32_t* num, 
                       gradient + 3 * (size_t)offsets[s]);
     }
     return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qcxms_egrad_batch(int32_t nsys, const int32_t* offsets, const int32_t* num, const double* xyz,
+                      const int32_t* charge, const int32_t* multiplicity, int32_t method, const double* etemp,
+                      double* qat, double* energy, double* gradient, int32_t* stat)
+{
+    return egrad_batch_impl(false, nsys, offsets, num, xyz, charge, multiplicity, method, etemp, qat, energy,
+                            gradient, stat);
+}
+
+int qcxms_egrad_batch_noalloc(int32_t nsys, const int32_t* offsets, const int32_t* num, const double* xyz,
+                              const int32_t* charge, const int32_t* multiplicity, int32_t method,
+                              const double* etemp, double* qat, double* energy, double* gradient, int32_t* stat)
+{
+    // the scratch array is shared by every call in this process: one call at a time
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return egrad_batch_impl(true, nsys, offsets, num, xyz, charge, multiplicity, method, etemp, qat, energy,
+                            gradient, stat);
 }
 
 int qcxms_get_xtb_egrad(int32_t nat, const int32_t* num, const double* xyz, int32_t charge, int32_t multiplicity,
