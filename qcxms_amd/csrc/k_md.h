@@ -492,20 +492,41 @@ __global__ __launch_bounds__(QX_SCC_BLOCK) void k_md_fused(BatchPtrs B, MdPtrs M
     const int sys = blockIdx.x;
     __shared__ FusedLds L;
     extern __shared__ double dyn[];
+#ifdef QX_PHASE_TIMING
+    // per-phase cycles of the step (slots 24..28: bookkeeping, setup, SCC,
+    // gradient, bookkeeping)
+    unsigned long long fz = clock64();
+#define QX_FUSED_PH(k)                                                                  \
+    do {                                                                                \
+        const unsigned long long t_ = clock64();                                        \
+        if (threadIdx.x == 0 && sys < kPhaseSys) g_phase[sys * kPhaseSlots + (k)] += t_ - fz; \
+        fz = t_;                                                                        \
+    } while (0)
+#else
+#define QX_FUSED_PH(k) \
+    do {               \
+    } while (0)
+#endif
     for (int step = 0; step < nsteps; ++step) {
         if (!M.active[sys]) break;
         md_pre_system(B, M, p, sys, L.m);
         __syncthreads();
+        QX_FUSED_PH(24);
         if (!M.active[sys]) break;
         setup_system(B, sys, 1, L.s, dyn);
         __syncthreads();
+        QX_FUSED_PH(25);
         scc_system<EIG>(B, opt, sys, L.c, dyn);
         __syncthreads();
+        QX_FUSED_PH(26);
         grad_system(B, sys, L.g);
         __syncthreads();
+        QX_FUSED_PH(27);
         md_post_system(B, M, p, sys, L.m);
         __syncthreads();
+        QX_FUSED_PH(28);
     }
+#undef QX_FUSED_PH
 }
 
 }  // namespace qx

@@ -37,7 +37,7 @@
 #define __forceinline__ inline
 #define __noinline__ __attribute__((noinline))
 #define __launch_bounds__(...)
-#define __align__(n) alignas(n)
+#define __align__(n) __attribute__((aligned(n)))
 #define HIP_SYMBOL(X) ((const void*)&(X))
 
 struct dim3 {
@@ -354,6 +354,12 @@ inline hipError_t hipMemcpyToSymbol(const void* sym, const void* src, size_t n, 
                                     hipMemcpyKind = hipMemcpyHostToDevice)
 {
     std::memcpy((char*)sym + off, src, n);
+    return hipSuccess;
+}
+// the address of a device symbol is its host address here
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym)
+{
+    *p = const_cast<void*>(sym);
     return hipSuccess;
 }
 inline hipError_t hipMemcpyFromSymbol(void* dst, const void* sym, size_t n, size_t off = 0,
