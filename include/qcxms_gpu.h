@@ -195,6 +195,60 @@ qcxms_batch *qcxms_md_batch(qcxms_md *md);
  * (src/md.f90:329).  Default 1. */
 int qcxms_md_set_generation(qcxms_md *md, const int32_t *isec);
 
+/* ---- fragmentation generations: the EI loop of src/main.F90:2208-2345 ----
+ * Each trajectory runs its MD (md.f90); when it ends with mdok,
+ * manage_fragments (write_fragments.f90:12-457) assigns fragment charges and
+ * picks the fragment tcont that the next generation continues with:
+ * its atoms recentred on their nuclear-charge-weighted centre (sum Z x / sum Z),
+ * velocities kept, no heating (velof = 0, tadd = 0, eimp = 0), isec + 1,
+ * the new total charge, and nmax shrunk as main.F90:2296-2320 does.  The
+ * ensemble advances in chunks of repack_every steps; after a chunk in which
+ * trajectories ended, the running trajectories and the new generations are
+ * laid out again on the device (state gathered in place, largest basis
+ * first). */
+typedef struct qcxms_ei qcxms_ei;
+
+/* One generation of one trajectory. */
+typedef struct {
+    int32_t traj;        /* trajectory index (creation order, from 0) */
+    int32_t isec;        /* generation, 1 for the first MD */
+    int32_t nat;         /* atoms of this generation */
+    int32_t nstep, exit_reason, mdok, fragstate, nfrag, max_steps;
+    int32_t tcont;       /* fragment continued by the next generation, 0: none */
+    int32_t mchrg;       /* total charge of this generation's MD */
+    int32_t nfrag_ok;    /* manage_fragments' verdict (0: more than 5 fragments) */
+    int32_t nres;        /* records in res */
+    int32_t continued;   /* 1 when a next generation was started */
+    int32_t asave_written; /* 1 when main writes asave to qcxms.res (the next
+                              generation failed, or the fragment had <= 3 atoms) */
+    int32_t pad;
+    double chrgcont;     /* statistical charge after manage_fragments */
+    double ttime, epot;
+    int32_t atoms[64];   /* the generation's atoms as indices into the first generation */
+    int32_t list[64];    /* their fragment labels at the end of the generation */
+    char asave[128];     /* record of the continued fragment */
+    char res[10][128];   /* qcxms.res records of this generation */
+} qcxms_ei_generation;
+
+/* ntraj trajectories as qcxms_md_create; maxsec: generation limit
+ * (input.f90:214, default 7); btf: Boltzmann temperature factor
+ * (input.f90:204, default 1); repack_every: MD steps between re-layouts
+ * (>= 1).  Trajectory t is numbered itrj = t + 1 in the records. */
+qcxms_ei *qcxms_ei_create(int32_t ntraj, const int32_t *offsets, const int32_t *num,
+                          const int32_t *charge, const double *mass, const int32_t *imass,
+                          const double *xyz, const double *velo, const double *velof,
+                          const double *eimp, const double *tadd, const qcxms_md_params *params,
+                          int32_t maxsec, double btf, int32_t repack_every);
+void qcxms_ei_destroy(qcxms_ei *ei);
+/* Advance by up to nsteps MD steps (every running generation); *running
+ * receives the number of trajectories still running (may be NULL). */
+int qcxms_ei_run(qcxms_ei *ei, int32_t nsteps, int32_t *running);
+/* Generations finished so far, in the order they finished; *count receives
+ * their number (at most cap are copied). */
+int qcxms_ei_generations(qcxms_ei *ei, int32_t cap, qcxms_ei_generation *out, int32_t *count);
+/* Re-layouts done so far. */
+int qcxms_ei_repacks(qcxms_ei *ei, int32_t *count);
+
 /* ---- batched drop-ins for single reference routines ---- */
 /* fragment_structure(nat,iat,xyz,3.0,1,0,list) + fragmass masses
  * (src/fragments.f90:10-182, as called at src/md.f90:450-451); systems with

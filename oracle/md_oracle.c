@@ -372,7 +372,8 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
         free(avxyz);
         return 0;
     }
-    double tav = 0, epav = 0, ekav = 0, edum = 0, eerror = 0, atlast = 0, dtime = 0, ttime = 0;
+    /* ttime carries over from an earlier generation (main.F90 passes it through md) */
+    double tav = 0, epav = 0, ekav = 0, edum = 0, eerror = 0, atlast = 0, dtime = 0, ttime = st->ttime;
     int nstep = 0, morestep = 0, fconst = 0, kdump = avdump, nfrag = 1;
     double fragT[10] = {0}, fragm[10] = {0};
     int32_t fragat[2000];

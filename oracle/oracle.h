@@ -58,7 +58,8 @@ typedef struct {
     /* inputs; isec: fragmentation generation (main.F90:2228), 0 or 1 for the first */
     int nat, chrg, nfragexit, max_steps, isec;
     double tstep, tadd, eimp, ax, ieetemp, etempin;
-    /* outputs (md.f90:684-700) */
+    /* outputs (md.f90:684-700); ttime is also an input: the time of earlier
+     * generations, to which this run adds */
     int nstep, nfrag, fragstate, mdok, exit_reason;
     double tav, epav, ekav, ttime, dtime, atlast, epot, ekin, eerror, etemp;
     double fragm[10], fragT[10];
@@ -155,6 +156,22 @@ int orc_manage_fragments(int nuc, const int32_t *iat, const double *xyz, const d
                          int32_t *list, int *nfrag_out, int *tcont_out, double *fragchrg3,
                          double *fragchrg, double *fragip, int *ipok, char *asave, char *res,
                          int *nres);
+
+/* ---- fragmentation generations of one EI trajectory (ei_oracle.c) ---- */
+typedef struct {
+    int isec, nat, nstep, exit_reason, mdok, fragstate, nfrag, max_steps, tcont, mchrg, nfrag_ok,
+        nres, continued, asave_written;
+    double chrgcont, ttime, epot;
+    int32_t atoms[64], list[64];
+    char asave[160];
+    char res[10][160];
+} orc_ei_gen;
+/* main.F90:2208-2345 for one trajectory; returns the number of generations
+ * written to gens (at most maxgen) */
+int orc_ei_run(int nat, const int32_t *iat, const int32_t *imass, const double *mass,
+               const double *velof, const double *xyz, const double *velo, int chrg, double eimp,
+               double tadd, double tstep, int nmax0, int nfragexit, int maxsec, double btf,
+               int itrj, const orc_scc_opts *opts, orc_ei_gen *gens, int maxgen);
 
 #ifdef __cplusplus
 }

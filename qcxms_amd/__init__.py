@@ -5,11 +5,11 @@ in include/qcxms_gpu.h; this package is its Python host layer, mirroring the
 reference's Fortran interface (src/tblite.f90 get_xtb_egrad, src/md.f90).
 """
 from .gpu import (EXIT_REASONS, GFN1_XTB, GFN2_XTB, IPEA1_XTB, Batch, FragmentReport,
-                  LibraryMissing, MdEnsemble, egrad_batch, egrad_many, fragment_ips_batch,
+                  EiEnsemble, LibraryMissing, MdEnsemble, egrad_batch, egrad_many, fragment_ips_batch,
                   fragments_batch, get_xtb_egrad, impactscale_batch, leapfrog_batch, library,
                   library_path, manage_fragments_batch)
 
 __all__ = ["EXIT_REASONS", "GFN1_XTB", "GFN2_XTB", "IPEA1_XTB", "Batch", "FragmentReport",
-           "LibraryMissing", "MdEnsemble", "egrad_batch", "egrad_many", "fragment_ips_batch",
+           "EiEnsemble", "LibraryMissing", "MdEnsemble", "egrad_batch", "egrad_many", "fragment_ips_batch",
            "fragments_batch", "get_xtb_egrad", "impactscale_batch", "leapfrog_batch", "library",
            "library_path", "manage_fragments_batch"]

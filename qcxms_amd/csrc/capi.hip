@@ -700,6 +700,148 @@ int check_offsets(int nsys, const int32_t* offsets, int maxnat)
     return QCXMS_OK;
 }
 
+// Host description of an MD ensemble to build: systems in launch order with
+// their atoms (offsets/num/charge), per-atom start data and MD records, and
+// for every system the system of an older ensemble whose state it takes
+// over (src >= 0; its host data here are then placeholders) or -1 (a new
+// system that starts with an initial energy and gradient, md.f90:182-184).
+struct MdLayout {
+    std::vector<int32_t> offsets, num, charge, imass, src;
+    std::vector<double> mass, xyz, velo, velof;
+    std::vector<qx::MdTraj> traj;
+};
+
+// MD record of a new trajectory generation (md.f90:88-260)
+qx::MdTraj new_md_record(const qcxms_md_params& p, double eimp, double tadd, int isec, int max_steps, double ttime)
+{
+    qx::MdTraj t;
+    std::memset(&t, 0, sizeof(t));
+    t.eimp = eimp;
+    t.tadd = tadd;
+    t.nfrag = 1;
+    t.isec = isec;
+    t.max_steps = max_steps;
+    t.ttime = ttime;
+    t.etemp = (p.etempin < 0.0) ? host_setetemp(1, eimp, p.ax, p.ieetemp) : p.etempin;
+    return t;
+}
+
+__global__ void k_set_active(int* active, const int* src, int n)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n && src[s] >= 0) active[s] = 1;
+}
+
+// Build an ensemble from L (see MdLayout); old supplies the state of the
+// systems with src >= 0 (gathered on the device by k_md_repack), params the
+// MD and SCC options when old is null.
+qcxms_md* md_build(const qcxms_md* old, const MdLayout& L, const qcxms_md_params* params)
+{
+    const int ntraj = (int)L.src.size();
+    qcxms_batch* b = qcxms_batch_create(ntraj, L.offsets.data(), L.num.data(), L.charge.data(), QCXMS_GFN2_XTB);
+    if (!b) return nullptr;
+    qcxms_md* md = new qcxms_md();
+    md->b = b;
+    md->ntraj = ntraj;
+    if (old) {
+        md->use_graph = old->use_graph;
+        md->fused = old->fused;
+        md->p = old->p;
+        b->opts = old->b->opts;
+        b->profiling = old->b->profiling;
+    } else {
+        md->use_graph = params->use_graph != 0;
+        md->fused = params->fused != 0;
+        b->opts.eig_mfma = params->eigensolver == 2 ? 0 : 1;
+        md->p.tstep = params->tstep;
+        md->p.etempin = params->etempin;
+        md->p.ax = params->ax;
+        md->p.ieetemp = params->ieetemp;
+        md->p.nfragexit = params->nfragexit;
+        md->p.max_steps = params->max_steps;
+        b->opts.keep = params->broyden_keep;
+        // warm-started MD steps start close to the fixed point: a lighter
+        // first mixing step (0.3 instead of 0.4) takes fewer SCC cycles
+        b->opts.damp = 0.3;
+        b->opts.extrapolate = params->extrapolate == 0 ? QX_DEFAULT_EXTRAPOLATE
+                                                       : (params->extrapolate < 0 ? 0 : params->extrapolate);
+        if (params->econv > 0.0) b->opts.econv = params->econv;
+        if (params->pconv > 0.0) b->opts.pconv = params->pconv;
+    }
+    b->warm_start = 1;
+    const int na = b->natoms;
+    std::vector<int> act(ntraj), nm(L.num.begin(), L.num.end()), im(L.imass.begin(), L.imass.end());
+    bool any_new = false;
+    for (int s = 0; s < ntraj; ++s) {
+        act[s] = L.src[s] < 0 ? 1 : 0;
+        any_new = any_new || act[s];
+        if (L.src[s] < 0) b->info[s].kt = L.traj[s].etemp * gfn2::kKtoAu;
+    }
+    std::vector<int> lst(na, 1);
+    int st = QCXMS_OK;
+    double* d_velof;
+    double* d_mass;
+    int* d_imass;
+    int* d_num;
+    int* d_src;
+    if ((st = mdalloc(md, &md->m.traj, ntraj, L.traj.data())) ||
+        (st = mdalloc(md, &md->m.velo, 3 * (size_t)na, L.velo.data())) ||
+        (st = mdalloc(md, &d_velof, (size_t)na, L.velof.data())) ||
+        (st = mdalloc(md, &d_mass, (size_t)na, L.mass.data())) ||
+        (st = mdalloc(md, &d_imass, (size_t)na, im.data())) || (st = mdalloc(md, &d_num, (size_t)na, nm.data())) ||
+        (st = mdalloc(md, &md->m.list, (size_t)na, lst.data())) || (st = mdalloc(md, &md->m.avchrg, (size_t)na)) ||
+        (st = mdalloc(md, &md->m.avxyz, 3 * (size_t)na)) ||
+        (st = mdalloc(md, &md->m.active, (size_t)ntraj, act.data())) ||
+        (st = mdalloc(md, &d_src, (size_t)ntraj, L.src.data()))) {
+        delete md;
+        return nullptr;
+    }
+    md->m.velof = d_velof;
+    md->m.mass = d_mass;
+    md->m.imass = d_imass;
+    md->m.num = d_num;
+    b->d.active = md->m.active;
+    if (hipMemcpyAsync(b->d.xyz, L.xyz.data(), sizeof(double) * 3 * na, hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * ntraj, hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess) {
+        g_err = "upload failed";
+        delete md;
+        return nullptr;
+    }
+    if (old) {
+        // the old ensemble's work must be done before its state is read
+        if (hipStreamSynchronize(old->b->stream) != hipSuccess) {
+            g_err = "synchronisation with the previous ensemble failed";
+            delete md;
+            return nullptr;
+        }
+        hipLaunchKernelGGL(qx::k_md_repack, dim3(ntraj), dim3(64), 0, b->stream, old->b->d, old->m, b->d, md->m,
+                           d_src, b->opts.keep > 0 ? 1 : 0, b->opts.mem);
+        if (hipGetLastError() != hipSuccess) {
+            g_err = "re-layout launch failed";
+            delete md;
+            return nullptr;
+        }
+    }
+    // the initial energy and gradient of the new systems only (md.f90:182)
+    if (any_new) {
+        if (enqueue_egrad(b, true) != QCXMS_OK) {
+            delete md;
+            return nullptr;
+        }
+        hipLaunchKernelGGL(qx::k_md_init, dim3(ntraj), dim3(64), 0, b->stream, b->d, md->m, md->p);
+    }
+    hipLaunchKernelGGL(k_set_active, dim3((ntraj + 255) / 256), dim3(256), 0, b->stream, md->m.active, d_src, ntraj);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
+        g_err = "initial energy/gradient failed on the device";
+        delete md;
+        return nullptr;
+    }
+    if (b->profiling) collect_times();
+    return md;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1061,103 +1203,34 @@ qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* 
         g_err = "tstep and max_steps must be positive";
         return nullptr;
     }
-    qcxms_batch* b = qcxms_batch_create(ntraj, offsets, num, charge, QCXMS_GFN2_XTB);
-    if (!b) return nullptr;
-    qcxms_md* md = new qcxms_md();
-    md->b = b;
-    md->ntraj = ntraj;
-    md->use_graph = params->use_graph != 0;
-    md->fused = params->fused != 0;
     if (params->eigensolver < 0 || params->eigensolver > 2) {
-        delete md;
         g_err = "eigensolver must be 0 (default), 1 (MFMA rotations, then Jacobi) or 2 (Jacobi)";
         return nullptr;
     }
-    b->opts.eig_mfma = params->eigensolver == 2 ? 0 : 1;
-    md->p.tstep = params->tstep;
-    md->p.etempin = params->etempin;
-    md->p.ax = params->ax;
-    md->p.ieetemp = params->ieetemp;
-    md->p.nfragexit = params->nfragexit;
-    md->p.max_steps = params->max_steps;
-    b->warm_start = 1;
-    if (params->broyden_keep < 0 || params->broyden_keep > b->opts.mem) {
-        delete md;
+    if (params->broyden_keep < 0 || params->broyden_keep > 100) {
         g_err = "broyden_keep must lie in [0, 100]";
         return nullptr;
     }
-    b->opts.keep = params->broyden_keep;
-    // warm-started MD steps start close to the fixed point: a lighter first
-    // mixing step (0.3 instead of 0.4) takes fewer SCC cycles per step
-    b->opts.damp = 0.3;
     if (params->extrapolate < -1 || params->extrapolate > 4) {
-        delete md;  // owns the batch
         g_err = "extrapolate must lie in -1..4";
         return nullptr;
     }
-    b->opts.extrapolate = params->extrapolate == 0 ? QX_DEFAULT_EXTRAPOLATE
-                                                   : (params->extrapolate < 0 ? 0 : params->extrapolate);
-    if (params->econv > 0.0) b->opts.econv = params->econv;
-    if (params->pconv > 0.0) b->opts.pconv = params->pconv;
-    const int na = b->natoms;
-    std::vector<qx::MdTraj> tr(ntraj);
-    std::vector<int> act(ntraj, 1);
-    for (int s = 0; s < ntraj; ++s) {
-        std::memset(&tr[s], 0, sizeof(qx::MdTraj));
-        tr[s].eimp = eimp[s];
-        tr[s].tadd = tadd[s];
-        tr[s].nfrag = 1;
-        tr[s].isec = 1;
-        const double et = (params->etempin < 0.0)
-                              ? host_setetemp(1, eimp[s], params->ax, params->ieetemp)
-                              : params->etempin;
-        tr[s].etemp = et;
-        b->info[s].kt = et * gfn2::kKtoAu;
-    }
-    std::vector<int> im(na, -1), nm(na);
-    for (int i = 0; i < na; ++i) {
-        if (imass) im[i] = imass[i];
-        nm[i] = num[i];
-    }
-    std::vector<int> lst(na, 1);
-    int st = QCXMS_OK;
-    double* d_velof;
-    double* d_mass;
-    int* d_imass;
-    int* d_num;
-    if ((st = mdalloc(md, &md->m.traj, ntraj, tr.data())) || (st = mdalloc(md, &md->m.velo, 3 * (size_t)na, velo)) ||
-        (st = mdalloc(md, &d_velof, (size_t)na, velof)) || (st = mdalloc(md, &d_mass, (size_t)na, mass)) ||
-        (st = mdalloc(md, &d_imass, (size_t)na, im.data())) || (st = mdalloc(md, &d_num, (size_t)na, nm.data())) ||
-        (st = mdalloc(md, &md->m.list, (size_t)na, lst.data())) || (st = mdalloc(md, &md->m.avchrg, (size_t)na)) ||
-        (st = mdalloc(md, &md->m.avxyz, 3 * (size_t)na)) ||
-        (st = mdalloc(md, &md->m.active, (size_t)ntraj, act.data()))) {
-        delete md;
-        return nullptr;
-    }
-    md->m.velof = d_velof;
-    md->m.mass = d_mass;
-    md->m.imass = d_imass;
-    md->m.num = d_num;
-    b->d.active = md->m.active;
-    if (hipMemcpyAsync(b->d.xyz, xyz, sizeof(double) * 3 * na, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
-        hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * ntraj, hipMemcpyHostToDevice, b->stream) !=
-            hipSuccess) {
-        g_err = "upload failed";
-        delete md;
-        return nullptr;
-    }
-    if (enqueue_egrad(b, true) != QCXMS_OK) {
-        delete md;
-        return nullptr;
-    }
-    hipLaunchKernelGGL(qx::k_md_init, dim3(ntraj), dim3(64), 0, b->stream, b->d, md->m, md->p);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
-        g_err = "initial energy/gradient failed on the device";
-        delete md;
-        return nullptr;
-    }
-    if (b->profiling) collect_times();
-    return md;
+    MdLayout L;
+    L.offsets.assign(offsets, offsets + ntraj + 1);
+    const int na = offsets[ntraj];
+    L.num.assign(num, num + na);
+    L.charge.resize(ntraj);
+    for (int s = 0; s < ntraj; ++s) L.charge[s] = charge ? charge[s] : 0;
+    L.mass.assign(mass, mass + na);
+    L.imass.assign(na, -1);
+    if (imass) L.imass.assign(imass, imass + na);
+    L.xyz.assign(xyz, xyz + 3 * (size_t)na);
+    L.velo.assign(velo, velo + 3 * (size_t)na);
+    L.velof.assign(velof, velof + na);
+    L.src.assign(ntraj, -1);
+    L.traj.resize(ntraj);
+    for (int s = 0; s < ntraj; ++s) L.traj[s] = new_md_record(*params, eimp[s], tadd[s], 1, params->max_steps, 0.0);
+    return md_build(nullptr, L, params);
 }
 
 void qcxms_md_destroy(qcxms_md* md)
@@ -1648,6 +1721,349 @@ int qcxms_manage_fragments_batch(int32_t ntraj, const int32_t* offsets, const in
             std::snprintf(dst, sizeof r.asave, "%s", rec.c_str());
         }
     }
+    return QCXMS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Fragmentation generations (src/main.F90:2208-2345, EI branch) for many
+// trajectories, with the ensemble laid out again after every chunk of steps
+// in which trajectories ended.
+
+struct qcxms_ei {
+    qcxms_md* md = nullptr;
+    MdLayout L;  // host mirror of md's layout (num, charge, mass, imass)
+    std::vector<int> slot;  // trajectory of each system of md
+    qcxms_md_params params{};
+    int maxsec = 7, repack = 100, repacks = 0;
+    double btf = 1.0;
+    struct Traj {
+        int isec = 1, nmax = 0, mchrg = 1, nfrag_ok = 1, running = 1;
+        double chrgcont = 1.0;
+        int last_gen = -1;        // index in gens of the last finished generation
+        std::vector<int> atoms;   // current atoms as indices into the first generation
+    };
+    std::vector<Traj> tr;
+    std::vector<qcxms_ei_generation> gens;
+    ~qcxms_ei() { delete md; }
+};
+
+namespace {
+
+// one chunk's bookkeeping: generations that ended, manage_fragments for the
+// finished runs, the next generations, and the new layout
+int ei_after_chunk(qcxms_ei* ei)
+{
+    qcxms_md* md = ei->md;
+    const int n = md->ntraj, na = md->b->natoms;
+    std::vector<qcxms_md_result> res(n);
+    std::vector<double> xyz(3 * (size_t)na), velo(3 * (size_t)na), axyz(3 * (size_t)na), achrg(na);
+    int st;
+    if ((st = qcxms_md_read(md, xyz.data(), velo.data(), nullptr, nullptr, nullptr, res.data()))) return st;
+    std::vector<int> fin, keep;
+    for (int s = 0; s < n; ++s) {
+        if (!ei->tr[ei->slot[s]].running) continue;  // ended in an earlier chunk
+        (res[s].running ? keep : fin).push_back(s);
+    }
+    if (fin.empty()) return QCXMS_OK;
+    if ((st = qcxms_md_read_averages(md, axyz.data(), achrg.data()))) return st;
+    const MdLayout& L = ei->L;
+    // generation records
+    std::vector<int> gidx(n, -1);
+    for (int s : fin) {
+        qcxms_ei::Traj& t = ei->tr[ei->slot[s]];
+        qcxms_ei_generation g;
+        std::memset(&g, 0, sizeof(g));
+        const qcxms_md_result& r = res[s];
+        g.traj = ei->slot[s];
+        g.isec = t.isec;
+        g.nat = L.offsets[s + 1] - L.offsets[s];
+        g.nstep = r.nstep;
+        g.exit_reason = r.exit_reason;
+        g.mdok = r.mdok;
+        g.fragstate = r.fragstate;
+        g.nfrag = r.nfrag;
+        g.max_steps = t.nmax;
+        g.mchrg = t.mchrg;
+        g.nfrag_ok = t.nfrag_ok;
+        g.chrgcont = t.chrgcont;
+        g.ttime = r.ttime;
+        g.epot = r.epot;
+        for (int i = 0; i < g.nat && i < 64; ++i) g.atoms[i] = t.atoms[i];
+        if (!r.mdok) {
+            // the run is not counted; the record of the fragment this run
+            // continued goes to qcxms.res (main.F90:2340-2345)
+            if (t.last_gen >= 0 && ei->gens[t.last_gen].asave[0]) ei->gens[t.last_gen].asave_written = 1;
+            t.running = 0;
+        }
+        gidx[s] = (int)ei->gens.size();
+        t.last_gen = gidx[s];
+        ei->gens.push_back(g);
+    }
+    // manage_fragments for the counted runs, one call per generation number
+    std::vector<int> isecs;
+    for (int s : fin)
+        if (res[s].mdok && std::find(isecs.begin(), isecs.end(), ei->tr[ei->slot[s]].isec) == isecs.end())
+            isecs.push_back(ei->tr[ei->slot[s]].isec);
+    std::vector<int> cont;  // finished systems whose trajectory goes on
+    std::vector<qcxms_fragment_report> rep(n);
+    for (int isec : isecs) {
+        std::vector<int> grp;
+        for (int s : fin)
+            if (res[s].mdok && ei->tr[ei->slot[s]].isec == isec) grp.push_back(s);
+        std::vector<int32_t> o{0}, nm, im, itrj, lst;
+        std::vector<double> x, ax, q, ms, atl;
+        std::vector<qcxms_fragment_report> rr(grp.size());
+        for (size_t k = 0; k < grp.size(); ++k) {
+            const int s = grp[k];
+            const qcxms_ei::Traj& t = ei->tr[ei->slot[s]];
+            for (int i = L.offsets[s]; i < L.offsets[s + 1]; ++i) {
+                nm.push_back(L.num[i]);
+                im.push_back(L.imass[i]);
+                ms.push_back(L.mass[i]);
+                q.push_back(achrg[i]);
+                for (int c = 0; c < 3; ++c) {
+                    x.push_back(xyz[3 * (size_t)i + c]);
+                    ax.push_back(axyz[3 * (size_t)i + c]);
+                }
+            }
+            o.push_back((int32_t)nm.size());
+            itrj.push_back(ei->slot[s] + 1);
+            atl.push_back(res[s].atlast);
+            std::memset(&rr[k], 0, sizeof(rr[k]));
+            rr[k].mchrg = t.mchrg;
+            rr[k].nfrag_ok = t.nfrag_ok;
+            rr[k].chrgcont = t.chrgcont;
+        }
+        lst.assign(nm.size(), 0);
+        if ((st = qcxms_manage_fragments_batch((int32_t)grp.size(), o.data(), nm.data(), x.data(), ax.data(), q.data(),
+                                               ms.data(), im.data(), atl.data(), itrj.data(), isec, ei->maxsec,
+                                               ei->btf, lst.data(), rr.data())))
+            return st;
+        for (size_t k = 0; k < grp.size(); ++k) {
+            const int s = grp[k];
+            qcxms_ei::Traj& t = ei->tr[ei->slot[s]];
+            const qcxms_fragment_report& r = rr[k];
+            rep[s] = r;
+            qcxms_ei_generation& g = ei->gens[gidx[s]];
+            g.tcont = r.tcont;
+            g.mchrg = t.mchrg;  // the MD's charge; the report carries the next one
+            g.nfrag_ok = r.nfrag_ok;
+            g.chrgcont = r.chrgcont;
+            g.nres = r.nres;
+            std::memcpy(g.asave, r.asave, sizeof(g.asave));
+            for (int j = 0; j < r.nres && j < 10; ++j) std::memcpy(g.res[j], r.res[j], sizeof(g.res[j]));
+            for (int i = 0; i < g.nat && i < 64; ++i) g.list[i] = lst[o[k] + i];
+            t.mchrg = r.mchrg;
+            t.chrgcont = r.chrgcont;
+            t.nfrag_ok = r.nfrag_ok;
+            if (!r.nfrag_ok || !(r.tcont > 0 && ei->maxsec > 0)) {
+                t.running = 0;
+                continue;
+            }
+            cont.push_back(s);
+        }
+    }
+    // next generations (main.F90:2290-2345)
+    struct NewSys {
+        int traj;
+        std::vector<int32_t> num, imass, atoms;
+        std::vector<double> mass, xyz, velo;
+        qx::MdTraj rec;
+        int charge;
+    };
+    std::vector<NewSys> ns;
+    const int nmax0 = ei->params.max_steps;
+    for (int s : cont) {
+        qcxms_ei::Traj& t = ei->tr[ei->slot[s]];
+        qcxms_ei_generation& g = ei->gens[gidx[s]];
+        int nmax = t.nmax - nmax0 / 5;
+        nmax = std::max(nmax, nmax0 / 2);
+        if (res[s].fragstate == 2) nmax = nmax0 / 3;
+        NewSys y;
+        y.traj = ei->slot[s];
+        double dum = 0.0, cema[3] = {0.0, 0.0, 0.0};
+        for (int i = L.offsets[s], a = 0; i < L.offsets[s + 1]; ++i, ++a) {
+            if (g.list[a] != g.tcont) continue;
+            y.num.push_back(L.num[i]);
+            y.imass.push_back(L.imass[i]);
+            y.mass.push_back(L.mass[i]);
+            y.atoms.push_back(t.atoms[a]);
+            for (int c = 0; c < 3; ++c) {
+                y.xyz.push_back(xyz[3 * (size_t)i + c]);
+                y.velo.push_back(velo[3 * (size_t)i + c]);
+            }
+            dum = dum + (double)L.num[i];
+            for (int c = 0; c < 3; ++c) cema[c] = cema[c] + xyz[3 * (size_t)i + c] * (double)L.num[i];
+        }
+        for (int c = 0; c < 3; ++c) cema[c] = cema[c] / dum;
+        const int nuc = (int)y.num.size();
+        if (nuc <= 10) nmax = nmax0 / 10;
+        if (nuc <= 3) {
+            // run not continued because of small fragment(s)
+            g.asave_written = g.asave[0] ? 1 : 0;
+            t.running = 0;
+            continue;
+        }
+        for (int i = 0; i < nuc; ++i)
+            for (int c = 0; c < 3; ++c) y.xyz[3 * (size_t)i + c] = y.xyz[3 * (size_t)i + c] - cema[c];
+        g.continued = 1;
+        t.isec += 1;
+        t.nmax = nmax;
+        t.atoms = y.atoms;
+        y.charge = t.mchrg;
+        y.rec = new_md_record(ei->params, 0.0, 0.0, t.isec, nmax, res[s].ttime);
+        ns.push_back(std::move(y));
+    }
+    // the new layout: running systems (state gathered on the device) and the
+    // new generations, largest basis first
+    struct Item {
+        int nao, traj, kept_slot, new_idx;
+    };
+    std::vector<Item> items;
+    auto nao_of = [](const int32_t* z, int cnt) {
+        int k = 0;
+        for (int i = 0; i < cnt; ++i) k += z[i] == 1 ? 1 : 4;
+        return k;
+    };
+    for (int s : keep)
+        items.push_back({nao_of(L.num.data() + L.offsets[s], L.offsets[s + 1] - L.offsets[s]), ei->slot[s], s, -1});
+    for (size_t k = 0; k < ns.size(); ++k)
+        items.push_back({nao_of(ns[k].num.data(), (int)ns[k].num.size()), ns[k].traj, -1, (int)k});
+    if (items.empty()) {
+        // every trajectory has ended: the last layout stays for reading
+        return QCXMS_OK;
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) {
+        return a.nao != c.nao ? a.nao > c.nao : a.traj < c.traj;
+    });
+    MdLayout N;
+    N.offsets.push_back(0);
+    std::vector<int> slot;
+    for (const Item& it : items) {
+        slot.push_back(it.traj);
+        if (it.kept_slot >= 0) {
+            const int s = it.kept_slot;
+            const int a0 = L.offsets[s], a1 = L.offsets[s + 1];
+            N.num.insert(N.num.end(), L.num.begin() + a0, L.num.begin() + a1);
+            N.imass.insert(N.imass.end(), L.imass.begin() + a0, L.imass.begin() + a1);
+            N.mass.insert(N.mass.end(), L.mass.begin() + a0, L.mass.begin() + a1);
+            N.xyz.insert(N.xyz.end(), 3 * (size_t)(a1 - a0), 0.0);
+            N.velo.insert(N.velo.end(), 3 * (size_t)(a1 - a0), 0.0);
+            N.velof.insert(N.velof.end(), (size_t)(a1 - a0), 0.0);
+            N.charge.push_back(L.charge[s]);
+            N.src.push_back(s);
+            qx::MdTraj dummy;
+            std::memset(&dummy, 0, sizeof(dummy));
+            N.traj.push_back(dummy);
+        } else {
+            const NewSys& y = ns[it.new_idx];
+            N.num.insert(N.num.end(), y.num.begin(), y.num.end());
+            N.imass.insert(N.imass.end(), y.imass.begin(), y.imass.end());
+            N.mass.insert(N.mass.end(), y.mass.begin(), y.mass.end());
+            N.xyz.insert(N.xyz.end(), y.xyz.begin(), y.xyz.end());
+            N.velo.insert(N.velo.end(), y.velo.begin(), y.velo.end());
+            N.velof.insert(N.velof.end(), y.num.size(), 0.0);
+            N.charge.push_back(y.charge);
+            N.src.push_back(-1);
+            N.traj.push_back(y.rec);
+        }
+        N.offsets.push_back((int32_t)N.num.size());
+    }
+    qcxms_md* nmd = md_build(md, N, nullptr);
+    if (!nmd) return QCXMS_INVALID;
+    delete md;
+    ei->md = nmd;
+    ei->L = std::move(N);
+    ei->slot = std::move(slot);
+    ei->repacks += 1;
+    return QCXMS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+qcxms_ei* qcxms_ei_create(int32_t ntraj, const int32_t* offsets, const int32_t* num, const int32_t* charge,
+                          const double* mass, const int32_t* imass, const double* xyz, const double* velo,
+                          const double* velof, const double* eimp, const double* tadd, const qcxms_md_params* params,
+                          int32_t maxsec, double btf, int32_t repack_every)
+{
+    g_err.clear();
+    if (repack_every < 1 || maxsec < 0 || !(btf > 0.0)) {
+        g_err = "repack_every must be >= 1, maxsec >= 0 and btf > 0";
+        return nullptr;
+    }
+    qcxms_md* md = qcxms_md_create(ntraj, offsets, num, charge, mass, imass, xyz, velo, velof, eimp, tadd, params);
+    if (!md) return nullptr;
+    qcxms_ei* ei = new qcxms_ei();
+    ei->md = md;
+    ei->params = *params;
+    ei->maxsec = maxsec;
+    ei->btf = btf;
+    ei->repack = repack_every;
+    const int na = offsets[ntraj];
+    MdLayout& L = ei->L;
+    L.offsets.assign(offsets, offsets + ntraj + 1);
+    L.num.assign(num, num + na);
+    L.imass.assign(na, -1);
+    if (imass) L.imass.assign(imass, imass + na);
+    L.mass.assign(mass, mass + na);
+    L.charge.resize(ntraj);
+    ei->tr.resize(ntraj);
+    for (int t = 0; t < ntraj; ++t) {
+        L.charge[t] = charge ? charge[t] : 0;
+        ei->slot.push_back(t);
+        qcxms_ei::Traj& T = ei->tr[t];
+        T.nmax = params->max_steps;
+        T.mchrg = L.charge[t];
+        T.chrgcont = (double)T.mchrg;
+        for (int i = 0; i < offsets[t + 1] - offsets[t]; ++i) T.atoms.push_back(i);
+    }
+    return ei;
+}
+
+void qcxms_ei_destroy(qcxms_ei* ei)
+{
+    if (!ei) return;
+    if (ei->md) (void)hipStreamSynchronize(ei->md->b->stream);
+    delete ei;
+}
+
+int qcxms_ei_run(qcxms_ei* ei, int32_t nsteps, int32_t* running)
+{
+    if (!ei || !ei->md) return fail("null ensemble");
+    int left = nsteps, st;
+    while (left > 0) {
+        int anyrun = 0;
+        for (const auto& t : ei->tr) anyrun |= t.running;
+        if (!anyrun) break;
+        const int chunk = std::min(left, ei->repack);
+        if ((st = qcxms_md_run(ei->md, chunk, nullptr))) return st;
+        left -= chunk;
+        if ((st = ei_after_chunk(ei))) return st;
+    }
+    if (running) {
+        int nr = 0;
+        for (const auto& t : ei->tr) nr += t.running;
+        *running = nr;
+    }
+    return QCXMS_OK;
+}
+
+int qcxms_ei_generations(qcxms_ei* ei, int32_t cap, qcxms_ei_generation* out, int32_t* count)
+{
+    if (!ei || !count) return fail("null argument");
+    *count = (int32_t)ei->gens.size();
+    for (int k = 0; k < cap && k < (int)ei->gens.size(); ++k) out[k] = ei->gens[k];
+    return QCXMS_OK;
+}
+
+int qcxms_ei_repacks(qcxms_ei* ei, int32_t* count)
+{
+    if (!ei || !count) return fail("null argument");
+    *count = ei->repacks;
     return QCXMS_OK;
 }
 

@@ -28,6 +28,7 @@ enum MdExit { kRunning = 0, kErrScf = 1, kErrDrift = 2, kNfrag6 = 3, kNfragExit 
 struct MdTraj {
     int nstep, nfrag, fconst, morestep, kdump, nadd, done, mdok, fragstate, exit_reason;
     int isec;  // fragmentation generation (main.F90:2228): 1 for the first MD of a trajectory
+    int max_steps;  // nmax of this generation (main.F90:2296-2320)
     double ekin, epot, tav, epav, ekav, edum, eerror, ttime, dtime, atlast, ekinstart, etemp,
         eimp, fadd, tadd;
     double fragm[10], fragT[10];
@@ -257,6 +258,7 @@ QX_DEV int dev_fragments(int nat, const int* num, const double* x, const double*
 __global__ __launch_bounds__(64) void k_md_init(BatchPtrs B, MdPtrs M, MdParams p)
 {
     const int sys = blockIdx.x;
+    if (!M.active[sys]) return;
     const SysInfo si = B.info[sys];
     const int lane = threadIdx.x, nat = si.nat, off = si.atom_off;
     __shared__ double term[QX_MAX_NAT];
@@ -281,6 +283,59 @@ __global__ __launch_bounds__(64) void k_md_init(BatchPtrs B, MdPtrs M, MdParams 
         t.done = 1;
         t.exit_reason = kErrScf;
         M.active[sys] = 0;
+    }
+}
+
+// Re-layout of a running ensemble: system s of the new batch (Bn, Mn) takes
+// the whole state of system src[s] of the old one (Bo, Mo), the same
+// molecule at another place: coordinates, gradient, charges, velocities,
+// heating factors, masses, element data, fragment labels, running averages,
+// the MD record, the electronic temperature, and the SCC's warm start
+// (moments of the last five steps, orthogonal basis U, warm flag; the
+// Broyden history when keep_hist).  src[s] < 0: a new system, left alone.
+__global__ __launch_bounds__(64) void k_md_repack(BatchPtrs Bo, MdPtrs Mo, BatchPtrs Bn, MdPtrs Mn, const int* src,
+                                                  int keep_hist, int mem)
+{
+    const int s = blockIdx.x, o = src[s];
+    if (o < 0) return;
+    const SysInfo sn = Bn.info[s], so = Bo.info[o];
+    const int lane = threadIdx.x, nat = sn.nat;
+    const size_t an = sn.atom_off, ao = so.atom_off;
+    for (int i = lane; i < nat; i += 64) {
+        Bn.qat[an + i] = Bo.qat[ao + i];
+        ((double*)Mn.velof)[an + i] = Mo.velof[ao + i];
+        ((double*)Mn.mass)[an + i] = Mo.mass[ao + i];
+        ((int*)Mn.imass)[an + i] = Mo.imass[ao + i];
+        ((int*)Mn.num)[an + i] = Mo.num[ao + i];
+        Mn.list[an + i] = Mo.list[ao + i];
+        Mn.avchrg[an + i] = Mo.avchrg[ao + i];
+    }
+    for (int i = lane; i < 3 * nat; i += 64) {
+        Bn.xyz[3 * an + i] = Bo.xyz[3 * ao + i];
+        Bn.grad[3 * an + i] = Bo.grad[3 * ao + i];
+        Mn.velo[3 * an + i] = Mo.velo[3 * ao + i];
+        Mn.avxyz[3 * an + i] = Mo.avxyz[3 * ao + i];
+    }
+    for (int i = lane; i < sn.nx; i += 64) {
+        Bn.x[sn.x_off + i] = Bo.x[so.x_off + i];
+        Bn.xo[sn.x_off + i] = Bo.xo[so.x_off + i];
+        Bn.xprev[sn.x_off + i] = Bo.xprev[so.x_off + i];
+        Bn.xprev2[sn.x_off + i] = Bo.xprev2[so.x_off + i];
+        Bn.xprev3[sn.x_off + i] = Bo.xprev3[so.x_off + i];
+        Bn.xprev4[sn.x_off + i] = Bo.xprev4[so.x_off + i];
+    }
+    const int ld2 = sn.ld * sn.ld;
+    for (int i = lane; i < ld2; i += 64) Bn.U[sn.mat_off + i] = Bo.U[so.mat_off + i];
+    if (keep_hist) {
+        const long long nh = 2LL * mem * sn.nx + 2LL * mem * mem + 2LL * mem + 2LL * sn.nx + 8;
+        for (long long i = lane; i < nh; i += 64) Bn.hist[sn.hist_off + i] = Bo.hist[so.hist_off + i];
+    }
+    if (lane == 0) {
+        Mn.traj[s] = Mo.traj[o];
+        Bn.warm[s] = Bo.warm[o];
+        Bn.iters_total[s] = Bo.iters_total[o];
+        Bn.energy[s] = Bo.energy[o];
+        ((SysInfo*)Bn.info)[s].kt = so.kt;
     }
 }
 
@@ -446,7 +501,7 @@ QX_DEV void md_post_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& 
             t.fragstate = 1;
             t.mdok = 1;
             reason = kMorestep;
-        } else if (t.nstep >= p.max_steps) {
+        } else if (t.nstep >= t.max_steps) {
             t.fragstate = 1;
             t.mdok = 1;
             reason = kMaxStep;

@@ -400,6 +400,99 @@ class MdEnsemble:
         return np.concatenate(out)
 
 
+class EiGeneration(C.Structure):
+    _fields_ = ([(n, C.c_int32) for n in ("traj", "isec", "nat", "nstep", "exit_reason", "mdok",
+                                          "fragstate", "nfrag", "max_steps", "tcont", "mchrg",
+                                          "nfrag_ok", "nres", "continued", "asave_written", "pad")]
+                + [(n, C.c_double) for n in ("chrgcont", "ttime", "epot")]
+                + [("atoms", C.c_int32 * 64), ("list", C.c_int32 * 64), ("asave", C.c_char * 128),
+                   ("res", (C.c_char * 128) * 10)])
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_
+             if f not in ("atoms", "list", "asave", "res", "pad")}
+        d["atoms"] = np.array(self.atoms[:self.nat], dtype=np.int32)
+        d["list"] = np.array(self.list[:self.nat], dtype=np.int32)
+        d["asave"] = self.asave.decode()
+        d["res"] = [self.res[k].value.decode() for k in range(self.nres)]
+        return d
+
+
+class EiEnsemble:
+    """Fragmentation generations of many EI trajectories (src/main.F90:
+    2208-2345) on the GPU: each trajectory's MD, then manage_fragments, then
+    the continued fragment as the next generation, with the ensemble laid out
+    again on the device every `repack` steps in which trajectories ended.
+    Trajectory dicts and MD options as MdEnsemble."""
+
+    def __init__(self, trajectories, tstep=0.5 * 41.3413733365614, max_steps=2000, nfragexit=3,
+                 etempin=-1.0, ax=0.0, ieetemp=0.0, use_graph=True, econv=0.0, pconv=0.0,
+                 extrapolate=0, broyden_keep=0, fused=False, eigensolver=0, maxsec=7, btf=1.0,
+                 repack=100):
+        L = library()
+        _md_sigs(L)
+        if not getattr(L, "_ei_sigs", False):
+            L.qcxms_ei_create.restype = C.c_void_p
+            L.qcxms_ei_create.argtypes = [C.c_int32, _ip, _ip, _ip, _dp, _ip, _dp, _dp, _dp, _dp, _dp,
+                                          C.POINTER(MdParams), C.c_int32, C.c_double, C.c_int32]
+            L.qcxms_ei_destroy.argtypes = [C.c_void_p]
+            L.qcxms_ei_run.argtypes = [C.c_void_p, C.c_int32, _ip]
+            L.qcxms_ei_generations.argtypes = [C.c_void_p, C.c_int32, C.POINTER(EiGeneration), _ip]
+            L.qcxms_ei_repacks.argtypes = [C.c_void_p, _ip]
+            L._ei_sigs = True
+        t = trajectories
+        nums = [np.asarray(x["num"], dtype=np.int32) for x in t]
+        off = np.cumsum([0] + [len(n) for n in nums]).astype(np.int32)
+        cat = lambda key, dt, default=None: np.ascontiguousarray(np.concatenate(
+            [np.asarray(x.get(key, default(x) if default else None), dtype=dt).reshape(-1) for x in t]))
+        num = np.concatenate(nums).astype(np.int32)
+        charge = np.array([int(x.get("charge", 1)) for x in t], dtype=np.int32)
+        p = MdParams(tstep, etempin, ax, ieetemp, nfragexit, max_steps, int(bool(use_graph)),
+                     int(extrapolate), econv, pconv, int(broyden_keep), int(bool(fused)),
+                     int(eigensolver))
+        h = L.qcxms_ei_create(len(t), _p(off, _ip), _p(num, _ip), _p(charge, _ip),
+                              _p(cat("mass", np.float64)),
+                              _p(cat("imass", np.int32, lambda x: np.full(len(x["num"]), -1)), _ip),
+                              _p(cat("xyz", np.float64)), _p(cat("velo", np.float64)),
+                              _p(cat("velof", np.float64, lambda x: np.ones(len(x["num"])))),
+                              _p(np.array([float(x["eimp"]) for x in t])),
+                              _p(np.array([float(x["tadd"]) for x in t])), C.byref(p), int(maxsec),
+                              float(btf), int(repack))
+        if not h:
+            raise RuntimeError("qcxms_ei_create failed: " + last_error())
+        self._h = C.c_void_p(h)
+        self.ntraj = len(t)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().qcxms_ei_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps):
+        running = C.c_int32()
+        if library().qcxms_ei_run(self._h, int(nsteps), C.byref(running)):
+            raise RuntimeError("qcxms_ei_run failed: " + last_error())
+        return running.value
+
+    def generations(self):
+        n = C.c_int32()
+        library().qcxms_ei_generations(self._h, 0, None, C.byref(n))
+        buf = (EiGeneration * max(n.value, 1))()
+        library().qcxms_ei_generations(self._h, n.value, buf, C.byref(n))
+        return [buf[k].as_dict() for k in range(n.value)]
+
+    def repacks(self):
+        n = C.c_int32()
+        library().qcxms_ei_repacks(self._h, C.byref(n))
+        return n.value
+
+
 def fragments_batch(nums, xyzs, masses):
     """fragment_structure + fragmass masses for each system (<= 64 atoms)."""
     L = library()

@@ -327,3 +327,46 @@ def manage_fragments(num, xyz, axyz, qat, mass, atlast, itrj, isec, maxsec, btf,
     return dict(rc=rc, mchrg=mc.value, chrgcont=cc.value, nfrag_ok=ok.value, list=lst,
                 nfrag=nf.value, tcont=tc.value, fragchrg3=q3, fragchrg=fq, fragip=fip,
                 ipok=ipok.value, asave=asave.value.decode(), res=recs)
+
+
+# ---- fragmentation generations (oracle/ei_oracle.c) ----
+class EiGen(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("isec", "nat", "nstep", "exit_reason", "mdok", "fragstate",
+                                        "nfrag", "max_steps", "tcont", "mchrg", "nfrag_ok", "nres",
+                                        "continued", "asave_written")]
+                + [(n, C.c_double) for n in ("chrgcont", "ttime", "epot")]
+                + [("atoms", C.c_int32 * 64), ("list", C.c_int32 * 64), ("asave", C.c_char * 160),
+                   ("res", (C.c_char * 160) * 10)])
+
+
+def ei_run(traj, nmax, tstep=0.5 * 41.3413733365614, nfragexit=3, maxsec=7, btf=1.0, itrj=1,
+           charge=1, opts=None, maxgen=16):
+    """Oracle EI generations (main.F90:2208-2345) of one trajectory dict;
+    returns a list of generation dicts."""
+    L = oracle()
+    L.orc_ei_run.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.c_int, C.c_double, C.c_double,
+                             C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p,
+                             C.c_void_p, C.c_int]
+    num = np.ascontiguousarray(traj["num"], dtype=np.int32)
+    nat = len(num)
+    imass = np.ascontiguousarray(traj.get("imass", np.full(nat, -1)), dtype=np.int32)
+    mass = np.ascontiguousarray(traj["mass"], dtype=np.float64)
+    velof = np.ascontiguousarray(traj.get("velof", np.ones(nat)), dtype=np.float64)
+    xyz = np.ascontiguousarray(traj["xyz"], dtype=np.float64)
+    velo = np.ascontiguousarray(traj["velo"], dtype=np.float64)
+    gens = (EiGen * maxgen)()
+    o = opts if opts is not None else default_opts()
+    ng = L.orc_ei_run(nat, ptr(num, ip), ptr(imass, ip), ptr(mass), ptr(velof), ptr(xyz), ptr(velo),
+                      int(charge), float(traj["eimp"]), float(traj["tadd"]), tstep, int(nmax),
+                      nfragexit, maxsec, btf, itrj, C.cast(C.pointer(o), C.c_void_p),
+                      C.cast(gens, C.c_void_p), maxgen)
+    return [ei_gen_dict(gens[k]) for k in range(ng)]
+
+
+def ei_gen_dict(g):
+    d = {f: getattr(g, f) for f, _ in g._fields_ if f not in ("atoms", "list", "asave", "res")}
+    d["atoms"] = np.array(g.atoms[:g.nat], dtype=np.int32)
+    d["list"] = np.array(g.list[:g.nat], dtype=np.int32)
+    d["asave"] = g.asave.decode()
+    d["res"] = [g.res[k].value.decode() for k in range(g.nres)]
+    return d
