@@ -165,43 +165,37 @@ def run_singlepoints(name, reps):
             "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()}}
 
 
-def run_c20_md(ntraj, steps, warmup):
+def run_c20_md(ntraj, steps, warmup, repack=100):
     """Batched fragmentation MD of C20H36O2N2+ (60 atoms, 132 basis
-    functions, matrices in global memory), IEE Poisson mean 0.8 eV/atom."""
+    functions, matrices in global memory), IEE Poisson mean 0.8 eV/atom,
+    through the fragmentation-generation driver: a trajectory that ends
+    continues with the fragment manage_fragments picks (main.F90:2208-2345),
+    and after every `repack` steps in which trajectories ended the running
+    ones are laid out again on the device."""
     import qcxms_amd
     from qcxms_amd import workloads as W
     trajs = W.md_ensemble(ntraj, base_seed=1, mean_ev=0.8, max_ev=2.0, temp=500.0,
                           builder=W.chain_c20h36o2n2, tadd_fs=100.0)
-    md = qcxms_amd.MdEnsemble(trajs, max_steps=10000, use_graph=True)
-    md.run(warmup)
-    md.set_profiling(True)
-    for k in ("setup", "scc", "grad", "md_pre", "md_post"):
-        md.kernel_time(k)
-    it0 = md.scc_iterations_total().astype(np.float64).sum()
-    n0 = sum(r.nstep for r in md.read()["results"])
+    ei = qcxms_amd.EiEnsemble(trajs, max_steps=10000, use_graph=True, repack=repack)
+    ei.run(warmup)
+    n0, r0, g0 = ei.steps(), ei.repacks(), len(ei.generations())
     t0 = time.perf_counter()
-    md.run(steps)
+    running = ei.run(steps)
     wall = time.perf_counter() - t0
-    res = md.read()["results"]
-    n1 = sum(r.nstep for r in res)
-    it1 = md.scc_iterations_total().astype(np.float64).sum()
-    kt = {k: md.kernel_time(k) for k in ("setup", "scc", "grad", "md_pre", "md_post")}
-    md.close()
-    scc_ms, nl = kt["scc"]
-    flops = (it1 - it0) * scc_flops(132)
-    ach = flops / (scc_ms * 1e-3) / 1e12 if scc_ms > 0 else 0.0
+    n1, r1 = ei.steps(), ei.repacks()
+    gens = ei.generations()
+    ei.close()
     return {"metric": "trajectory-MD-steps/sec on 1 GPU (batched)", "value": (n1 - n0) / wall,
             "unit": "trajectory-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
-            "dtype": "f64", "data": "synthetic",
+            "ms_per_step": 1e3 * wall / max(steps, 1), "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"batched fragmentation MD: {ntraj} x C20H36O2N2+ (60 atoms, "
-                                   "132 basis functions), IEE Poisson 0.8 eV/atom (<= 2.0)",
-                       "ntraj": ntraj,
-                       "scc_iterations_per_trajectory_step": (it1 - it0) / max(n1 - n0, 1),
-                       "nfrag_histogram": np.bincount([r.nfrag for r in res]).tolist()},
-            "roofline": {"bound": "mfma", "kernel": "k_scc", "achieved": ach,
-                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": ach / FP64_PEAK_TFLOPS, "avg_launch_ms": scc_ms / max(nl, 1)},
-            "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()}}
+                                   "132 basis functions), IEE Poisson 0.8 eV/atom (<= 2.0), "
+                                   "tadd 100 fs, fragment continuation (main.F90:2208-2345)",
+                       "ntraj": ntraj, "repack_every": repack, "relayouts": r1 - r0,
+                       "generations_ended": len(gens) - g0, "running_after": running,
+                       "trajectory_steps": n1 - n0,
+                       "exit_reasons": np.bincount([g["exit_reason"] for g in gens],
+                                                   minlength=9).tolist() if gens else []}}
 
 
 def run_one_per_launch(steps, warmup, fused=False):

@@ -248,6 +248,10 @@ int qcxms_ei_run(qcxms_ei *ei, int32_t nsteps, int32_t *running);
 int qcxms_ei_generations(qcxms_ei *ei, int32_t cap, qcxms_ei_generation *out, int32_t *count);
 /* Re-layouts done so far. */
 int qcxms_ei_repacks(qcxms_ei *ei, int32_t *count);
+/* Trajectory-steps executed so far over all generations. */
+int qcxms_ei_steps(qcxms_ei *ei, uint64_t *total);
+/* The current layout's ensemble (timing hooks: qcxms_md_batch). */
+qcxms_md *qcxms_ei_md(qcxms_ei *ei);
 
 /* ---- batched drop-ins for single reference routines ---- */
 /* fragment_structure(nat,iat,xyz,3.0,1,0,list) + fragmass masses

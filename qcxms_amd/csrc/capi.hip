@@ -447,6 +447,7 @@ int allocate(qcxms_batch* b)
     ALLOC(d.P, mt);
     ALLOC(d.W, mt);
     ALLOC(d.U, mt);
+    ALLOC(d.E, mt);
     ALLOC(d.gam, b->gam_total);
     ALLOC(d.wd4, b->d4_total);
     ALLOC(d.x, b->nxtot);
@@ -2057,6 +2058,24 @@ int qcxms_ei_generations(qcxms_ei* ei, int32_t cap, qcxms_ei_generation* out, in
     if (!ei || !count) return fail("null argument");
     *count = (int32_t)ei->gens.size();
     for (int k = 0; k < cap && k < (int)ei->gens.size(); ++k) out[k] = ei->gens[k];
+    return QCXMS_OK;
+}
+
+qcxms_md* qcxms_ei_md(qcxms_ei* ei) { return ei ? ei->md : nullptr; }
+
+int qcxms_ei_steps(qcxms_ei* ei, uint64_t* total)
+{
+    if (!ei || !total || !ei->md) return fail("null argument");
+    uint64_t n = 0;
+    std::vector<char> seen(ei->tr.size(), 0);
+    for (const qcxms_ei_generation& g : ei->gens) n += (uint64_t)g.nstep;
+    // generations still in the current layout
+    std::vector<qcxms_md_result> res(ei->md->ntraj);
+    int st;
+    if ((st = qcxms_md_read(ei->md, nullptr, nullptr, nullptr, nullptr, nullptr, res.data()))) return st;
+    for (int s = 0; s < ei->md->ntraj; ++s)
+        if (ei->tr[ei->slot[s]].running && res[s].running) n += (uint64_t)res[s].nstep;
+    *total = n;
     return QCXMS_OK;
 }
 

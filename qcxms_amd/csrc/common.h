@@ -99,6 +99,8 @@ struct BatchPtrs {
     double* P;        // density
     double* W;        // energy-weighted density
     double* U;        // eigenvectors in the orthogonalised basis (warm start)
+    double* E;        // scratch matrix of the MFMA eigen refinement (systems whose SCC
+                      // matrices are in global memory)
     double* gam;      // nsh^2 per system
     double* wd4;      // ndim^2 per system
     double* x;        // mixing vector [qsh, dp, qp] (converged, warm start)

@@ -1133,10 +1133,11 @@ QX_DEV void scc_system(const BatchPtrs& B, const SccOpts& opt, int sys, SccLds& 
         scc_body<true, EIG>(si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc, L.f2,
                        L.rot, L.rpq, L.s_e, L.s_any);
     } else {
-        // the energy-weighted density slot doubles as the row-sum scratch
+        // the energy-weighted density slot doubles as the row-sum scratch;
+        // the MFMA eigen refinement's fifth matrix is the batch's E slot
         SccBuffers bf{B.T + si.mat_off, B.C + si.mat_off, B.U + si.mat_off, B.P + si.mat_off,
-                      B.W + si.mat_off, si.ld, nullptr};
-        scc_body<false, false>(si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc,
+                      B.W + si.mat_off, si.ld, EIG ? B.E + si.mat_off : nullptr};
+        scc_body<false, EIG>(si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc,
                         L.f2, L.rot, L.rpq, L.s_e, L.s_any);
     }
 }

@@ -439,6 +439,9 @@ class EiEnsemble:
             L.qcxms_ei_run.argtypes = [C.c_void_p, C.c_int32, _ip]
             L.qcxms_ei_generations.argtypes = [C.c_void_p, C.c_int32, C.POINTER(EiGeneration), _ip]
             L.qcxms_ei_repacks.argtypes = [C.c_void_p, _ip]
+            L.qcxms_ei_steps.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+            L.qcxms_ei_md.restype = C.c_void_p
+            L.qcxms_ei_md.argtypes = [C.c_void_p]
             L._ei_sigs = True
         t = trajectories
         nums = [np.asarray(x["num"], dtype=np.int32) for x in t]
@@ -491,6 +494,17 @@ class EiEnsemble:
         n = C.c_int32()
         library().qcxms_ei_repacks(self._h, C.byref(n))
         return n.value
+
+    def steps(self):
+        """Trajectory-steps executed so far over all generations."""
+        n = C.c_uint64()
+        if library().qcxms_ei_steps(self._h, C.byref(n)):
+            raise RuntimeError("qcxms_ei_steps failed: " + last_error())
+        return n.value
+
+    def batch_handle(self):
+        """Batch of the current layout (timing hooks)."""
+        return library().qcxms_md_batch(library().qcxms_ei_md(self._h))
 
 
 def fragments_batch(nums, xyzs, masses):
