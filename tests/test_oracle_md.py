@@ -203,3 +203,26 @@ def test_impactscale_increments_are_exact():
     for k in range(1, 20001):
         assert s == (k - 1) * c
         s = s + c
+
+
+def test_intenergy_is_ekinet_of_each_fragment():
+    """md.f90:715-741: the temperature of fragment j is the ekinet temperature
+    (mdinit.f90:57-76, pinned to the compiled reference) of its atoms alone."""
+    import ctypes as C
+    L = ob.oracle()
+    rng = np.random.default_rng(8)
+    nat = 23
+    lst = rng.integers(1, 5, nat).astype(np.int32)
+    lst[:4] = [1, 2, 3, 4]
+    mass = rng.uniform(1800.0, 30000.0, nat)
+    velo = rng.normal(0.0, 3e-4, (nat, 3))
+    t = np.zeros(10)
+    L.orc_intenergy.argtypes = [C.c_int, ob.ip, ob.dp, ob.dp, C.c_int, ob.dp]
+    L.orc_intenergy(nat, ob.ptr(lst, ob.ip), ob.ptr(mass), ob.ptr(np.ascontiguousarray(velo)), 4,
+                    ob.ptr(t))
+    for j in range(1, 5):
+        sel = lst == j
+        e, temp = C.c_double(), C.c_double()
+        L.orc_ekinet(int(sel.sum()), ob.ptr(np.ascontiguousarray(velo[sel])),
+                     ob.ptr(np.ascontiguousarray(mass[sel])), C.byref(e), C.byref(temp))
+        assert abs(t[j - 1] - temp.value) < 1e-12 * temp.value, (j, t[j - 1], temp.value)
