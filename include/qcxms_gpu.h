@@ -36,11 +36,20 @@ const char *qcxms_gpu_last_error(void);
 /* Drop-in for get_xtb_egrad (src/tblite.f90:65-93): one structure, fresh
  * wavefunction per call.  multiplicity enters as uhf = min(mult-1, 0) = 0
  * (src/tblite.f90:111), i.e. a spin-restricted Fermi-smeared solution.
+ * etemp <= 0 (getspec's call, src/mo_spec.f90:63 via src/iniqm.f90:409)
+ * asks for integer occupations and is evaluated at 1e-3 K.
  * qat[nat], gradient[3*nat] are written; *stat receives the reference's
- * status code (0, 5 or -1) and is also the return value. */
+ * status code (0, 5 or -1) and is also the return value.  The device arrays
+ * of the structure are kept for the next call with the same atoms and
+ * charge (one structure per MD step), until the process ends. */
 int qcxms_get_xtb_egrad(int32_t nat, const int32_t *num, const double *xyz, int32_t charge,
                         int32_t multiplicity, int32_t method, double etemp, double *qat,
                         double *energy, double *gradient, int32_t *stat);
+/* spec_calc of get_xtb_egrad (src/tblite.f90:153-166 feeding write_qmo,
+ * src/mo_energ.f90:7-80): the MO spectrum of the structure of the last
+ * qcxms_get_xtb_egrad call in this process, as qcxms_batch_mo_spectrum. */
+int qcxms_get_xtb_mo_spectrum(int32_t cap, int32_t *nao, int32_t *ihomo, double *emo,
+                              double *focc, double *qmo);
 
 /* Batched single points over ragged systems (the per-trajectory egrad calls
  * of iniqm.f90:641-655, many at once).  System s owns atoms
@@ -104,6 +113,16 @@ int qcxms_batch_multipoles(qcxms_batch *b, double *dipoles, double *quadrupoles)
  * basis size.  Any output pointer except nao may be NULL. */
 int qcxms_batch_orbitals(qcxms_batch *b, int32_t sys, int32_t cap, int32_t *nao, double *emo,
                          double *focc, double *coeff, double *overlap, int32_t *aoat);
+
+/* MO spectrum of system sys after the batch's last evaluation, computed on
+ * the GPU: emo[nao] orbital energies ascending (Eh), focc[nao] electrons per
+ * orbital, qmo[nao*nat] (orbital-major) the Mulliken population of each
+ * orbital on each atom plus 1e-10, normalised per orbital (write_qmo,
+ * src/mo_energ.f90:36-52), *ihomo the 1-based index of the highest orbital
+ * holding more than 1e-6 electrons (wfn%homo, src/tblite.f90:159).  cap is
+ * the capacity of the per-orbital buffers (qmo holds cap*nat). */
+int qcxms_batch_mo_spectrum(qcxms_batch *b, int32_t sys, int32_t cap, int32_t *nao, int32_t *ihomo,
+                            double *emo, double *focc, double *qmo);
 
 /* ---- timing hooks (HIP events on the library's stream) ---- */
 /* enable != 0 records events around every kernel launch of this batch. */

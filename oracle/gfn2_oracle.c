@@ -1400,9 +1400,15 @@ void orc_scc_default_opts(orc_scc_opts *o)
 }
 
 /* --------------------------------------------------------------- driver */
+/* orbitals of the converged SCC for the MO spectrum (all NULL: not wanted) */
+typedef struct {
+    double *eps, *focc, *coeff, *overlap;
+    int32_t *aoat;
+} orbitals_out_t;
+
 static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, double etemp,
                     const orc_scc_opts *opts_in, double *energy, double *grad, double *qat_out,
-                    double *dpat, double *qpat, orc_gfn2_info *info)
+                    double *dpat, double *qpat, orc_gfn2_info *info, const orbitals_out_t *orb)
 {
     orc_scc_opts opts;
     if (opts_in) opts = *opts_in;
@@ -1698,6 +1704,18 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
     }
     free(dedcn_d4);
     free(qat);
+    if (orb) {
+        /* eigenvalues ascending (orc_sygv), column k of F the AO coefficients
+         * of orbital k; focc per spin, the orbital holds 2 focc electrons */
+        for (int k = 0; k < n; ++k) {
+            if (orb->eps) orb->eps[k] = eps[k];
+            if (orb->focc) orb->focc[k] = 2.0 * focc[k];
+        }
+        if (orb->coeff) memcpy(orb->coeff, F, sizeof(double) * n * n);
+        if (orb->overlap) memcpy(orb->overlap, y.S, sizeof(double) * n * n);
+        if (orb->aoat)
+            for (int mu = 0; mu < n; ++mu) orb->aoat[mu] = y.ao2at[mu];
+    }
     free(x); free(xo); free(dx); free(vs); free(vd); free(vq);
     free(F); free(Sw); free(P); free(eps); free(focc); free(work);
     broyden_free(&br);
@@ -1711,13 +1729,47 @@ int orc_gfn2_egrad(int nat, const int32_t *num, const double *xyz, int charge, d
 {
     if (grad) memset(grad, 0, sizeof(double) * 3 * nat);
     return gfn2_run(nat, num, xyz, charge, etemp, opts, energy, grad, qat, dipole_at, qpole_at,
-                    info);
+                    info, NULL);
 }
 
 int orc_gfn2_energy(int nat, const int32_t *num, const double *xyz, int charge, double etemp,
                     const orc_scc_opts *opts, double *energy, double *qat)
 {
-    return gfn2_run(nat, num, xyz, charge, etemp, opts, energy, NULL, qat, NULL, NULL, NULL);
+    return gfn2_run(nat, num, xyz, charge, etemp, opts, energy, NULL, qat, NULL, NULL, NULL, NULL);
+}
+
+int orc_gfn2_orbitals(int nat, const int32_t *num, const double *xyz, int charge, double etemp,
+                      const orc_scc_opts *opts, double *energy, double *eps, double *focc,
+                      double *coeff, double *overlap, int32_t *aoat)
+{
+    orbitals_out_t o = {eps, focc, coeff, overlap, aoat};
+    return gfn2_run(nat, num, xyz, charge, etemp, opts, energy, NULL, NULL, NULL, NULL, NULL, &o);
+}
+
+/* Mulliken population of every orbital on every atom, write_qmo
+ * (mo_energ.f90:36-52): for orbital k and atom a the sum over the AOs j of
+ * a and all AOs m of C(j,k) C(m,k) S(j,m), plus 1e-10, then each orbital's
+ * row divided by its sum over the atoms.  coeff[mu*n + k] (row-major, column
+ * k = orbital k), overlap[n*n], aoat[n] 0-based atom of each AO;
+ * qmo[k*nat + a]. */
+void orc_mo_populations(int nat, int n, const double *coeff, const double *overlap,
+                        const int32_t *aoat, double *qmo)
+{
+    for (int k = 0; k < n; ++k) {
+        double *row = qmo + (size_t)k * nat;
+        for (int a = 0; a < nat; ++a) {
+            double q = 0.0;
+            for (int j = 0; j < n; ++j) {
+                if (aoat[j] != a) continue;
+                for (int m = 0; m < n; ++m)
+                    q += coeff[(size_t)j * n + k] * coeff[(size_t)m * n + k] * overlap[(size_t)j * n + m];
+            }
+            row[a] = q + 1e-10;
+        }
+        double sum = 0.0;
+        for (int a = 0; a < nat; ++a) sum += row[a];
+        for (int a = 0; a < nat; ++a) row[a] /= sum;
+    }
 }
 
 /* matrices for unit tests: S, H0 (n^2), Drow (3 n^2), Qrow (6 n^2) */

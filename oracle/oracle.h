@@ -107,6 +107,17 @@ int orc_gfn2_egrad(int nat, const int32_t *num, const double *xyz, int charge, d
 int orc_gfn2_energy(int nat, const int32_t *num, const double *xyz, int charge,
                     double etemp, const orc_scc_opts *opts, double *energy, double *qat);
 
+/* orbitals of the converged SCC (getspec's single point): eps[nao] ascending
+ * (Eh), focc[nao] electrons per orbital, coeff[nao*nao] row-major with column
+ * k = orbital k, overlap[nao*nao], aoat[nao] 0-based atom of each AO; any
+ * output may be NULL */
+int orc_gfn2_orbitals(int nat, const int32_t *num, const double *xyz, int charge, double etemp,
+                      const orc_scc_opts *opts, double *energy, double *eps, double *focc,
+                      double *coeff, double *overlap, int32_t *aoat);
+/* MO populations of write_qmo (mo_energ.f90:36-52): qmo[nao*nat] */
+void orc_mo_populations(int nat, int n, const double *coeff, const double *overlap,
+                        const int32_t *aoat, double *qmo);
+
 /* building blocks exposed for unit pinning */
 int  orc_gfn2_supported(int z);
 void orc_gfn2_cn(int nat, const int32_t *num, const double *xyz, double *cn, double *dcn);

@@ -37,6 +37,7 @@ __global__ __launch_bounds__(256) void k_fill_zero(double* p, size_t n)
 
 #include "k_grad.h"
 #include "k_md.h"
+#include "k_spec.h"
 
 namespace {
 
@@ -1065,6 +1066,37 @@ int qcxms_batch_orbitals(qcxms_batch* b, int32_t sys, int32_t cap, int32_t* nao,
     return QCXMS_OK;
 }
 
+int qcxms_batch_mo_spectrum(qcxms_batch* b, int32_t sys, int32_t cap, int32_t* nao, int32_t* ihomo, double* emo,
+                            double* focc, double* qmo)
+{
+    if (!b || !nao || !ihomo || !emo || !focc || !qmo) return fail("null argument");
+    if (sys < 0 || sys >= b->nsys) return fail("system index out of range");
+    if (b->arena) return fail("the MO spectrum needs a batch with its own device memory");
+    const SysInfo& si = b->info[sys];
+    const int n = si.nao, nat = si.nat;
+    *nao = n;
+    if (cap < n) return fail("orbital buffers smaller than the basis");
+    double* dv = nullptr;
+    HIPCHK(hipMalloc((void**)&dv, sizeof(double) * (size_t)n * (2 + nat)));
+    std::vector<double> host((size_t)n * (2 + nat));
+    hipLaunchKernelGGL(qx::k_mo_pop, dim3(1), dim3(QX_BLOCK), 0, b->stream, b->d, (int)sys, dv, dv + n, dv + 2 * n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host.data(), dv, sizeof(double) * host.size(), hipMemcpyDeviceToHost,
+                                            b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(dv);
+    if (e != hipSuccess) return fail(std::string("k_mo_pop: ") + hipGetErrorString(e));
+    std::copy(host.begin(), host.begin() + n, emo);
+    std::copy(host.begin() + n, host.begin() + 2 * n, focc);
+    std::copy(host.begin() + 2 * n, host.end(), qmo);
+    // HOMO: the highest orbital holding electrons (1-based, at least 1)
+    int h = 1;
+    for (int k = 0; k < n; ++k)
+        if (focc[k] > 1e-6) h = k + 1;
+    *ihomo = h;
+    return QCXMS_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -1086,6 +1118,21 @@ qcxms_batch* arena_batch_create(int32_t nsys, const int32_t* offsets, const int3
     }
     return b;
 }
+
+// The batch of the single-structure drop-in (qcxms_get_xtb_egrad), kept from
+// call to call while the atoms and the charge stay the same: the MD of
+// iniqm.f90:641-655 evaluates one structure per step, and rebuilding the
+// device arrays every step would cost more than the evaluation.  Each call
+// still starts from a fresh wavefunction (warm start off), as
+// tblite.f90:133.  Held until the process ends.
+struct SingleBatch {
+    std::mutex mu;
+    qcxms_batch* b = nullptr;
+    std::vector<int32_t> num;
+    int32_t charge = 0;
+};
+SingleBatch g_single;
+constexpr double kEtempAufbau = 1.0e-3;  // K
 
 // qcxms_egrad_batch and its allocation-free variant (arena != 0: every device
 // array of the call is carved from the scratch array)
@@ -1180,13 +1227,47 @@ int qcxms_egrad_batch_noalloc(int32_t nsys, const int32_t* offsets, const int32_
 int qcxms_get_xtb_egrad(int32_t nat, const int32_t* num, const double* xyz, int32_t charge, int32_t multiplicity,
                         int32_t method, double etemp, double* qat, double* energy, double* gradient, int32_t* stat)
 {
-    const int32_t offs[2] = {0, nat};
+    (void)multiplicity;  // uhf = min(mult-1, 0) = 0 for every multiplicity (tblite.f90:111)
+    g_err.clear();
+    auto done = [&](int s) {
+        if (stat) *stat = s;
+        return s;
+    };
+    if (method != QCXMS_GFN2_XTB) {
+        g_err = "only the GFN2-xTB Hamiltonian (method 2) is implemented";
+        return done(QCXMS_UNKNOWN_METHOD);
+    }
+    if (nat < 1 || !num || !xyz) {
+        g_err = "null argument or no atoms";
+        return done(QCXMS_FATAL);
+    }
+    // etemp 0 (getspec, mo_spec.f90:63) asks for integer occupations: the
+    // Fermi filling at kEtempAufbau K gives them to within exp(-1e6)
+    const double et = etemp > 0.0 ? etemp : kEtempAufbau;
+    std::lock_guard<std::mutex> lk(g_single.mu);
+    if (!g_single.b || g_single.charge != charge || g_single.num.size() != (size_t)nat ||
+        !std::equal(g_single.num.begin(), g_single.num.end(), num)) {
+        if (g_single.b) qcxms_batch_destroy(g_single.b);
+        g_single.b = nullptr;
+        g_single.num.assign(num, num + nat);
+        g_single.charge = charge;
+        const int32_t offs[2] = {0, nat};
+        g_single.b = qcxms_batch_create(1, offs, num, &charge, method);
+        if (!g_single.b) {
+            const bool elem = g_err.find("no GFN2 parameters") != std::string::npos;
+            return done(elem ? QCXMS_UNKNOWN_METHOD : QCXMS_FATAL);
+        }
+    }
     int32_t s = 0;
-    const int st = qcxms_egrad_batch(1, offs, num, xyz, &charge, &multiplicity, method, &etemp, qat, energy,
-                                     gradient, &s);
-    const int out = (st == QCXMS_UNKNOWN_METHOD) ? QCXMS_UNKNOWN_METHOD : (st == QCXMS_OK ? s : QCXMS_FATAL);
-    if (stat) *stat = out;
-    return out;
+    const int st = qcxms_batch_egrad(g_single.b, xyz, &et, energy, gradient, qat, &s);
+    return done(st == QCXMS_OK ? s : QCXMS_FATAL);
+}
+
+int qcxms_get_xtb_mo_spectrum(int32_t cap, int32_t* nao, int32_t* ihomo, double* emo, double* focc, double* qmo)
+{
+    std::lock_guard<std::mutex> lk(g_single.mu);
+    if (!g_single.b) return fail("no structure evaluated by qcxms_get_xtb_egrad yet");
+    return qcxms_batch_mo_spectrum(g_single.b, 0, cap, nao, ihomo, emo, focc, qmo);
 }
 
 qcxms_md* qcxms_md_create(int32_t ntraj, const int32_t* offsets, const int32_t* num, const int32_t* charge,

@@ -281,18 +281,6 @@ def vary_energies(e_in, e_distr, rng):
 
 # ---------------------------------------------------------------------------
 # MO spectrum of the ground state and the randomised ensemble (main.F90:756-886)
-def mo_populations(coeff, overlap, aoat, nat):
-    """write_qmo (mo_energ.f90:7-80): Mulliken population of every MO on
-    every atom, + 1e-10, normalised per MO.  Returns qmo [nao, nat]."""
-    nao = coeff.shape[0]
-    sc = overlap @ coeff                  # (S C)[j, n]
-    per_ao = coeff * sc                   # C[j, n] sum_k C[k, n] S[j, k]
-    qmo = np.zeros((nao, nat))
-    for ia in range(1, nat + 1):
-        qmo[:, ia - 1] = per_ao[aoat == ia].sum(axis=0) + 1.0e-10
-    return qmo / qmo.sum(axis=1, keepdims=True)
-
-
 SPEC_ETEMP = 1.0e-3  # K: integer occupations for the MO spectrum (reference: etemp 0)
 
 
@@ -309,18 +297,16 @@ def getspec(num, xyz, charge=0):
         e, g, q, st = b.egrad(np.asarray(xyz, dtype=np.float64), SPEC_ETEMP)
         if st[0] != 0:
             raise RuntimeError("QC initialization error")
-        eps, focc, coeff, ovl, aoat = b.orbitals(0)
+        eps, focc, mopop, ihomo = b.mo_spectrum(0)
     finally:
         b.close()
     nel = int(sum(int(z) - (0 if z <= 2 else 2) for z in num)) - charge
     nb = nel // 2
-    occ = np.nonzero(focc > 1e-6)[0]
-    ihomo = max(int(occ.max()) + 1 if len(occ) else 1, 1)
     emo = -eps
     if emo[0] - emo[ihomo - 1] < 1.0 / AUTOEV:   # eps in eV: emo(ihomo) - emo(1) < 1
         raise RuntimeError("weird epsilons")
     return dict(emo=emo, ehomo=float(emo[ihomo - 1]), ihomo=ihomo, nb=nb,
-                mopop=mo_populations(coeff, ovl, aoat, len(num)), energy=float(e[0]))
+                mopop=mopop, energy=float(e[0]))
 
 
 def irand(n, rng):

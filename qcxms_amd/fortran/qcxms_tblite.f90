@@ -10,8 +10,10 @@
 !> provided on the GPU) method, -1 failed calculation.  A HIP runtime or
 !> argument error of the library (-2) is reported as -1.
 !> output_file receives a short log of the call, as the reference's context
-!> unit does.  spec_calc (MO populations for the setup step,
-!> src/tblite.f90:153-166) is not provided: no tmp.mspec is written.
+!> unit does.  spec_calc (src/tblite.f90:153-166) writes the MO spectrum of
+!> the structure, tmp.mspec and qcxms.Mspec.tbxtb, in the layout of
+!> write_qmo (src/mo_energ.f90:7-80), from the orbital energies, occupations
+!> and Mulliken MO populations the GPU computes (qcxms_get_xtb_mo_spectrum).
 module qcxms_tblite
    use, intrinsic :: iso_c_binding, only : c_int32_t, c_double, c_char, c_ptr, &
       & c_f_pointer, c_associated
@@ -35,6 +37,9 @@ module qcxms_tblite
 
    integer, parameter :: stat_fatal = -1
 
+   !> Hartree to eV (src/xtb_mctc_convert.f90:27)
+   real(wp), parameter :: autoev = 27.21138505_wp
+
    interface
       !> int qcxms_get_xtb_egrad(int32_t nat, const int32_t *num, const double *xyz,
       !>     int32_t charge, int32_t multiplicity, int32_t method, double etemp,
@@ -50,6 +55,17 @@ module qcxms_tblite
          integer(c_int32_t), intent(out) :: stat
          integer(c_int32_t) :: rc
       end function c_get_xtb_egrad
+
+      !> int qcxms_get_xtb_mo_spectrum(int32_t cap, int32_t *nao, int32_t *ihomo,
+      !>     double *emo, double *focc, double *qmo)
+      function c_get_xtb_mo_spectrum(cap, nao, ihomo, emo, focc, qmo) result(rc) &
+            & bind(C, name="qcxms_get_xtb_mo_spectrum")
+         import :: c_int32_t, c_double
+         integer(c_int32_t), value :: cap
+         integer(c_int32_t), intent(out) :: nao, ihomo
+         real(c_double), intent(out) :: emo(*), focc(*), qmo(*)
+         integer(c_int32_t) :: rc
+      end function c_get_xtb_mo_spectrum
 
       function c_last_error() result(msg) bind(C, name="qcxms_gpu_last_error")
          import :: c_ptr
@@ -113,11 +129,51 @@ subroutine get_xtb_egrad(num, xyz, charge, multiplicity, method, etemp, &
       write(unit, '(a)') "[Fatal] Singlepoint calculation failed"
       write(unit, '(a,a)') "-> ", last_error()
    end select
-   if (spec_calc) then
-      write(unit, '(a)') "[Info] MO populations (spec_calc) are not produced by the GPU path"
+   if (spec_calc .and. stat == 0) then
+      if (.not. write_mo_spectrum(nat)) then
+         write(unit, '(a)') "[Fatal] MO spectrum failed"
+         write(unit, '(a,a)') "-> ", last_error()
+         stat = stat_fatal
+      end if
    end if
    close(unit)
 end subroutine get_xtb_egrad
+
+!> tmp.mspec and qcxms.Mspec.tbxtb of the last structure evaluated, in the
+!> layout src/mo_energ.f90:54-76 writes (read back by readpopmo,
+!> src/mo_energ.f90:82-107): orbital energies in eV, occupations, and the
+!> populations qmo(k, atom) normalised per orbital.
+logical function write_mo_spectrum(nat) result(ok)
+   integer, intent(in) :: nat
+   integer(c_int32_t) :: cap, nao, ihomo
+   real(c_double), allocatable :: emo(:), focc(:), qmo(:, :)
+   integer :: io_tmp, io_mspec, j, k
+
+   cap = int(9 * nat, c_int32_t)
+   allocate(emo(cap), focc(cap), qmo(nat, cap))
+   ok = c_get_xtb_mo_spectrum(cap, nao, ihomo, emo, focc, qmo) == 0
+   if (.not. ok) return
+
+   open(newunit=io_tmp, file='tmp.mspec', status='replace')
+   open(newunit=io_mspec, file='qcxms.Mspec.tbxtb', status='replace')
+   write(io_mspec, *) nao, ihomo
+   do k = 1, nao
+      write(io_mspec, *)
+      write(io_mspec, '(1(i3, 1x, f10.3))') k, emo(k) * autoev
+      write(io_mspec, '(1(1x, f6.2))') focc(k)
+      write(io_mspec, '(10(1x, f6.2))') (qmo(j, k) * 100.0_wp, j = 1, nat)
+   end do
+   write(io_tmp, *) nao, ihomo
+   do k = 1, nao
+      write(io_tmp, *) emo(k) * autoev
+      write(io_tmp, *) focc(k)
+      do j = 1, nat
+         write(io_tmp, *) qmo(j, k)
+      end do
+   end do
+   close(io_tmp)
+   close(io_mspec)
+end function write_mo_spectrum
 
 !> Message of the library's last failure on this thread
 function last_error() result(msg)

@@ -49,6 +49,9 @@ def library():
         L.qcxms_batch_multipoles.argtypes = [C.c_void_p, _dp, _dp]
         L.qcxms_batch_orbitals.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp,
                                            _dp, _ip]
+        L.qcxms_batch_mo_spectrum.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _ip, _ip, _dp,
+                                              _dp, _dp]
+        L.qcxms_get_xtb_mo_spectrum.argtypes = [C.c_int32, _ip, _ip, _dp, _dp, _dp]
         L.qcxms_batch_scc_iterations_total.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         L.qcxms_get_xtb_egrad.argtypes = [C.c_int32, _ip, _dp, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_double, _dp, _dp, _dp, _ip]
@@ -78,6 +81,23 @@ def get_xtb_egrad(num, xyz, charge, multiplicity, method=GFN2_XTB, etemp=300.0):
                           int(method), float(etemp), _p(qat), C.byref(e), _p(grad),
                           C.byref(stat))
     return qat, e.value, grad, stat.value
+
+
+def _mo_spectrum(call, nat):
+    cap = 256
+    n, h = C.c_int32(), C.c_int32()
+    emo, focc, qmo = np.zeros(cap), np.zeros(cap), np.zeros(cap * nat)
+    if call(cap, C.byref(n), C.byref(h), _p(emo), _p(focc), _p(qmo)):
+        raise RuntimeError("MO spectrum failed: " + last_error())
+    k = n.value
+    return emo[:k].copy(), focc[:k].copy(), qmo[:k * nat].reshape(k, nat).copy(), h.value
+
+
+def get_xtb_mo_spectrum(nat):
+    """MO spectrum of the structure of the last get_xtb_egrad call:
+    (emo [nao] Eh ascending, focc [nao] electrons per orbital, qmo [nao, nat]
+    Mulliken MO populations normalised per orbital, ihomo 1-based)."""
+    return _mo_spectrum(library().qcxms_get_xtb_mo_spectrum, nat)
 
 
 class Batch:
@@ -170,6 +190,13 @@ class Batch:
         k = n.value
         return (emo[:k].copy(), focc[:k].copy(), coeff[:k * k].reshape(k, k).T.copy(),
                 ovl[:k * k].reshape(k, k).T.copy(), aoat[:k].copy())
+
+    def mo_spectrum(self, sys=0):
+        """(emo [nao] Eh ascending, focc [nao], qmo [nao, nat], ihomo) of system
+        sys after the last evaluation, computed on the GPU (write_qmo's MO
+        populations, mo_energ.f90:36-52)."""
+        nat = int(self.offsets[sys + 1] - self.offsets[sys])
+        return _mo_spectrum(lambda *a: library().qcxms_batch_mo_spectrum(self._h, int(sys), *a), nat)
 
 
 def egrad_batch(systems, xyz_list, etemp=300.0, method=GFN2_XTB):

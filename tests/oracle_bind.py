@@ -120,6 +120,44 @@ def gfn2_egrad(num, xyz, charge=0, etemp=300.0, opts=None, grad=True, moments=Fa
     return out
 
 
+def gfn2_orbitals(num, xyz, charge=0, etemp=300.0, opts=None):
+    """Oracle single point, returning the converged orbitals: dict(stat,
+    energy, eps[n] ascending, focc[n] (electrons per orbital), coeff[n, n]
+    (column k = orbital k), S[n, n], aoat[n] (0-based atom))."""
+    L = oracle()
+    num = np.ascontiguousarray(num, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = gfn2_matrices_nao(num, xyz)
+    e = C.c_double()
+    eps, focc = np.zeros(n), np.zeros(n)
+    coeff, S = np.zeros((n, n)), np.zeros((n, n))
+    aoat = np.zeros(n, dtype=np.int32)
+    o = opts if opts is not None else default_opts()
+    st = L.orc_gfn2_orbitals(len(num), ptr(num, ip), ptr(xyz), int(charge), C.c_double(etemp),
+                             C.byref(o), C.byref(e), ptr(eps), ptr(focc), ptr(coeff), ptr(S),
+                             ptr(aoat, ip))
+    return dict(stat=st, energy=e.value, eps=eps, focc=focc, coeff=coeff, S=S, aoat=aoat)
+
+
+def mo_populations(coeff, S, aoat, nat):
+    """write_qmo's MO populations (oracle restatement): qmo[nao, nat]."""
+    n = len(aoat)
+    coeff = np.ascontiguousarray(coeff, dtype=np.float64)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    aoat = np.ascontiguousarray(aoat, dtype=np.int32)
+    q = np.zeros((n, nat))
+    oracle().orc_mo_populations(int(nat), int(n), ptr(coeff), ptr(S), ptr(aoat, ip), ptr(q))
+    return q
+
+
+def gfn2_matrices_nao(num, xyz):
+    nao = C.c_int()
+    oracle().orc_gfn2_matrices(len(num), ptr(np.ascontiguousarray(num, dtype=np.int32), ip),
+                               ptr(np.ascontiguousarray(xyz, dtype=np.float64)), None, None, None,
+                               None, C.byref(nao))
+    return nao.value
+
+
 def gfn2_matrices(num, xyz):
     L = oracle()
     num = np.ascontiguousarray(num, dtype=np.int32)

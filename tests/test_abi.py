@@ -12,7 +12,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "qcxms_gpu.h")
 LIB = os.path.join(ROOT, "qcxms_amd", "libqcxms_gpu.so")
-DRIVER = os.path.join(ROOT, "qcxms_amd", "fortran", "_build", "egrad_driver")
+DRIVER = os.path.join(ROOT, "tests", "fortran", "_build", "egrad_driver")
 
 
 def declared_functions():

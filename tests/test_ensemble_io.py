@@ -88,21 +88,6 @@ def test_fortran_d_edge_cases():
     assert E.fortran_i(123456, 4) == "****"
 
 
-def test_mo_populations_single_centre_and_normalised():
-    # orbitals on single atoms with S = 1: each MO lives on its own atom
-    aoat = np.array([1, 1, 2, 3])
-    coeff = np.eye(4)
-    q = E.mo_populations(coeff, np.eye(4), aoat, 3)
-    np.testing.assert_allclose(q.sum(axis=1), 1.0, rtol=0, atol=1e-15)
-    assert q[0, 0] > 1 - 1e-9 and q[2, 1] > 1 - 1e-9 and q[3, 2] > 1 - 1e-9
-    # a bonding combination over two atoms with overlap s splits evenly
-    s = 0.3
-    S = np.array([[1.0, s], [s, 1.0]])
-    c = np.array([[1.0], [1.0]]) / np.sqrt(2 + 2 * s)
-    q = E.mo_populations(np.hstack([c, c]), S, np.array([1, 2]), 2)
-    np.testing.assert_allclose(q[0], [0.5, 0.5], atol=1e-12)
-
-
 def test_momap_finds_best_orbital_triple():
     """With few orbitals the 5001 random trials cover every (mo1, mo2, vmo)
     combination, so momap returns the exhaustive optimum."""
