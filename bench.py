@@ -14,9 +14,13 @@ state resident in HBM.  By default the steps run in the persistent MD kernel
 (k_md_fused: whole MD iterations per trajectory, many steps per launch) with
 the SCC eigenvectors refined by FP64 MFMA rotations; --fused 0 runs one launch
 per kernel and step.  The roofline object describes the dominant kernel
-(k_md_fused, or the SCC kernel k_scc with --fused 0) with the analytic FP64
-flop count per SCC iteration 15 n^3 + 30 n^2 (n = basis functions), timed by
-HIP events on the library's stream.  cpu_baseline: the C oracle's restatement of the same MD loop with the
+(k_md_fused, or the SCC kernel k_scc with --fused 0), timed by HIP events on
+the library's stream.  Its algorithmic FP64 flops: per trajectory-step
+F_setup + (SCC iterations) F_iter + F_int + F_grad + F_pair, with
+F_setup = 4/3 n^3, F_iter = 15 n^3 + 30 n^2 (n = basis functions) and the
+integral, integral-derivative and pair-term costs per primitive pair, AO pair,
+atom pair and atom triple counted in the oracle's FP64 operations
+(profiles/flop_model.json); k_scc alone is priced with F_iter only.  cpu_baseline: the C oracle's restatement of the same MD loop with the
 same forces (not QCxMS+tblite, which cannot be built here), one trajectory per
 host thread as bin/pqcxms runs it, on a bounded sample.
 """
@@ -39,6 +43,43 @@ HBM_PEAK_GBS = 8000.0
 def scc_flops(n):
     """Analytic FP64 flops of one SCC iteration of an n-function system."""
     return 15.0 * n ** 3 + 30.0 * n ** 2
+
+
+FLOP_MODEL = os.path.join(ROOT, "profiles", "flop_model.json")
+
+
+def fixed_step_flops(num):
+    """FP64 flops of one energy + gradient of the structure num outside the
+    SCC iterations: {setup, int, grad, pair} from the frozen per-unit costs
+    (shell pairs ia >= ib with their primitive and AO pairs for the integrals,
+    pairs on different atoms for their derivatives, atom pairs and triples for
+    the pair terms)."""
+    m = json.load(open(FLOP_MODEL))
+    nprim = {int(k): v for k, v in m["nprim"].items()}
+    shells = [(a, l, g) for a, z in enumerate(num) for l, g in enumerate(nprim[int(z)])]
+    c = dict(pp_all=0.0, pp_ao_all=0.0, sp_all=0.0, pp_off=0.0, pp_ao_off=0.0, sp_off=0.0)
+    for i, (ai, li, gi) in enumerate(shells):
+        for aj, lj, gj in shells[:i + 1]:
+            npp, nao = gi * gj, (2 * li + 1) * (2 * lj + 1)
+            c["pp_all"] += npp
+            c["pp_ao_all"] += npp * nao
+            c["sp_all"] += 1
+            if ai != aj:
+                c["pp_off"] += npp
+                c["pp_ao_off"] += npp * nao
+                c["sp_off"] += 1
+    n = sum(2 * l + 1 for _, l, _ in shells)
+    nat, nsh = len(num), len(shells)
+    fi, fg, fp = m["F_int"], m["F_grad"], m["F_pair"]
+    return {"setup": 4.0 / 3.0 * n ** 3,
+            "int": fi["per_prim_pair"] * c["pp_all"] + fi["per_prim_pair_ao_pair"] * c["pp_ao_all"]
+            + fi["per_shell_pair"] * c["sp_all"],
+            "grad": 2.0 * n ** 3 + fg["per_prim_pair"] * c["pp_off"]
+            + fg["per_prim_pair_ao_pair"] * c["pp_ao_off"] + fg["per_shell_pair"] * c["sp_off"],
+            "pair": fp["per_atom"] * nat + fp["per_atom_pair"] * nat * (nat - 1) / 2.0
+            + fp["per_atom_triple"] * nat * (nat - 1) * (nat - 2) / 6.0
+            + fp["per_shell_pair_sq"] * nsh * nsh,
+            "n": n}
 
 
 def workload(ntraj):
@@ -72,7 +113,6 @@ def traffic_from_profiles(fused=False, steps_per_launch=1.0):
 def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1, fused=False, eigensolver=0):
     import qcxms_amd
     trajs = workload(ntraj)
-    nbf = 50
     md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=False, lanes=lanes, fused=fused,
                               eigensolver=eigensolver)
     md.run(warmup)
@@ -95,9 +135,15 @@ def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1, fused=False, ei
     wall = t1 - t0
     traj_steps = n1 - n0
     # the dominant kernel: the SCC, or the persistent kernel that contains it
+    # (and with it setup, gradient and MD bookkeeping of every step)
     scc_ms, scc_launches = kt["fused" if fused else "scc"]
-    flops = (it1 - it0) * scc_flops(nbf)
+    fx = fixed_step_flops(trajs[0]["num"])
+    scc_part = (it1 - it0) * scc_flops(fx["n"])
+    fixed_part = traj_steps * (fx["setup"] + fx["int"] + fx["grad"] + fx["pair"])
+    flops = scc_part + (fixed_part if fused else 0.0)
     achieved = flops / (scc_ms * 1e-3) / 1e12 if scc_ms > 0 else 0.0
+    per_step = {k: fx[k] / 1e6 for k in ("setup", "int", "grad", "pair")}
+    per_step["scc"] = scc_part / max(traj_steps, 1) / 1e6
     out = dict(wall=wall, traj_steps=traj_steps, value=traj_steps / wall,
                ms_per_step=1e3 * wall / steps, running=running,
                scc_iters_per_traj_step=(it1 - it0) / max(traj_steps, 1),
@@ -105,7 +151,9 @@ def run_gpu(ntraj, steps, warmup, use_graph_check=True, lanes=1, fused=False, ei
                kernels={k: dict(ms_total=v[0], launches=v[1],
                                 ms_avg=(v[0] / v[1] if v[1] else 0.0)) for k, v in kt.items()},
                achieved_tflops=achieved, scc_avg_ms=scc_ms / max(scc_launches, 1),
-               scc_flops_per_launch=flops / max(scc_launches, 1))
+               scc_flops_per_launch=flops / max(scc_launches, 1),
+               achieved_scc_only=scc_part / (scc_ms * 1e-3) / 1e12 if scc_ms > 0 else 0.0,
+               mflop_per_traj_step=per_step)
     # the same steps through the per-step kernels, each step replayed as one
     # captured hipGraph (no per-kernel events)
     if use_graph_check:
@@ -157,12 +205,24 @@ def run_singlepoints(name, reps):
     iters = np.asarray(b.info()[0], dtype=np.float64)
     b.close()
     assert np.all(st == 0)
+    # algorithmic flops of one launch of the three kernels together (cold SCC:
+    # the same iterations on every repetition)
+    flops = 0.0
+    for (num, _, _), it in zip(sysl, iters):
+        fx = fixed_step_flops(num)
+        flops += fx["setup"] + fx["int"] + fx["grad"] + fx["pair"] + it * scc_flops(fx["n"])
+    kernel_ms = sum(v[0] for v in kt.values()) / max(reps, 1)
+    achieved = flops / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
     return {"metric": "GFN2-xTB energy+gradient single points/sec on 1 GPU (batched)",
             "value": len(sysl) * reps / wall, "unit": "single-points/s", "n_gpus": 1,
             "reps": reps, "dtype": "f64", "data": "synthetic",
             "config": {"workload": desc, "nsys": len(sysl),
                        "scc_iterations_mean": float(iters.mean())},
-            "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()}}
+            "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()},
+            "roofline": {"bound": "mfma", "kernel": "k_setup + k_scc + k_grad",
+                         "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP64_PEAK_TFLOPS, "flops_per_launch": flops,
+                         "source": "profiles/flop_model.json"}}
 
 
 def run_c20_md(ntraj, steps, warmup, repack=100):
@@ -415,7 +475,12 @@ def main():
                      "traffic": traffic_from_profiles(
                          bool(args.fused), args.steps / max(g["dominant_launches"], 1)),
                      "flops_per_launch": g["scc_flops_per_launch"],
-                     "avg_launch_ms": g["scc_avg_ms"]},
+                     "avg_launch_ms": g["scc_avg_ms"],
+                     "flop_model": {"mflop_per_trajectory_step": g["mflop_per_traj_step"],
+                                    "counted_in_launch": (["setup", "scc", "int", "grad", "pair"]
+                                                          if args.fused else ["scc"]),
+                                    "source": "profiles/flop_model.json"},
+                     "achieved_scc_only": g["achieved_scc_only"]},
         "kernels_ms_avg": {k: v["ms_avg"] for k, v in g["kernels"].items()},
         "per_step_hipgraph": {"value": g.get("per_step_hipgraph_value"),
                               "unit": "trajectory-steps/s",

@@ -44,6 +44,17 @@
 #define EVTOAU (1.0 / ORC_AUTOEV)
 #define PI 3.14159265358979323846
 
+/* Flop-count build (oracle/count_flops.py, -DORC_FLOP_COUNT): every FP64
+ * operation adds to orc_flops[orc_flop_phase]; the markers below name the
+ * phase of the work that follows.  They do nothing in the normal build. */
+enum { PH_OTHER, PH_PAIR, PH_INT, PH_SCC, PH_GRAD_W, PH_GRAD_INT, PH_GRAD_PAIR };
+#ifdef ORC_FLOP_COUNT
+int orc_flop_phase;
+#define ORC_PHASE(k) (orc_flop_phase = (k))
+#else
+#define ORC_PHASE(k) ((void)0)
+#endif
+
 static int elem_index(int z)
 {
     switch (z) {
@@ -745,6 +756,7 @@ static void build_geometry(sys_t *y)
     y->dcn = (double *)malloc(sizeof(double) * nat * nat);
     y->cnd4 = (double *)malloc(sizeof(double) * nat);
     y->dcnd4 = (double *)malloc(sizeof(double) * nat * nat);
+    ORC_PHASE(PH_PAIR);
     orc_gfn2_cn(nat, y->num, y->xyz, y->cn, y->dcn);
     orc_d4_cn(nat, y->num, y->xyz, y->cnd4, y->dcnd4);
     y->hself = (double *)malloc(sizeof(double) * nsh);
@@ -758,6 +770,7 @@ static void build_geometry(sys_t *y)
     y->Drow = (double *)calloc(3 * (size_t)n * n, sizeof(double));
     y->Qrow = (double *)calloc(6 * (size_t)n * n, sizeof(double));
     double s[9], dpb[27], qpb[54], dpa[27], qpa[54], qtb[54], qta[54];
+    ORC_PHASE(PH_INT);
     for (int ia = 0; ia < nsh; ++ia)
         for (int ib = 0; ib <= ia; ++ib) {
             const shell_t *sa = &y->sh[ia], *sb = &y->sh[ib];
@@ -796,6 +809,7 @@ static void build_geometry(sys_t *y)
             y->H0[(size_t)mu * n + mu] = y->hself[is];
         }
     /* ES2 gamma */
+    ORC_PHASE(PH_PAIR);
     y->gam = (double *)malloc(sizeof(double) * nsh * nsh);
     for (int ia = 0; ia < nsh; ++ia)
         for (int ib = 0; ib < nsh; ++ib) {
@@ -1410,6 +1424,7 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
                     const orc_scc_opts *opts_in, double *energy, double *grad, double *qat_out,
                     double *dpat, double *qpat, orc_gfn2_info *info, const orbitals_out_t *orb)
 {
+    ORC_PHASE(PH_OTHER);
     orc_scc_opts opts;
     if (opts_in) opts = *opts_in;
     else orc_scc_default_opts(&opts);
@@ -1432,6 +1447,7 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
     double eold = 0.0, etot = 0.0, ts = 0.0;
     eparts_t ep = {0};
     int conv = 0, it;
+    ORC_PHASE(PH_SCC);
     for (it = 1; it <= opts.maxiter; ++it) {
         interaction(&y, x, vs, vd, vq, &ep);
         fock(&y, vs, vd, vq, F);
@@ -1465,6 +1481,7 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
     }
     if (it > opts.maxiter) it = opts.maxiter;
     /* final state: moments xo from P; potentials vs/vd/vq at xo (computed above) */
+    ORC_PHASE(PH_PAIR);
     double *dedcn_d4 = (double *)calloc(nat, sizeof(double));
     double eatm = d4_atm(&y, grad ? grad : NULL, dedcn_d4);
     double eband = 0.0;
@@ -1496,6 +1513,7 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
     if (grad) {
         /* grad already holds the ATM pair terms (without the CN chain) */
         double *dedcn = (double *)calloc(nat, sizeof(double)); /* GFN2 CN */
+        ORC_PHASE(PH_GRAD_W);
         /* W = 2 C diag(f e) C^T */
         double *W = work; /* n^2 */
         for (int mu = 0; mu < n; ++mu)
@@ -1505,6 +1523,7 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
                     v += F[(size_t)mu * n + k] * focc[k] * eps[k] * F[(size_t)nu * n + k];
                 W[(size_t)mu * n + nu] = W[(size_t)nu * n + mu] = 2.0 * v;
             }
+        ORC_PHASE(PH_GRAD_PAIR);
         /* repulsion */
         for (int i = 0; i < nat; ++i)
             for (int j = 0; j < i; ++j) {
@@ -1603,6 +1622,7 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
                 dedcn[i] += 0.5 * der0 * dri;
                 dedcn[j] += 0.5 * der0 * drj;
             }
+        ORC_PHASE(PH_GRAD_INT);
         /* integral terms over shell pairs on different atoms */
         double s[9], dpb[27], qpb[54], ds[27], ddp[81], dqp[162];
         double dpa[27], qpa[54], qta[54], qtb[54], dqta[162], dqtb[162], ddpa[81], dqpa[162];
@@ -1684,6 +1704,7 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
                 dedcn[A] += dh * (-pa->kcn[ka] * EVTOAU);
                 dedcn[B] += dh * (-pb->kcn[kb] * EVTOAU);
             }
+        ORC_PHASE(PH_GRAD_PAIR);
         /* GFN2 CN chain */
         for (int i = 0; i < nat; ++i)
             for (int j = 0; j < i; ++j) {
@@ -1702,6 +1723,7 @@ static int gfn2_run(int nat, const int32_t *num, const double *xyz, int charge, 
             }
         free(dedcn);
     }
+    ORC_PHASE(PH_OTHER);
     free(dedcn_d4);
     free(qat);
     if (orb) {
