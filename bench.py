@@ -190,21 +190,30 @@ def run_singlepoints(name, reps):
         desc = "64 single points: C12H24O2N2+ (40 atoms, 88 basis functions), bonds stretched <= +30 %"
     nbf = lambda num: int(sum(1 if z == 1 else 4 for z in num))
     sysl.sort(key=lambda t: -nbf(t[0]))
-    b = qcxms_amd.Batch([(t[0], t[1]) for t in sysl])
-    b.set_warm_start(0)
     xyz = np.concatenate([t[2] for t in sysl])
-    b.egrad(xyz, 5000.0)
-    b.set_profiling(True)
-    for k in ("setup", "scc", "grad"):
-        b.kernel_time(k)
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        e, g, q, st = b.egrad(xyz, 5000.0)
-    wall = time.perf_counter() - t0
-    kt = {k: b.kernel_time(k) for k in ("setup", "scc", "grad")}
-    iters = np.asarray(b.info()[0], dtype=np.float64)
-    b.close()
-    assert np.all(st == 0)
+    kernels = ("setup", "scc", "dblocks", "grad")
+
+    def timed_batch(stored):
+        b = qcxms_amd.Batch([(t[0], t[1]) for t in sysl])
+        b.set_warm_start(0)
+        b.set_stored_derivatives(stored)
+        b.egrad(xyz, 5000.0)
+        b.set_profiling(True)
+        for k in kernels:
+            b.kernel_time(k)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e, g, q, st = b.egrad(xyz, 5000.0)
+        wall = time.perf_counter() - t0
+        kt = {k: b.kernel_time(k) for k in kernels}
+        iters = np.asarray(b.info()[0], dtype=np.float64)
+        mem = b.device_bytes()
+        b.close()
+        assert np.all(st == 0)
+        return wall, kt, iters, mem, g
+
+    wall, kt, iters, mem, g_rec = timed_batch(False)
+    kt = {k: v for k, v in kt.items() if v[1]}
     # algorithmic flops of one launch of the three kernels together (cold SCC:
     # the same iterations on every repetition)
     flops = 0.0
@@ -213,6 +222,14 @@ def run_singlepoints(name, reps):
         flops += fx["setup"] + fx["int"] + fx["grad"] + fx["pair"] + it * scc_flops(fx["n"])
     kernel_ms = sum(v[0] for v in kt.values()) / max(reps, 1)
     achieved = flops / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
+    # the same single points with the derivative integral blocks stored by
+    # k_dblocks and contracted by the gradient, against recomputing them
+    w2, kt2, _, mem2, g_sto = timed_batch(True)
+    form = lambda w, k, m: {"value": len(sysl) * reps / w, "unit": "single-points/s",
+                            "device_mib": m / 2 ** 20,
+                            "kernels_ms_avg": {n: (v[0] / v[1]) for n, v in k.items() if v[1]}}
+    derivative_blocks = {"recomputed": form(wall, kt, mem), "stored": form(w2, kt2, mem2),
+                         "max_abs_grad_difference": float(np.abs(g_sto - g_rec).max())}
     return {"metric": "GFN2-xTB energy+gradient single points/sec on 1 GPU (batched)",
             "value": len(sysl) * reps / wall, "unit": "single-points/s", "n_gpus": 1,
             "reps": reps, "dtype": "f64", "data": "synthetic",
@@ -222,7 +239,8 @@ def run_singlepoints(name, reps):
             "roofline": {"bound": "mfma", "kernel": "k_setup + k_scc + k_grad",
                          "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP64_PEAK_TFLOPS, "flops_per_launch": flops,
-                         "source": "profiles/flop_model.json"}}
+                         "source": "profiles/flop_model.json"},
+            "derivative_blocks": derivative_blocks}
 
 
 def run_c20_md(ntraj, steps, warmup, repack=100):

@@ -93,6 +93,15 @@ int qcxms_batch_set_scc(qcxms_batch *b, double econv, double pconv, int32_t maxi
  * moments and orbitals (same fixed point, fewer iterations).  Default 0
  * (fresh wavefunction per call, as src/tblite.f90:133). */
 int qcxms_batch_set_warm_start(qcxms_batch *b, int32_t warm);
+/* on != 0: the gradient contracts derivative integral blocks that a kernel
+ * (k_dblocks) stores per AO pair of every off-atom shell pair (34 doubles per
+ * AO pair, allocated at the first request and kept); 0 (default): the
+ * gradient recomputes them from the primitives and stores nothing.  The
+ * results agree to rounding; the two forms trade device memory for work.
+ * No counterpart in the reference (tblite forms them inside its gradient). */
+int qcxms_batch_set_stored_derivatives(qcxms_batch *b, int32_t on);
+/* Device memory the batch holds (bytes), for the memory comparison above. */
+int qcxms_batch_device_bytes(qcxms_batch *b, uint64_t *bytes);
 /* Energy, gradient and Mulliken charges for all systems at xyz (host
  * arrays).  etemp[nsys] in K.  Outputs may be NULL when not needed. */
 int qcxms_batch_egrad(qcxms_batch *b, const double *xyz, const double *etemp, double *energy,
@@ -128,7 +137,8 @@ int qcxms_batch_mo_spectrum(qcxms_batch *b, int32_t sys, int32_t cap, int32_t *n
 /* enable != 0 records events around every kernel launch of this batch. */
 int qcxms_batch_set_profiling(qcxms_batch *b, int32_t enable);
 /* Accumulated kernel time (ms) and launch count for kernel names
- * "setup", "scc", "grad", "md_pre", "md_post"; counters reset on read. */
+ * "setup", "scc", "grad", "dblocks", "md_pre", "md_post", "fused"; counters
+ * reset on read. */
 int qcxms_batch_kernel_time(qcxms_batch *b, const char *kernel, double *ms, int32_t *launches);
 
 

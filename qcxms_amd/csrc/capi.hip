@@ -261,8 +261,10 @@ struct qcxms_batch {
     SccOpts opts{1e-9, 1e-8, 0.4, 1e-11, 100, 100, 0, 0, 0, 0};
     int warm_start = 0;
     int profiling = 0;
-    Timer t_setup, t_scc, t_grad, t_md_pre, t_md_post, t_fused;
+    Timer t_setup, t_scc, t_grad, t_md_pre, t_md_post, t_fused, t_dblk;
     std::vector<void*> allocs;
+    size_t alloc_bytes = 0;       // device memory the batch allocated
+    double* dblk_mem = nullptr;   // stored derivative blocks (allocated on request)
     BatchPtrs d{};
     SysInfo* d_info = nullptr;
     int* d_elem = nullptr;
@@ -302,6 +304,7 @@ int dalloc(qcxms_batch* b, T** p, size_t n)
     HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
     HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), b->stream));
     b->allocs.push_back((void*)*p);
+    b->alloc_bytes += n * sizeof(T);
     return QCXMS_OK;
 }
 
@@ -583,6 +586,11 @@ int enqueue_egrad(qcxms_batch* b, bool with_grad)
     if ((st = timed(b, b->t_scc, [&] {
              hipLaunchKernelGGL(b->opts.eig_mfma ? qx::k_scc<true> : qx::k_scc<false>, dim3(b->nsys),
                                 dim3(QX_SCC_BLOCK), l2, b->stream, b->d, launch_opts(b, l2));
+         })))
+        return st;
+    if (with_grad && b->d.dblk &&
+        (st = timed(b, b->t_dblk, [&] {
+             hipLaunchKernelGGL(qx::k_dblocks, dim3(b->nsys), dim3(QX_BLOCK), 0, b->stream, b->d);
          })))
         return st;
     if (with_grad &&
@@ -946,10 +954,33 @@ int qcxms_batch_kernel_time(qcxms_batch* b, const char* kernel, double* ms, int3
     else if (!std::strcmp(kernel, "md_pre")) t = &b->t_md_pre;
     else if (!std::strcmp(kernel, "md_post")) t = &b->t_md_post;
     else if (!std::strcmp(kernel, "fused")) t = &b->t_fused;
+    else if (!std::strcmp(kernel, "dblocks")) t = &b->t_dblk;
     else return fail("unknown kernel name");
     if (ms) *ms = t->ms;
     if (launches) *launches = t->launches;
     *t = Timer();
+    return QCXMS_OK;
+}
+
+int qcxms_batch_set_stored_derivatives(qcxms_batch* b, int32_t on)
+{
+    if (!b) return fail("null argument");
+    if (b->arena) return fail("stored derivative blocks need a batch with its own device memory");
+    if (on && !b->dblk_mem) {
+        // 9 AO-pair slots of qx::kDblk values per shell-pair work item
+        const size_t n = b->items.size() * 9 * (size_t)qx::kDblk;
+        HIPCHK(hipMalloc((void**)&b->dblk_mem, n * sizeof(double)));
+        b->allocs.push_back((void*)b->dblk_mem);
+        b->alloc_bytes += n * sizeof(double);
+    }
+    b->d.dblk = on ? b->dblk_mem : nullptr;
+    return QCXMS_OK;
+}
+
+int qcxms_batch_device_bytes(qcxms_batch* b, uint64_t* bytes)
+{
+    if (!b || !bytes) return fail("null argument");
+    *bytes = b->alloc_bytes;
     return QCXMS_OK;
 }
 

@@ -115,6 +115,7 @@ struct BatchPtrs {
     const int2* spitem;  // shell-pair work items (ia, ib), ia > ib or same shell
     double* spbuf;    // per shell pair (ia, ib): gradient on A (3), dE/dCN_A, dE/dCN_B
     double* aesg;     // multipole-kernel factors, QX_AES_COMP planes of nat x nat per system
+    double* dblk;     // stored derivative blocks per off-atom shell-pair item (null: recomputed)
     double* xprev;    // mixing vector of the step before (extrapolated MD start)
     double* xprev2;   // and of the step before that
     double* xprev3;   // and of the one before that (cubic extrapolation)

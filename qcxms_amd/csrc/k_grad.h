@@ -6,7 +6,9 @@
 //         + explicit pair terms (repulsion, ES2 gamma, AES kernels, D4)
 //         + CN chain rules (self-energies and multipole radii through the
 //           GFN2 CN, D4 weights through the covalent D4 CN).
-// Derivative integral blocks are recomputed on the fly (never stored).
+// Derivative integral blocks are recomputed on the fly by default; with
+// B.dblk set, k_dblocks has stored them per AO pair (contracted over the
+// primitives) and the gradient only contracts the stored values.
 // All reductions are ordered: every partial result has one owner thread, so
 // the gradient is bit-reproducible from run to run.
 #pragma once
@@ -43,12 +45,102 @@ QX_DEV void grad_shell_pair(const ElemParams& ea, int ka, const ElemParams& eb, 
     }
 }
 
+// Stored derivative blocks: per off-atom shell-pair work item 9 AO-pair slots
+// (bra component * 3 + ket component) of kDblk values: s, dpb[3], ds[3],
+// ddp[3][3], dqp[3][6] (the DerivAcc members that the density weights, each
+// summed over the primitive pairs).
+constexpr int kDblk = 34;
+
+template <int LA, int LB, int A = 0, int B = 0>
+QX_DEV void store_derivs(const ElemParams& ea, int ka, const ElemParams& eb, int kb, const double* ra,
+                         const double* rb, double* out)
+{
+    DerivAcc acc{};
+    const int nga = ea.nprim[ka], ngb = eb.nprim[kb];
+    for (int ip = 0; ip < nga; ++ip) {
+        const double al = ea.alpha[ka][ip], ca = ea.coeff[ka][ip];
+        for (int jp = 0; jp < ngb; ++jp) {
+            PrimPair q;
+            prim_pair(al, eb.alpha[kb][jp], ca * eb.coeff[kb][jp], ra, rb, q);
+            ao_derivs<LA, LB, A, B>(q, 1.0, 0.0, acc);
+        }
+    }
+    double* o = out + (A * 3 + B) * kDblk;
+    o[0] = acc.s;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        o[1 + d] = acc.dpb[d];
+        o[4 + d] = acc.ds[d];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[7 + 3 * d + c] = acc.ddp[d][c];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) o[16 + 6 * d + k] = acc.dqp[d][k];
+    }
+    if constexpr (LB == 1 && B < 2) store_derivs<LA, LB, A, B + 1>(ea, ka, eb, kb, ra, rb, out);
+    else if constexpr (LA == 1 && A < 2) store_derivs<LA, LB, A + 1, 0>(ea, ka, eb, kb, ra, rb, out);
+}
+
+// The derivative blocks of every off-atom shell pair of every system, stored
+// for the gradient (one workgroup per system, one thread per work item).
+__global__ __launch_bounds__(QX_BLOCK) void k_dblocks(BatchPtrs B)
+{
+    const int sys = blockIdx.x;
+    if (B.active && !B.active[sys]) return;
+    const SysInfo si = B.info[sys];
+    __shared__ ShellDesc sh[QX_MAX_NSH];
+    __shared__ int ao_at[QX_MAX_NAO], ao_sh[QX_MAX_NAO];
+    __shared__ double sx[3 * QX_MAX_NAT];
+    build_shell_tables(si, B.elem, sh, ao_at, ao_sh);
+    for (int k = threadIdx.x; k < 3 * si.nat; k += QX_BLOCK) sx[k] = B.xyz[3 * (size_t)si.atom_off + k];
+    __syncthreads();
+    for (int it = threadIdx.x; it < si.nitem_off; it += QX_BLOCK) {
+        const int2 pr = B.spitem[si.item_off + it];
+        const ShellDesc sa = sh[pr.x], sb = sh[pr.y];
+        const ElemParams& ea = g_tab.e[sa.el];
+        const ElemParams& eb = g_tab.e[sb.el];
+        const double* ra = sx + 3 * sa.at;
+        const double* rb = sx + 3 * sb.at;
+        double* out = B.dblk + (size_t)(si.item_off + it) * 9 * kDblk;
+        switch (sa.l * 2 + sb.l) {
+        case 3: store_derivs<1, 1>(ea, sa.k, eb, sb.k, ra, rb, out); break;
+        case 2: store_derivs<1, 0>(ea, sa.k, eb, sb.k, ra, rb, out); break;
+        case 1: store_derivs<0, 1>(ea, sa.k, eb, sb.k, ra, rb, out); break;
+        default: store_derivs<0, 0>(ea, sa.k, eb, sb.k, ra, rb, out); break;
+        }
+    }
+}
+
+// Density-weighted sums of the stored blocks of one work item (the same
+// DerivAcc that grad_shell_pair forms from the primitives)
+QX_DEV void grad_stored_pair(const double* blk, int na, int nb, const double* P, const double* W, int ld,
+                             int mu0, int nu0, DerivAcc& acc)
+{
+    for (int a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b) {
+            const size_t o = (size_t)(mu0 + a) * ld + nu0 + b;
+            const double pm = P[o], wm = W[o];
+            const double* v = blk + (a * 3 + b) * kDblk;
+            acc.s += pm * v[0];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                acc.dpb[d] += pm * v[1 + d];
+                acc.ds[d] += pm * v[4 + d];
+                acc.wds[d] += wm * v[4 + d];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc.ddp[d][c] += pm * v[7 + 3 * d + c];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc.dqp[d][k] += pm * v[16 + 6 * d + k];
+            }
+        }
+}
+
 // LDS of one system's gradient
 struct GradLds {
     SysShared ss;
     double dedcn[QX_MAX_NAT], dedcn4[QX_MAX_NAT], gacc[3 * QX_MAX_NAT];
 };
 
+template <bool STORED = false>
 QX_DEV void grad_system(const BatchPtrs& B, int sys, GradLds& Lg)
 {
     const SysInfo si = B.info[sys];
@@ -139,11 +231,16 @@ QX_DEV void grad_system(const BatchPtrs& B, int sys, GradLds& Lg)
 #pragma unroll
             for (int k = 0; k < 6; ++k) acc.dqp[d][k] = 0.0;
         }
-        switch (sa.l * 2 + sb.l) {
-        case 3: grad_shell_pair<1, 1>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
-        case 2: grad_shell_pair<1, 0>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
-        case 1: grad_shell_pair<0, 1>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
-        default: grad_shell_pair<0, 0>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
+        if constexpr (STORED) {
+            grad_stored_pair(B.dblk + (size_t)(si.item_off + it) * 9 * kDblk, sa.nao, sb.nao, P, W, ld,
+                             sa.ao0, sb.ao0, acc);
+        } else {
+            switch (sa.l * 2 + sb.l) {
+            case 3: grad_shell_pair<1, 1>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
+            case 2: grad_shell_pair<1, 0>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
+            case 1: grad_shell_pair<0, 1>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
+            default: grad_shell_pair<0, 0>(ea, sa.k, eb, sb.k, ra, rb, P, W, ld, sa.ao0, sb.ao0, acc); break;
+            }
         }
         // potential contraction: moments about A follow from those about B by
         // the shift t, which brings in the value terms s and dpb
@@ -428,7 +525,8 @@ __global__ __launch_bounds__(QX_BLOCK) void k_grad(BatchPtrs B)
     const int sys = blockIdx.x;
     if (B.active && !B.active[sys]) return;
     __shared__ GradLds Lg;
-    grad_system(B, sys, Lg);
+    if (B.dblk) grad_system<true>(B, sys, Lg);
+    else grad_system<false>(B, sys, Lg);
 }
 
 }  // namespace qx

@@ -42,6 +42,8 @@ def library():
         L.qcxms_batch_set_scc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32,
                                           C.c_double, C.c_double]
         L.qcxms_batch_set_warm_start.argtypes = [C.c_void_p, C.c_int32]
+        L.qcxms_batch_set_stored_derivatives.argtypes = [C.c_void_p, C.c_int32]
+        L.qcxms_batch_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         L.qcxms_batch_set_profiling.argtypes = [C.c_void_p, C.c_int32]
         L.qcxms_batch_kernel_time.argtypes = [C.c_void_p, C.c_char_p, _dp, _ip]
         L.qcxms_batch_egrad.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _ip]
@@ -136,6 +138,17 @@ class Batch:
 
     def set_warm_start(self, warm):
         library().qcxms_batch_set_warm_start(self._h, int(bool(warm)))
+
+    def set_stored_derivatives(self, on):
+        """on: the gradient contracts stored derivative blocks (more device
+        memory); off (default): it recomputes them."""
+        if library().qcxms_batch_set_stored_derivatives(self._h, int(bool(on))):
+            raise RuntimeError("qcxms_batch_set_stored_derivatives failed: " + last_error())
+
+    def device_bytes(self):
+        n = C.c_uint64()
+        library().qcxms_batch_device_bytes(self._h, C.byref(n))
+        return n.value
 
     def set_profiling(self, on):
         library().qcxms_batch_set_profiling(self._h, int(bool(on)))

@@ -1,10 +1,14 @@
 // Driver of the CPU-emulated library build: qcxms_egrad_batch on systems read
-// from a binary file, results written to another.
+// from a binary file, results written to another.  With QX_EMU_STORED=1 in
+// the environment the systems go through a persistent batch whose gradient
+// contracts stored derivative blocks (qcxms_batch_set_stored_derivatives).
 //   input : int32 nsys, method; int32 offsets[nsys+1]; int32 num[nat];
 //           double xyz[3 nat]; int32 charge[nsys], mult[nsys]; double etemp[nsys]
 //   output: int32 rc; double energy[nsys]; double grad[3 nat]; double qat[nat];
 //           int32 stat[nsys]
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "../../include/qcxms_gpu.h"
@@ -36,8 +40,18 @@ int main(int argc, char** argv)
     std::fclose(f);
     std::vector<double> e(nsys), g(3 * (size_t)nat), q(nat);
     std::vector<int32_t> st(nsys);
-    int32_t rc = qcxms_egrad_batch(nsys, off.data(), num.data(), xyz.data(), chg.data(), mult.data(), method,
-                                   et.data(), q.data(), e.data(), g.data(), st.data());
+    int32_t rc;
+    const char* stored = std::getenv("QX_EMU_STORED");
+    if (stored && !std::strcmp(stored, "1")) {
+        qcxms_batch* b = qcxms_batch_create(nsys, off.data(), num.data(), chg.data(), method);
+        rc = b ? qcxms_batch_set_stored_derivatives(b, 1) : -2;
+        if (rc == 0) rc = qcxms_batch_set_scc(b, 1e-11, 1e-9, 100, 0.4, 1e-11);
+        if (rc == 0) rc = qcxms_batch_egrad(b, xyz.data(), et.data(), e.data(), g.data(), q.data(), st.data());
+        if (b) qcxms_batch_destroy(b);
+    } else {
+        rc = qcxms_egrad_batch(nsys, off.data(), num.data(), xyz.data(), chg.data(), mult.data(), method,
+                               et.data(), q.data(), e.data(), g.data(), st.data());
+    }
     if (rc != 0) std::fprintf(stderr, "qcxms_egrad_batch: %d %s\n", rc, qcxms_gpu_last_error());
     FILE* o = std::fopen(argv[2], "wb");
     if (!o) return 2;

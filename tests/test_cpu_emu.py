@@ -41,6 +41,26 @@ def test_emulated_kernels_bounds_and_parity():
 
 
 @pytest.mark.timeout(900)
+def test_emulated_stored_derivative_blocks():
+    """k_dblocks and the gradient from stored blocks, emulated with the
+    sanitizers: every access in bounds, and the oracle's results."""
+    eb.build()
+    systems, coords = [], []
+    for s in (5026, 5004):
+        num, xyz, charge = W.random_chain(s)
+        systems.append((num, charge, W.getspin(num, charge)))
+        coords.append(xyz)
+    rc, e, g, q, st, err = eb.egrad_batch(systems, coords, 5000.0, stored=True)
+    assert rc == 0 and np.all(st == 0), err
+    re, rg, rq, rst = ob.gfn2_egrad_batch([s[:2] for s in systems], coords, 5000.0,
+                                          opts=ob.default_opts(econv=1e-12, pconv=1e-10))
+    de = np.abs(e - re).max()
+    dg = max(np.abs(g[i] - rg[i]).max() for i in range(len(systems)))
+    print(f"emulated stored-block gradient: dE={de:.2e} dg={dg:.2e}")
+    assert de <= 1e-8 and dg <= 1e-6
+
+
+@pytest.mark.timeout(900)
 def test_emulated_md_heating_fused_and_stepwise():
     """Three MD steps inside the heating window (impactscale every step) of a
     5-atom and a 3-atom trajectory through the emulated library: one

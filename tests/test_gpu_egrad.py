@@ -109,9 +109,54 @@ def test_warm_start_same_fixed_point():
     b.set_warm_start(True)
     e0, g0, q0, _ = b.egrad(xyz, 5000.0)
     it0 = b.info()[0][0]
-    xyz2 = xyz + 0.01
+    # a new geometry (every atom displaced independently, as after an MD step)
+    xyz2 = xyz + np.random.default_rng(11).normal(scale=0.01, size=xyz.shape)
     e1, g1, q1, st = b.egrad(xyz2, 5000.0)
     it1 = b.info()[0][0]
     assert st[0] == 0
     assert it1 < it0
     check(num, xyz2, 1, 5000.0, e1[0], g1, q1, "warm")
+
+
+@pytest.mark.parametrize("case", ["octane", "c12_stretched_global", "ragged"])
+def test_stored_derivative_blocks_give_the_same_gradient(case):
+    """The gradient from stored derivative blocks (k_dblocks) equals the
+    recomputed one to rounding, on LDS and global-memory systems and a ragged
+    batch, and matches the oracle; the stored form holds more device memory."""
+    q = gpu()
+    if case == "octane":
+        systems = [W.octane(seed=3)]
+        charges = [1]
+    elif case == "c12_stretched_global":
+        systems = [W.chain_c12h24o2n2(seed=2)]
+        charges = [1]
+    else:
+        systems, charges = [], []
+        for s in range(6):
+            num, xyz, chg = W.random_chain(300 + s)
+            systems.append((num, xyz))
+            charges.append(chg)
+    b = q.Batch([(n, c) for (n, _), c in zip(systems, charges)])
+    b.set_scc(econv=1e-11, pconv=1e-9)
+    xyz = np.concatenate([x for _, x in systems])
+    e0, g0, q0, st0 = b.egrad(xyz, 5000.0)
+    mem0 = b.device_bytes()
+    b.set_stored_derivatives(True)
+    e1, g1, q1, st1 = b.egrad(xyz, 5000.0)
+    mem1 = b.device_bytes()
+    b.set_stored_derivatives(False)
+    e2, g2, _, _ = b.egrad(xyz, 5000.0)
+    b.close()
+    dg = np.abs(g1 - g0).max()
+    print(f"{case}: max|g_stored - g_recomputed| {dg:.2e}; device memory {mem0 / 2**20:.1f} -> "
+          f"{mem1 / 2**20:.1f} MiB", flush=True)
+    assert np.all(st0 == 0) and np.all(st1 == 0)
+    assert np.array_equal(e0, e1) and np.array_equal(q0, q1)
+    assert dg <= 1e-10
+    assert np.array_equal(g2, g0)
+    assert mem1 > mem0
+    off = 0
+    for i, ((num, x), c) in enumerate(zip(systems, charges)):
+        k = slice(off, off + len(num))
+        check(num, x, c, 5000.0, e1[i], g1[k], q1[k], f"{case} system {i}")
+        off += len(num)
