@@ -214,6 +214,18 @@ def run_singlepoints(name, reps):
 
     wall, kt, iters, mem, g_rec = timed_batch(False)
     kt = {k: v for k, v in kt.items() if v[1]}
+    # the same batch with the launch plan before tuning (one launch, Jacobi,
+    # LDS matrices up to 56 basis functions)
+    table, mixed = qcxms_amd.get_tuning()
+    qcxms_amd.set_tuning([(256, 0, 1)], (0, 1))
+    try:
+        w_un, kt_un, _, _, _ = timed_batch(False)
+    finally:
+        qcxms_amd.set_tuning(table, mixed)
+    autotuned = {"table": table, "mixed": mixed,
+                 "untuned_value": len(sysl) * reps / w_un,
+                 "untuned_scc_ms_avg": kt_un["scc"][0] / max(kt_un["scc"][1], 1),
+                 "source": "qcxms_amd/autotune.py, profiles/tuning_mi355x.json"}
     # algorithmic flops of one launch of the three kernels together (cold SCC:
     # the same iterations on every repetition)
     flops = 0.0
@@ -240,7 +252,8 @@ def run_singlepoints(name, reps):
                          "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP64_PEAK_TFLOPS, "flops_per_launch": flops,
                          "source": "profiles/flop_model.json"},
-            "derivative_blocks": derivative_blocks}
+            "derivative_blocks": derivative_blocks,
+            "autotuned_launch_plan": autotuned}
 
 
 def run_c20_md(ntraj, steps, warmup, repack=100):

@@ -93,6 +93,22 @@ int qcxms_batch_set_scc(qcxms_batch *b, double econv, double pconv, int32_t maxi
  * moments and orbitals (same fixed point, fewer iterations).  Default 0
  * (fresh wavefunction per call, as src/tblite.f90:133). */
 int qcxms_batch_set_warm_start(qcxms_batch *b, int32_t warm);
+/* SCC launch plan of single-point batches, chosen per basis size by the
+ * autotuner (qcxms_amd/autotune.py; its MI355X table is compiled in): bin k
+ * takes the systems with nao <= max_nao[k] (the first bin that fits, the
+ * last one whatever is larger); eig[k] = 1 runs the MFMA eigen refinement
+ * before Jacobi, lds[k] = 1 keeps SCC matrices in LDS where they fit (up to
+ * 56 basis functions).  A batch whose systems fall into one bin launches
+ * with that bin's choice; a batch spanning several bins launches once with
+ * (mixed_eig, mixed_lds), or, with mixed_eig = -1, once per contiguous run of
+ * systems of one bin, LDS sized for that run, on streams forked from and
+ * joined to its own.  The table is read when a batch is created; nbins = 0
+ * makes later batches use one launch with their own options.  MD ensembles
+ * always use one launch.  No counterpart in the reference. */
+int qcxms_set_tuning(int32_t nbins, const int32_t *max_nao, const int32_t *eig, const int32_t *lds,
+                     int32_t mixed_eig, int32_t mixed_lds);
+int qcxms_get_tuning(int32_t cap, int32_t *nbins, int32_t *max_nao, int32_t *eig, int32_t *lds,
+                     int32_t *mixed_eig, int32_t *mixed_lds);
 /* on != 0: the gradient contracts derivative integral blocks that a kernel
  * (k_dblocks) stores per AO pair of every off-atom shell pair (34 doubles per
  * AO pair, allocated at the first request and kept); 0 (default): the

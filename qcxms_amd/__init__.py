@@ -6,10 +6,11 @@ reference's Fortran interface (src/tblite.f90 get_xtb_egrad, src/md.f90).
 """
 from .gpu import (EXIT_REASONS, GFN1_XTB, GFN2_XTB, IPEA1_XTB, Batch, FragmentReport,
                   EiEnsemble, LibraryMissing, MdEnsemble, egrad_batch, egrad_many, fragment_ips_batch,
-                  fragments_batch, get_xtb_egrad, get_xtb_mo_spectrum, impactscale_batch,
-                  leapfrog_batch, library, library_path, manage_fragments_batch)
+                  fragments_batch, get_tuning, get_xtb_egrad, get_xtb_mo_spectrum,
+                  impactscale_batch, leapfrog_batch, library, library_path, manage_fragments_batch,
+                  set_tuning)
 
 __all__ = ["EXIT_REASONS", "GFN1_XTB", "GFN2_XTB", "IPEA1_XTB", "Batch", "FragmentReport",
            "EiEnsemble", "LibraryMissing", "MdEnsemble", "egrad_batch", "egrad_many", "fragment_ips_batch",
-           "fragments_batch", "get_xtb_egrad", "get_xtb_mo_spectrum", "impactscale_batch", "leapfrog_batch", "library",
-           "library_path", "manage_fragments_batch"]
+           "fragments_batch", "get_tuning", "get_xtb_egrad", "get_xtb_mo_spectrum", "impactscale_batch", "leapfrog_batch", "library",
+           "library_path", "manage_fragments_batch", "set_tuning"]

@@ -38,6 +38,7 @@ __global__ __launch_bounds__(256) void k_fill_zero(double* p, size_t n)
 #include "k_grad.h"
 #include "k_md.h"
 #include "k_spec.h"
+#include "tuning_table.h"
 
 namespace {
 
@@ -51,6 +52,21 @@ unsigned char* g_scratch_dev = nullptr;  // device address of scratch on g_devic
 std::mutex g_scratch_mu;                 // one allocation-free call at a time
 size_t g_eig_dyn_max = 0;  // the same with the MFMA eigen refinement's fifth matrix
 DevTables g_host_tab;
+
+// Launch choices of the SCC for single-point batches by basis size: bin k
+// takes the systems with nao <= max_nao (the first bin that fits); eig: MFMA
+// eigen refinement, lds: SCC matrices in LDS where they fit.  A batch whose
+// systems fall into one bin takes that bin's choice; a batch spanning several
+// bins takes the mixed choice in one launch (eig >= 0) or one launch per run
+// of systems of one bin (eig < 0).  The compiled-in table is the MI355X
+// autotuner's (tuning_table.h); qcxms_set_tuning replaces it for batches
+// created afterwards.  No bins: one launch with the batch's own options.
+struct TuneBin {
+    int max_nao, eig, lds;
+};
+std::vector<TuneBin> g_tune;
+TuneBin g_tune_mixed{0, -1, 0};
+bool g_tune_init = false;
 
 int fail(const std::string& msg)
 {
@@ -239,6 +255,11 @@ int ensure_init(int device)
     HIPCHK(hipFuncSetAttribute((const void*)qx::k_md_fused<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)g_scc_dyn_max));
     g_device = device;
+    if (!g_tune_init) {
+        for (const auto& t : qx::kTuneTable) g_tune.push_back({t[0], t[1], t[2]});
+        g_tune_mixed = {0, qx::kTuneMixed[0], qx::kTuneMixed[1]};
+        g_tune_init = true;
+    }
     g_ready = true;
     return QCXMS_OK;
 }
@@ -265,6 +286,13 @@ struct qcxms_batch {
     std::vector<void*> allocs;
     size_t alloc_bytes = 0;       // device memory the batch allocated
     double* dblk_mem = nullptr;   // stored derivative blocks (allocated on request)
+    // SCC launch plan by basis-size bin (single-point batches; empty: one
+    // launch): side streams forked from and joined to `stream`
+    std::vector<TuneBin> tune;
+    TuneBin tune_mixed{0, -1, 0};
+    static constexpr int kSide = 4;
+    hipStream_t side[kSide] = {};
+    hipEvent_t fork = nullptr, join[kSide] = {};
     BatchPtrs d{};
     SysInfo* d_info = nullptr;
     int* d_elem = nullptr;
@@ -281,6 +309,11 @@ struct qcxms_batch {
     {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : allocs) (void)hipFree(p);
+        for (int k = 0; k < kSide; ++k) {
+            if (side[k]) (void)hipStreamDestroy(side[k]);
+            if (join[k]) (void)hipEventDestroy(join[k]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
         if (stream && owns_stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -552,18 +585,23 @@ size_t lds_setup_bytes(const qcxms_batch* b)
 }
 // four m x m working matrices and the AO row sums for the systems that keep
 // them in LDS
-size_t lds_scc_bytes(const qcxms_batch* b)
+size_t lds_scc_bytes_range(const qcxms_batch* b, int s0, int s1, int lds_max_m, int eig)
 {
     size_t lm = 0;
-    for (const SysInfo& si : b->info) {
-        const size_t m = (size_t)((si.nao + 1) & ~1);
-        if (m <= (size_t)qx::kFastM) lm = std::max(lm, m);
+    for (int s = s0; s < s1; ++s) {
+        const size_t m = (size_t)((b->info[s].nao + 1) & ~1);
+        if (m <= (size_t)lds_max_m) lm = std::max(lm, m);
     }
     const size_t base = sizeof(double) * (4 * lm * (lm + 1) + qx::kRs * lm);
     // a fifth matrix for the MFMA eigen refinement when it is on and the LDS
     // holds it
     const size_t ext = base + sizeof(double) * lm * (lm + 1);
-    return b->opts.eig_mfma && ext <= g_eig_dyn_max ? ext : base;
+    return eig && ext <= g_eig_dyn_max ? ext : base;
+}
+
+size_t lds_scc_bytes(const qcxms_batch* b)
+{
+    return lds_scc_bytes_range(b, 0, b->nsys, qx::kFastM, b->opts.eig_mfma);
 }
 
 // the SCC options of one launch with its dynamic LDS size
@@ -571,7 +609,72 @@ SccOpts launch_opts(const qcxms_batch* b, size_t lds)
 {
     SccOpts o = b->opts;
     o.lds_doubles = (int)(lds / sizeof(double));
+    o.lds_max_m = qx::kFastM;
+    o.first_sys = 0;
     return o;
+}
+
+// bin of a basis size in a tuning table (the last bin takes what is larger)
+int tune_bin(const std::vector<TuneBin>& t, int nao)
+{
+    for (size_t k = 0; k < t.size(); ++k)
+        if (nao <= t[k].max_nao) return (int)k;
+    return (int)t.size() - 1;
+}
+
+// The SCC by launch plan: contiguous runs of systems in one bin, each run
+// one launch with its bin's options and LDS sized for its own systems, the
+// runs spread over side streams forked from the batch stream and joined back.
+int enqueue_scc_plan(qcxms_batch* b)
+{
+    struct Run {
+        int s0, s1, bin;
+    };
+    std::vector<Run> runs;
+    for (int s = 0; s < b->nsys; ++s) {
+        const int k = tune_bin(b->tune, b->info[s].nao);
+        if (runs.empty() || runs.back().bin != k) runs.push_back({s, s + 1, k});
+        else runs.back().s1 = s + 1;
+    }
+    auto launch = [&](const Run& r, hipStream_t st) {
+        const TuneBin& t = r.bin >= 0 ? b->tune[r.bin] : b->tune_mixed;
+        const int lmax = t.lds ? qx::kFastM : 0;
+        const size_t lds = lds_scc_bytes_range(b, r.s0, r.s1, lmax, t.eig);
+        SccOpts o = b->opts;
+        o.eig_mfma = t.eig;
+        o.lds_doubles = (int)(lds / sizeof(double));
+        o.lds_max_m = lmax;
+        o.first_sys = r.s0;
+        hipLaunchKernelGGL(t.eig ? qx::k_scc<true> : qx::k_scc<false>, dim3(r.s1 - r.s0), dim3(QX_SCC_BLOCK),
+                           lds, st, b->d, o);
+        return hipGetLastError();
+    };
+    if (runs.size() == 1) {
+        HIPCHK(launch(runs[0], b->stream));
+        return QCXMS_OK;
+    }
+    if (b->tune_mixed.eig >= 0) {
+        // systems of several bins in one launch with the mixed choice
+        HIPCHK(launch(Run{0, b->nsys, -1}, b->stream));
+        return QCXMS_OK;
+    }
+    if (!b->fork) {
+        HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        for (int k = 0; k < qcxms_batch::kSide; ++k) {
+            HIPCHK(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming));
+        }
+    }
+    HIPCHK(hipEventRecord(b->fork, b->stream));
+    const int nside = std::min<int>((int)runs.size(), qcxms_batch::kSide);
+    for (int k = 0; k < nside; ++k) HIPCHK(hipStreamWaitEvent(b->side[k], b->fork, 0));
+    // largest systems first (the batch's order), dealt round-robin
+    for (size_t r = 0; r < runs.size(); ++r) HIPCHK(launch(runs[r], b->side[r % nside]));
+    for (int k = 0; k < nside; ++k) {
+        HIPCHK(hipEventRecord(b->join[k], b->side[k]));
+        HIPCHK(hipStreamWaitEvent(b->stream, b->join[k], 0));
+    }
+    return QCXMS_OK;
 }
 
 // setup -> SCC -> gradient for every system of the batch (masked by d.active)
@@ -583,11 +686,15 @@ int enqueue_egrad(qcxms_batch* b, bool with_grad)
              hipLaunchKernelGGL(qx::k_setup, dim3(b->nsys), dim3(QX_BLOCK), l1, b->stream, b->d, 1);
          })))
         return st;
+    int plan_st = QCXMS_OK;
     if ((st = timed(b, b->t_scc, [&] {
-             hipLaunchKernelGGL(b->opts.eig_mfma ? qx::k_scc<true> : qx::k_scc<false>, dim3(b->nsys),
-                                dim3(QX_SCC_BLOCK), l2, b->stream, b->d, launch_opts(b, l2));
+             if (!b->tune.empty()) plan_st = enqueue_scc_plan(b);
+             else
+                 hipLaunchKernelGGL(b->opts.eig_mfma ? qx::k_scc<true> : qx::k_scc<false>, dim3(b->nsys),
+                                    dim3(QX_SCC_BLOCK), l2, b->stream, b->d, launch_opts(b, l2));
          })))
         return st;
+    if (plan_st) return plan_st;
     if (with_grad && b->d.dblk &&
         (st = timed(b, b->t_dblk, [&] {
              hipLaunchKernelGGL(qx::k_dblocks, dim3(b->nsys), dim3(QX_BLOCK), 0, b->stream, b->d);
@@ -749,6 +856,7 @@ qcxms_md* md_build(const qcxms_md* old, const MdLayout& L, const qcxms_md_params
 {
     const int ntraj = (int)L.src.size();
     qcxms_batch* b = qcxms_batch_create(ntraj, L.offsets.data(), L.num.data(), L.charge.data(), QCXMS_GFN2_XTB);
+    if (b) b->tune.clear();  // the MD kernels run one launch per step (or a persistent one)
     if (!b) return nullptr;
     qcxms_md* md = new qcxms_md();
     md->b = b;
@@ -890,6 +998,11 @@ qcxms_batch* qcxms_batch_create(int32_t nsys, const int32_t* offsets, const int3
         return nullptr;
     }
     qcxms_batch* b = new qcxms_batch();
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        b->tune = g_tune;
+        b->tune_mixed = g_tune_mixed;
+    }
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         g_err = "stream creation failed";
         b->stream = nullptr;
@@ -959,6 +1072,49 @@ int qcxms_batch_kernel_time(qcxms_batch* b, const char* kernel, double* ms, int3
     if (ms) *ms = t->ms;
     if (launches) *launches = t->launches;
     *t = Timer();
+    return QCXMS_OK;
+}
+
+int qcxms_set_tuning(int32_t nbins, const int32_t* max_nao, const int32_t* eig, const int32_t* lds,
+                     int32_t mixed_eig, int32_t mixed_lds)
+{
+    if (mixed_eig < -1 || mixed_eig > 1 || (mixed_lds != 0 && mixed_lds != 1))
+        return fail("tuning table: the mixed choice is eig -1 (binned launches), 0 or 1 and lds 0 or 1");
+    if (nbins < 0 || nbins > 32) return fail("tuning table: 0 to 32 bins");
+    if (nbins > 0 && (!max_nao || !eig || !lds)) return fail("null argument");
+    std::vector<TuneBin> t;
+    for (int k = 0; k < nbins; ++k) {
+        if (max_nao[k] < 1 || (k > 0 && max_nao[k] <= max_nao[k - 1]))
+            return fail("tuning table: bin limits must be positive and increasing");
+        if ((eig[k] != 0 && eig[k] != 1) || (lds[k] != 0 && lds[k] != 1))
+            return fail("tuning table: eig and lds are 0 or 1");
+        t.push_back({max_nao[k], eig[k], lds[k]});
+    }
+    const int st = ensure_init(0);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_tune = t;
+    g_tune_mixed = {0, mixed_eig, mixed_lds};
+    g_tune_init = true;
+    return QCXMS_OK;
+}
+
+int qcxms_get_tuning(int32_t cap, int32_t* nbins, int32_t* max_nao, int32_t* eig, int32_t* lds,
+                     int32_t* mixed_eig, int32_t* mixed_lds)
+{
+    if (!nbins) return fail("null argument");
+    const int st = ensure_init(0);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(g_mu);
+    *nbins = (int32_t)g_tune.size();
+    if (mixed_eig) *mixed_eig = g_tune_mixed.eig;
+    if (mixed_lds) *mixed_lds = g_tune_mixed.lds;
+    if (cap < (int)g_tune.size()) return g_tune.empty() ? QCXMS_OK : fail("tuning buffers too small");
+    for (size_t k = 0; k < g_tune.size(); ++k) {
+        if (max_nao) max_nao[k] = g_tune[k].max_nao;
+        if (eig) eig[k] = g_tune[k].eig;
+        if (lds) lds[k] = g_tune[k].lds;
+    }
     return QCXMS_OK;
 }
 

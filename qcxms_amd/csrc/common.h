@@ -71,6 +71,9 @@ struct SccOpts {
     int keep;         // > 0: Broyden secant pairs kept across warm-started calls (ring of keep)
     int lds_doubles;  // dynamic LDS of the launch in doubles (set per launch by the host)
     int eig_mfma;     // 1: MFMA simultaneous-rotation eigen refinement before Jacobi
+    int lds_max_m;    // systems of at most this (even) basis size keep their SCC matrices in
+                      // LDS, the others in global memory (set per launch, <= qx::kFastM)
+    int first_sys;    // system of workgroup 0 (a launch over a range of the batch's systems)
 };
 
 // Device-resident batch state.  Every pointer is a flat array over all systems.

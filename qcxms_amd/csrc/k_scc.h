@@ -14,8 +14,8 @@
 // The Jacobi tolerance follows the SCC: 1e-3 of the previous iteration's RMS
 // moment change, never looser than 1e-6 and never tighter than the requested
 // floor, so the converged iterate is diagonalised to the floor.
-// Systems with at most kFastM basis functions keep A, C, F and P in LDS;
-// larger ones use per-system global scratch.  C starts as X U with X = L^-T
+// Systems with at most opt.lds_max_m (<= kFastM) basis functions keep A, C,
+// F and P in LDS; larger ones use per-system global scratch.  C starts as X U with X = L^-T
 // and U the previous call's orthogonal basis (identity on a cold start).
 #pragma once
 #include "k_setup.h"
@@ -708,12 +708,11 @@ struct SccBuffers {
 };
 
 template <bool FAST, bool EIG>
-QX_DEV void scc_body(const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, SysShared& ss,
+QX_DEV void scc_body(int sys, const SysInfo& si, const BatchPtrs& B, const SccOpts& opt, SysShared& ss,
                      SccBuffers bf, double* xin, double* xout, double* vs, double* vd, double* vq,
                      double* eps, double* focc, double* f2, double2* rot, int2* rpq, double* s_e,
                      int* s_any)
 {
-    const int sys = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nat = si.nat, nsh = si.nsh, n = si.nao, ld = si.ld, nx = si.nx;
     const int m = (n + 1) & ~1;
@@ -1123,21 +1122,21 @@ QX_DEV void scc_system(const BatchPtrs& B, const SccOpts& opt, int sys, SccLds& 
     const SysInfo si = B.info[sys];
     load_system(si, B, L.ss);
     const int m = (si.nao + 1) & ~1;
-    if (m <= kFastM) {
+    if (m <= opt.lds_max_m) {
         // odd leading dimension: column-wise LDS accesses hit distinct banks
         const size_t mm = (size_t)m * (m + 1);
         // a fifth matrix after the row sums when the launch's LDS holds it
         double* E = EIG && 5 * mm + kRs * (size_t)m <= (size_t)opt.lds_doubles ? dyn + 4 * mm + kRs * m
                                                                             : nullptr;
         SccBuffers bf{dyn, dyn + mm, dyn + 2 * mm, dyn + 3 * mm, dyn + 4 * mm, m + 1, E};
-        scc_body<true, EIG>(si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc, L.f2,
+        scc_body<true, EIG>(sys, si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc, L.f2,
                        L.rot, L.rpq, L.s_e, L.s_any);
     } else {
         // the energy-weighted density slot doubles as the row-sum scratch;
         // the MFMA eigen refinement's fifth matrix is the batch's E slot
         SccBuffers bf{B.T + si.mat_off, B.C + si.mat_off, B.U + si.mat_off, B.P + si.mat_off,
                       B.W + si.mat_off, si.ld, EIG ? B.E + si.mat_off : nullptr};
-        scc_body<false, EIG>(si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc,
+        scc_body<false, EIG>(sys, si, B, opt, L.ss, bf, L.xin, L.xout, L.vs, L.vd, L.vq, L.eps, L.focc,
                         L.f2, L.rot, L.rpq, L.s_e, L.s_any);
     }
 }
@@ -1145,7 +1144,7 @@ QX_DEV void scc_system(const BatchPtrs& B, const SccOpts& opt, int sys, SccLds& 
 template <bool EIG>
 __global__ __launch_bounds__(kNT) void k_scc(BatchPtrs B, SccOpts opt)
 {
-    const int sys = blockIdx.x;
+    const int sys = opt.first_sys + blockIdx.x;
     if (B.active && !B.active[sys]) return;
     __shared__ SccLds L;
     extern __shared__ double dyn[];

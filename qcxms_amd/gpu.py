@@ -42,6 +42,8 @@ def library():
         L.qcxms_batch_set_scc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32,
                                           C.c_double, C.c_double]
         L.qcxms_batch_set_warm_start.argtypes = [C.c_void_p, C.c_int32]
+        L.qcxms_set_tuning.argtypes = [C.c_int32, _ip, _ip, _ip, C.c_int32, C.c_int32]
+        L.qcxms_get_tuning.argtypes = [C.c_int32, _ip, _ip, _ip, _ip, _ip, _ip]
         L.qcxms_batch_set_stored_derivatives.argtypes = [C.c_void_p, C.c_int32]
         L.qcxms_batch_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         L.qcxms_batch_set_profiling.argtypes = [C.c_void_p, C.c_int32]
@@ -83,6 +85,30 @@ def get_xtb_egrad(num, xyz, charge, multiplicity, method=GFN2_XTB, etemp=300.0):
                           int(method), float(etemp), _p(qat), C.byref(e), _p(grad),
                           C.byref(stat))
     return qat, e.value, grad, stat.value
+
+
+def set_tuning(table, mixed=(-1, 0)):
+    """SCC launch plan for batches created afterwards: table = [(max_nao,
+    eig, lds), ...] by increasing max_nao ([] for one launch per batch with
+    its own options); mixed = (eig, lds) for batches spanning several bins,
+    eig = -1 for one launch per run of one bin."""
+    n = len(table)
+    a = np.array([t[0] for t in table] or [0], dtype=np.int32)
+    e = np.array([t[1] for t in table] or [0], dtype=np.int32)
+    l = np.array([t[2] for t in table] or [0], dtype=np.int32)
+    if library().qcxms_set_tuning(n, _p(a, _ip), _p(e, _ip), _p(l, _ip), int(mixed[0]), int(mixed[1])):
+        raise RuntimeError("qcxms_set_tuning failed: " + last_error())
+
+
+def get_tuning():
+    """(table, mixed) as set_tuning takes them."""
+    cap = 32
+    n, me, ml = C.c_int32(), C.c_int32(), C.c_int32()
+    a, e, l = (np.zeros(cap, dtype=np.int32) for _ in range(3))
+    if library().qcxms_get_tuning(cap, C.byref(n), _p(a, _ip), _p(e, _ip), _p(l, _ip), C.byref(me),
+                                  C.byref(ml)):
+        raise RuntimeError("qcxms_get_tuning failed: " + last_error())
+    return [(int(a[k]), int(e[k]), int(l[k])) for k in range(n.value)], (me.value, ml.value)
 
 
 def _mo_spectrum(call, nat):

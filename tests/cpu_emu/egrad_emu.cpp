@@ -2,6 +2,8 @@
 // from a binary file, results written to another.  With QX_EMU_STORED=1 in
 // the environment the systems go through a persistent batch whose gradient
 // contracts stored derivative blocks (qcxms_batch_set_stored_derivatives).
+// QX_EMU_TUNING="max_nao,eig,lds;..." sets the SCC launch plan first
+// (qcxms_set_tuning, one launch per run of systems of one bin).
 //   input : int32 nsys, method; int32 offsets[nsys+1]; int32 num[nat];
 //           double xyz[3 nat]; int32 charge[nsys], mult[nsys]; double etemp[nsys]
 //   output: int32 rc; double energy[nsys]; double grad[3 nat]; double qat[nat];
@@ -42,6 +44,21 @@ int main(int argc, char** argv)
     std::vector<int32_t> st(nsys);
     int32_t rc;
     const char* stored = std::getenv("QX_EMU_STORED");
+    const char* tuning = std::getenv("QX_EMU_TUNING");
+    if (tuning && *tuning) {
+        std::vector<int32_t> a, e, l;
+        int x, y, z, n = 0;
+        for (const char* p = tuning; std::sscanf(p, "%d,%d,%d%n", &x, &y, &z, &n) == 3;) {
+            a.push_back(x);
+            e.push_back(y);
+            l.push_back(z);
+            p += n;
+            if (*p != ';') break;
+            ++p;
+        }
+        rc = qcxms_set_tuning((int32_t)a.size(), a.data(), e.data(), l.data(), -1, 0);
+        if (rc != 0) std::fprintf(stderr, "qcxms_set_tuning: %s\n", qcxms_gpu_last_error());
+    }
     if (stored && !std::strcmp(stored, "1")) {
         qcxms_batch* b = qcxms_batch_create(nsys, off.data(), num.data(), chg.data(), method);
         rc = b ? qcxms_batch_set_stored_derivatives(b, 1) : -2;

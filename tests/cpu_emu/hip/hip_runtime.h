@@ -373,6 +373,11 @@ inline hipError_t hipEventCreate(hipEvent_t* e)
     *e = new qx_emu_event;
     return hipSuccess;
 }
+#define hipEventDisableTiming 0x2
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+// every emulated launch completes in program order, so a cross-stream wait
+// has nothing to wait for
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t)
 {
     qx_emu::run_or_record([=] { e->t = std::chrono::steady_clock::now(); });

@@ -16,10 +16,13 @@ def build():
     subprocess.run(["make", "-s", "-C", EMU_DIR], check=True)
 
 
-def egrad_batch(systems, xyz_list, etemp, method=2, timeout=1800, stored=False):
+def egrad_batch(systems, xyz_list, etemp, method=2, timeout=1800, stored=False, tuning=None):
     """systems: [(num, charge[, multiplicity])]; returns (rc, energy, [grad],
     [qat], stat, stderr).  stored: the gradient from stored derivative
-    blocks (a persistent batch, SCC thresholds 1e-11 / 1e-9)."""
+    blocks (a persistent batch, SCC thresholds 1e-11 / 1e-9).  tuning:
+    [(max_nao, eig, lds), ...], the SCC launch plan; by default one launch
+    with the Jacobi eigensolver and LDS matrices (emulating the MFMA
+    refinement takes minutes; the MD emulation covers it)."""
     nsys = len(systems)
     off = np.cumsum([0] + [len(s[0]) for s in systems]).astype(np.int32)
     num = np.concatenate([np.asarray(s[0], dtype=np.int32) for s in systems])
@@ -38,7 +41,8 @@ def egrad_batch(systems, xyz_list, etemp, method=2, timeout=1800, stored=False):
             mult.tofile(f)
             et.tofile(f)
         env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1",
-                   QX_EMU_STORED="1" if stored else "0")
+                   QX_EMU_STORED="1" if stored else "0",
+                   QX_EMU_TUNING=";".join(f"{a},{e},{l}" for a, e, l in (tuning or [(256, 0, 1)])))
         p = subprocess.run([EMU_BIN, fin, fout], capture_output=True, text=True, timeout=timeout, env=env)
         if p.returncode != 0:
             raise RuntimeError(f"emulated run failed ({p.returncode}):\n{p.stderr[-6000:]}")

@@ -53,3 +53,19 @@ def test_fortran_drop_in_module_linked():
     assert "get_xtb_egrad" in out and "qcxms_get_xtb_egrad" in out
     mod = os.path.join(ROOT, "qcxms_amd", "fortran", "_build", "qcxms_tblite.mod")
     assert os.path.exists(mod)
+
+
+def test_tuning_table_is_validated_before_any_device_call():
+    """qcxms_set_tuning rejects malformed tables on the host (no GPU needed)."""
+    lib = C.CDLL(LIB)
+    lib.qcxms_set_tuning.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32), C.c_int32, C.c_int32]
+    lib.qcxms_gpu_last_error.restype = C.c_char_p
+    arr = lambda *v: (C.c_int32 * len(v))(*v)
+    assert lib.qcxms_set_tuning(2, arr(40, 24), arr(0, 0), arr(1, 1), -1, 0) == -2
+    assert b"increasing" in lib.qcxms_gpu_last_error()
+    assert lib.qcxms_set_tuning(1, arr(40), arr(2), arr(1), -1, 0) == -2
+    assert b"0 or 1" in lib.qcxms_gpu_last_error()
+    assert lib.qcxms_set_tuning(33, arr(*range(1, 34)), arr(*[0] * 33), arr(*[0] * 33), -1, 0) == -2
+    assert lib.qcxms_set_tuning(1, arr(40), arr(1), arr(1), 2, 0) == -2
+    assert b"mixed" in lib.qcxms_gpu_last_error()

@@ -3,8 +3,9 @@ thread per GPU thread, built with the host AddressSanitizer and
 UndefinedBehaviorSanitizer, static and dynamic LDS filled with NaN bytes at
 every block start) through qcxms_egrad_batch, against the oracle.
 
-Covers, without a GPU: every global and LDS access of k_setup, k_scc and
-k_grad stays inside its buffer, no value is read before it is written, and
+Covers, without a GPU: every global and LDS access of k_setup, k_scc,
+k_dblocks and k_grad stays inside its buffer (also for SCC launches over a
+range of the batch's systems), no value is read before it is written, and
 the kernel arithmetic (including the FP64 MFMA tile layout and the wavefront
 reductions) gives the oracle's energy, gradient and charges.  Systems: a
 3-atom neutral (LDS-resident matrices), a 5-atom doublet cation and a
@@ -41,23 +42,32 @@ def test_emulated_kernels_bounds_and_parity():
 
 
 @pytest.mark.timeout(900)
-def test_emulated_stored_derivative_blocks():
-    """k_dblocks and the gradient from stored blocks, emulated with the
-    sanitizers: every access in bounds, and the oracle's results."""
+def test_emulated_binned_plan_with_stored_derivative_blocks():
+    """An SCC launch plan that puts each of three systems (12, 8 and 6 basis
+    functions, largest first) into its own launch, with LDS sized for that
+    launch and the middle one's matrices in global memory (workgroup 0 of a
+    launch is not system 0), and the gradient from stored derivative blocks
+    (k_dblocks): every access in bounds, and the oracle's results for every
+    system.  (The eigensolver choice per bin is covered on the GPU; the MD
+    emulation below covers the MFMA refinement.)"""
     eb.build()
     systems, coords = [], []
-    for s in (5026, 5004):
+    for s in (5045, 5004, 5026):
         num, xyz, charge = W.random_chain(s)
         systems.append((num, charge, W.getspin(num, charge)))
         coords.append(xyz)
-    rc, e, g, q, st, err = eb.egrad_batch(systems, coords, 5000.0, stored=True)
+    naos = [int(sum(1 if z == 1 else 4 for z in s[0])) for s in systems]
+    assert naos == sorted(naos, reverse=True) and len(set(naos)) == 3
+    table = [(naos[2], 0, 1), (naos[1], 0, 0), (256, 0, 1)]
+    rc, e, g, q, st, err = eb.egrad_batch(systems, coords, 5000.0, stored=True, tuning=table)
     assert rc == 0 and np.all(st == 0), err
     re, rg, rq, rst = ob.gfn2_egrad_batch([s[:2] for s in systems], coords, 5000.0,
                                           opts=ob.default_opts(econv=1e-12, pconv=1e-10))
     de = np.abs(e - re).max()
     dg = max(np.abs(g[i] - rg[i]).max() for i in range(len(systems)))
-    print(f"emulated stored-block gradient: dE={de:.2e} dg={dg:.2e}")
-    assert de <= 1e-8 and dg <= 1e-6
+    dq = max(np.abs(q[i] - rq[i]).max() for i in range(len(systems)))
+    print(f"emulated binned plan + stored blocks (naos {naos}): dE={de:.2e} dg={dg:.2e} dq={dq:.2e}")
+    assert de <= 1e-8 and dg <= 1e-6 and dq <= 1e-6
 
 
 @pytest.mark.timeout(900)
