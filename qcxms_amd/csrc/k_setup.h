@@ -92,11 +92,14 @@ struct IntegralOut {
 
 // S, H0 (off-atom), D and Q blocks of one bra AO (component A) of a shell
 // pair.  S and H0 are symmetric; the moment integrals are stored transposed:
-// element (mu, nu), taken about the atom of mu, sits at nu * ld + mu.
+// element (mu, nu), taken about the atom of mu, sits at nu * ld + mu.  A row
+// writes element (A, b) at both places; for a shell paired with itself
+// (same), row b writes element (b, A) itself, so row A writes only the
+// transposed places and every address has one writer.
 template <int LA, int LB, int A>
 QX_DEV void setup_shell_row(const ShellDesc& sa, const ShellDesc& sb, const ElemParams& ea,
                             const ElemParams& eb, const double* ra, const double* rb,
-                            const double t[3], double hs, bool off, const IntegralOut& io)
+                            const double t[3], double hs, bool off, bool same, const IntegralOut& io)
 {
     double s[3], dpb[3][3], qpb[3][6];
     shell_row_values<LA, LB, A>(ea, sa.k, eb, sb.k, ra, rb, s, dpb, qpb);
@@ -109,21 +112,19 @@ QX_DEV void setup_shell_row(const ShellDesc& sa, const ShellDesc& sb, const Elem
         traceless6(qpb[b], qtb);
         const int mu = sa.ao0 + A, nu = sb.ao0 + b;
         const size_t o1 = (size_t)mu * io.ld + nu, o2 = (size_t)nu * io.ld + mu;
-        io.S[o1] = s[b];
         io.S[o2] = s[b];
-        if (off) {
-            io.H[o1] = hs * s[b];
-            io.H[o2] = hs * s[b];
-        }
+        if (off) io.H[o2] = hs * s[b];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            io.D[c * io.ms + o2] = dpa[c];
-            io.D[c * io.ms + o1] = dpb[b][c];
-        }
+        for (int c = 0; c < 3; ++c) io.D[c * io.ms + o2] = dpa[c];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            io.Q[k * io.ms + o2] = qta[k];
-            io.Q[k * io.ms + o1] = qtb[k];
+        for (int k = 0; k < 6; ++k) io.Q[k * io.ms + o2] = qta[k];
+        if (!same) {
+            io.S[o1] = s[b];
+            if (off) io.H[o1] = hs * s[b];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) io.D[c * io.ms + o1] = dpb[b][c];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) io.Q[k * io.ms + o1] = qtb[k];
         }
     }
 }
@@ -275,15 +276,16 @@ QX_DEV void setup_system(const BatchPtrs& B, int sys, int with_chol, SetupLds& L
                  h0_scale(ea, sa.k, eb, sb.k, r, pir, dpir, kxy);
         }
         const IntegralOut io{S, H, D, Q, ld, ms};
+        const bool same = ia == ib;
         switch ((sa.l * 2 + sb.l) * 3 + row) {
-        case 9: setup_shell_row<1, 1, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
-        case 10: setup_shell_row<1, 1, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
-        case 11: setup_shell_row<1, 1, 2>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
-        case 6: setup_shell_row<1, 0, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
-        case 7: setup_shell_row<1, 0, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
-        case 8: setup_shell_row<1, 0, 2>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
-        case 3: setup_shell_row<0, 1, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
-        default: setup_shell_row<0, 0, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, io); break;
+        case 9: setup_shell_row<1, 1, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, same, io); break;
+        case 10: setup_shell_row<1, 1, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, same, io); break;
+        case 11: setup_shell_row<1, 1, 2>(sa, sb, ea, eb, ra, rb, t, hs, off, same, io); break;
+        case 6: setup_shell_row<1, 0, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, same, io); break;
+        case 7: setup_shell_row<1, 0, 1>(sa, sb, ea, eb, ra, rb, t, hs, off, same, io); break;
+        case 8: setup_shell_row<1, 0, 2>(sa, sb, ea, eb, ra, rb, t, hs, off, same, io); break;
+        case 3: setup_shell_row<0, 1, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, same, io); break;
+        default: setup_shell_row<0, 0, 0>(sa, sb, ea, eb, ra, rb, t, hs, off, same, io); break;
         }
     }
     __syncthreads();
