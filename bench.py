@@ -289,6 +289,36 @@ def run_c20_md(ntraj, steps, warmup, repack=100):
                                                    minlength=9).tolist() if gens else []}}
 
 
+def run_cid(ntraj, steps, warmup):
+    """Batched CID collision MD (cid.f90:724-1111): ntraj x C8H18+ (26 atoms)
+    hit by N2 at E_LAB = 40 eV, set up as first collisions with seeded random
+    offsets; the ion and the gas (28 atoms, 58 basis functions) in every
+    GFN2 evaluation."""
+    import qcxms_amd
+    from qcxms_amd import workloads as W
+    trajs = W.md_ensemble(ntraj, base_seed=1, temp=500.0, tadd_fs=100.0)
+    rng = np.random.default_rng(4242)
+    systems = [qcxms_amd.cid_setup(t["num"], t["xyz"], t["velo"], t["mass"], 0.5 * W.FSTOAU, gas="N2",
+                                   elab_ev=40.0, rand=tuple(rng.random(4))) for t in trajs]
+    e = qcxms_amd.CidEnsemble(systems, charge=1)
+    e.run(warmup)
+    n0 = sum(r.nstep for r in e.read()["results"])
+    t0 = time.perf_counter()
+    running = e.run(steps)
+    wall = time.perf_counter() - t0
+    res = e.read()["results"]
+    e.close()
+    n1 = sum(r.nstep for r in res)
+    return {"metric": "CID trajectory-MD-steps/sec on 1 GPU (batched)", "value": (n1 - n0) / wall,
+            "unit": "trajectory-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": 1e3 * wall / max(steps, 1), "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"batched CID collision MD: {ntraj} x C8H18+ + N2 (28 atoms, 58 basis "
+                                   "functions), E_LAB 40 eV, GFN2-xTB FP64, etemp 5000 K, 0.5 fs",
+                       "ntraj": ntraj, "trajectory_steps": n1 - n0, "running_after": running,
+                       "collided": sum(r.collided for r in res),
+                       "fragmented": sum(r.fragmented for r in res)}}
+
+
 def run_one_per_launch(steps, warmup, fused=False):
     import qcxms_amd
     trajs = workload(1)
@@ -457,7 +487,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=1,
                     help="split the ensemble into this many groups on separate streams")
     ap.add_argument("--config", default="md256",
-                    choices=["md256", "uneven1024", "stretched64", "c20md"],
+                    choices=["md256", "uneven1024", "stretched64", "c20md", "cid256"],
                     help="md256: the headline batched MD (default); the others print the "
                          "throughput of the other workload configurations")
     args = ap.parse_args()
@@ -467,7 +497,9 @@ def main():
     if args.config != "md256":
         if args.gpus != 1:
             sys.exit("bench.py measures one GPU only")
-        if args.config == "c20md":
+        if args.config == "cid256":
+            out = run_cid(args.ntraj, args.steps, args.warmup)
+        elif args.config == "c20md":
             out = run_c20_md(args.ntraj if args.ntraj != 256 else 4096, args.steps, args.warmup)
         else:
             out = run_singlepoints(args.config, max(1, args.steps))

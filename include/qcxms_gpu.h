@@ -367,6 +367,55 @@ int qcxms_manage_fragments_batch(int32_t ntraj, const int32_t *offsets, const in
                                  const int32_t *itrj, int32_t isec, int32_t maxsec, double btf,
                                  int32_t *list, qcxms_fragment_report *report);
 
+/* ---- CID: the collision MD of src/cid.f90:724-1111 (one collision, icoll = 1)
+ * for many trajectories.  System s holds the ion's nion[s] atoms followed by
+ * the collision gas (one atom, or two for N2; only gases whose elements have
+ * GFN2 parameters here can run, i.e. N2); the last atom is the one whose
+ * distance to the ion's centre of mass is tracked.  xyz and velo are the
+ * state after cid()'s set-up (cid.f90:276-676: the ion's velocities include
+ * the flight towards the gas); tinit[s] is the ion's initial temperature,
+ * beta[s] = m_gas / (m_gas + m_ion), velo_cm[s] the centre-of-mass speed of
+ * the previous collision (m/s, 0 for the first).  Each step: leap-frog of
+ * ion and gas, GFN2 energy and gradient of the whole system with checkqc, the
+ * ion's speed and temperature, its connectivity, the averaged fragment
+ * structure, the distance test every 10 steps (five receding tests are the
+ * collision: 800 more steps of 1 fs), the fragmentation extension, the stop
+ * test. */
+typedef struct qcxms_cid qcxms_cid;
+
+typedef struct {
+    double tstep;   /* time step (au) */
+    double etemp;   /* electronic temperature (K), constant in CID */
+    double econv;   /* SCC energy criterion (0: library default) */
+    double pconv;   /* SCC moment criterion (0: library default) */
+    int32_t ntot;   /* maximum number of steps (cid.f90:217: 15000) */
+    int32_t reserved;
+} qcxms_cid_params;
+
+typedef struct {
+    int32_t nstep, nfrag, done, gradfail, collided, fragmented, total_steps, collision_step,
+        check_fragmented, count_average;
+    double ttime, epot, ekin, new_velo, avgt, atlast, ecom, velo_diff, lowest_com;
+    double fragT[10], eint[10];  /* fragment temperatures and kinetic energies at the end */
+} qcxms_cid_result;
+
+qcxms_cid *qcxms_cid_create(int32_t ntraj, const int32_t *offsets, const int32_t *nion,
+                            const int32_t *num, const int32_t *charge, const double *mass,
+                            const int32_t *imass, const double *xyz, const double *velo,
+                            const double *tinit, const double *beta, const double *velo_cm,
+                            const qcxms_cid_params *params);
+void qcxms_cid_destroy(qcxms_cid *cd);
+/* advance every running trajectory by up to nsteps steps; *running = those
+ * still going */
+int qcxms_cid_run(qcxms_cid *cd, int32_t nsteps, int32_t *running);
+/* state of all atoms (ion and gas), the ion's fragment labels, the averaged
+ * structure cid returns (cid.f90:1097-1102; gas rows 0) and one record per
+ * trajectory; any output may be NULL */
+int qcxms_cid_read(qcxms_cid *cd, double *xyz, double *velo, double *grad, double *qat,
+                   int32_t *list, double *axyz, qcxms_cid_result *results);
+/* the ensemble's underlying batch, for the timing hooks */
+qcxms_batch *qcxms_cid_batch(qcxms_cid *cd);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,29 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass,
 #define ORC_MD_TRACE 8
 void orc_md_set_trace(double *buf, int cap);
 
+/* ---- CID: the collision MD of cid.f90:724-1111 (one collision, icoll = 1) ----
+ * The system holds the ion's nuc atoms, then the gas atoms (nuc0 in all; the
+ * last atom is the tracked one).  collided and count_average are inputs as
+ * well: the reference keeps them (SAVE) from one collision to the next. */
+typedef struct {
+    /* inputs */
+    int nuc, nuc0, chrg, ntot, collided, count_average;
+    double tstep, etemp, tinit, beta, velo_cm, ttime;
+    /* outputs */
+    int nstep, nfrag, gradfail, fragmented, total_steps, step_counter, check_fragmented,
+        collision_step, done;
+    double epot, ekin, lowest_com, new_velo, avgt, atlast, ecom, velo_diff;
+    double fragT[10], eint[10];
+} orc_cid_state;
+
+int orc_cid_add_steps(int nuc);                                          /* cid.f90:232-234 */
+/* xyz0/velo0/grad0 [3*nuc0] (grad0 and achrg0 [nuc0] are set by the initial
+ * egrad), list [nuc], axyz [3*nuc] (the averaged structure cid returns; may be
+ * NULL); steps_limit > 0 bounds the steps of this call. */
+int orc_cid_run(orc_cid_state *st, const int32_t *iat0, const double *mass0, double *xyz0,
+                double *velo0, double *grad0, double *achrg0, int32_t *list, double *axyz,
+                orc_egrad_fn egrad, void *ctx, int steps_limit);
+
 /* ---- GFN2-xTB single point with analytic gradient ---- */
 typedef struct {
     double econv, pconv;     /* SCC convergence: |dE| and RMS charge/moment change */

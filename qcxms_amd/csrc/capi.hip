@@ -38,6 +38,7 @@ __global__ __launch_bounds__(256) void k_fill_zero(double* p, size_t n)
 #include "k_grad.h"
 #include "k_md.h"
 #include "k_spec.h"
+#include "k_cid.h"
 #include "tuning_table.h"
 
 namespace {
@@ -2353,5 +2354,242 @@ int qcxms_ei_repacks(qcxms_ei* ei, int32_t* count)
     *count = ei->repacks;
     return QCXMS_OK;
 }
+
+}  // extern "C"
+
+// ---- CID: the collision MD of src/cid.f90:724-1111 for many trajectories ----
+
+struct qcxms_cid {
+    qcxms_batch* b = nullptr;
+    qx::CidParams p{};
+    qx::CidPtrs c{};
+    int ntraj = 0;
+    std::vector<void*> allocs;
+    ~qcxms_cid()
+    {
+        if (b && b->stream) (void)hipStreamSynchronize(b->stream);
+        for (void* q : allocs) (void)hipFree(q);
+        delete b;
+    }
+};
+
+namespace {
+
+template <class T>
+int cidalloc(qcxms_cid* cd, T** p, size_t n, const T* host = nullptr)
+{
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
+    cd->allocs.push_back((void*)*p);
+    if (host) HIPCHK(hipMemcpyAsync(*p, host, n * sizeof(T), hipMemcpyHostToDevice, cd->b->stream));
+    else HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), cd->b->stream));
+    return QCXMS_OK;
+}
+
+int cid_running(qcxms_cid* cd, int32_t* running)
+{
+    std::vector<int> act(cd->ntraj);
+    HIPCHK(hipMemcpyAsync(act.data(), cd->c.active, sizeof(int) * cd->ntraj, hipMemcpyDeviceToHost, cd->b->stream));
+    HIPCHK(hipStreamSynchronize(cd->b->stream));
+    int n = 0;
+    for (int v : act) n += v != 0;
+    *running = n;
+    return QCXMS_OK;
+}
+
+int cid_enqueue_step(qcxms_cid* cd)
+{
+    qcxms_batch* b = cd->b;
+    hipLaunchKernelGGL(qx::k_cid_pre, dim3(cd->ntraj), dim3(64), 0, b->stream, b->d, cd->c, cd->p);
+    HIPCHK(hipGetLastError());
+    int st = enqueue_egrad(b, true);
+    if (st) return st;
+    hipLaunchKernelGGL(qx::k_cid_post, dim3(cd->ntraj), dim3(64), 0, b->stream, b->d, cd->c, cd->p);
+    HIPCHK(hipGetLastError());
+    return QCXMS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+qcxms_cid* qcxms_cid_create(int32_t ntraj, const int32_t* offsets, const int32_t* nion, const int32_t* num,
+                            const int32_t* charge, const double* mass, const int32_t* imass, const double* xyz,
+                            const double* velo, const double* tinit, const double* beta, const double* velo_cm,
+                            const qcxms_cid_params* params)
+{
+    g_err.clear();
+    if (!nion || !num || !mass || !xyz || !velo || !tinit || !beta || !params) {
+        g_err = "null argument";
+        return nullptr;
+    }
+    if (check_offsets(ntraj, offsets, QX_MAX_NAT) != QCXMS_OK) return nullptr;
+    for (int s = 0; s < ntraj; ++s) {
+        const int nat = offsets[s + 1] - offsets[s];
+        if (nion[s] < 1 || nion[s] >= nat) {
+            g_err = "each system needs at least one ion atom followed by the gas atoms";
+            return nullptr;
+        }
+    }
+    if (!(params->tstep > 0.0) || !(params->etemp > 0.0) || params->ntot < 1) {
+        g_err = "tstep, etemp and ntot must be positive";
+        return nullptr;
+    }
+    qcxms_batch* b = qcxms_batch_create(ntraj, offsets, num, charge, QCXMS_GFN2_XTB);
+    if (!b) return nullptr;
+    qcxms_cid* cd = new qcxms_cid();
+    cd->b = b;
+    cd->ntraj = ntraj;
+    b->tune.clear();  // one SCC launch per step, as the MD ensembles
+    b->opts.eig_mfma = 1;
+    b->opts.damp = 0.3;
+    b->opts.extrapolate = QX_DEFAULT_EXTRAPOLATE;
+    if (params->econv > 0.0) b->opts.econv = params->econv;
+    if (params->pconv > 0.0) b->opts.pconv = params->pconv;
+    b->warm_start = 1;
+    cd->p.tstep = params->tstep;
+    cd->p.dumpdist = 10;   // cid.f90:215
+    cd->p.dumpavg = 50;    // cid.f90:216
+    cd->p.cnt_steps = 50;  // cid.f90:231
+    const int na = b->natoms;
+    std::vector<qx::CidTraj> tr(ntraj);
+    for (int s = 0; s < ntraj; ++s) {
+        qx::CidTraj& t = tr[s];
+        std::memset(&t, 0, sizeof(t));
+        t.nuc = nion[s];
+        t.nuc0 = offsets[s + 1] - offsets[s];
+        t.ntot = params->ntot;
+        // cid.f90:232-234
+        t.add_steps = 0;
+        if (t.nuc > 10) t.add_steps = (t.nuc / 10) * 500;
+        if (t.nuc >= 40) t.add_steps = (t.nuc / 10) * 1000;
+        t.tinit = tinit[s];
+        t.beta = beta[s];
+        t.velo_cm = velo_cm ? velo_cm[s] : 0.0;
+        double sm = 0.0;
+        for (int i = 0; i < t.nuc; ++i) sm = sm + mass[offsets[s] + i];
+        t.summass = sm;
+        t.total_steps = params->ntot;
+        t.check_fragmented = 1;
+        t.nfrag = 1;
+        b->info[s].kt = params->etemp * gfn2::kKtoAu;
+    }
+    std::vector<int> act(ntraj, 1), lst(na, 1), im(na, -1);
+    if (imass) im.assign(imass, imass + na);
+    std::vector<int> nm(num, num + na);
+    double* d_mass;
+    int* d_imass;
+    int* d_num;
+    if (cidalloc(cd, &cd->c.traj, ntraj, tr.data()) || cidalloc(cd, &cd->c.velo, 3 * (size_t)na, velo) ||
+        cidalloc(cd, &d_mass, (size_t)na, mass) || cidalloc(cd, &d_imass, (size_t)na, im.data()) ||
+        cidalloc(cd, &d_num, (size_t)na, nm.data()) || cidalloc(cd, &cd->c.list, (size_t)na, lst.data()) ||
+        cidalloc(cd, &cd->c.avxyz, 3 * (size_t)na) || cidalloc(cd, &cd->c.avxyz2, 3 * (size_t)na) ||
+        cidalloc(cd, &cd->c.store, 3 * (size_t)na) || cidalloc(cd, &cd->c.active, (size_t)ntraj, act.data())) {
+        delete cd;
+        return nullptr;
+    }
+    cd->c.mass = d_mass;
+    cd->c.imass = d_imass;
+    cd->c.num = d_num;
+    b->d.active = cd->c.active;
+    if (hipMemcpyAsync(b->d.xyz, xyz, sizeof(double) * 3 * na, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->d_info, b->info.data(), sizeof(SysInfo) * ntraj, hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess) {
+        g_err = "upload failed";
+        delete cd;
+        return nullptr;
+    }
+    // the initial forces of ion + gas (cid.f90:703)
+    if (enqueue_egrad(b, true) != QCXMS_OK) {
+        delete cd;
+        return nullptr;
+    }
+    hipLaunchKernelGGL(qx::k_cid_init, dim3(ntraj), dim3(64), 0, b->stream, b->d, cd->c);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
+        g_err = "initial energy/gradient failed on the device";
+        delete cd;
+        return nullptr;
+    }
+    if (b->profiling) collect_times();
+    return cd;
+}
+
+void qcxms_cid_destroy(qcxms_cid* cd) { delete cd; }
+
+int qcxms_cid_run(qcxms_cid* cd, int32_t nsteps, int32_t* running)
+{
+    if (!cd) return fail("null ensemble");
+    int st, run = cd->ntraj;
+    for (int s = 0; s < nsteps && run > 0; ++s) {
+        if ((st = cid_enqueue_step(cd))) return st;
+        // look for the end of the ensemble every 50 steps
+        if ((s + 1) % 50 == 0 && (st = cid_running(cd, &run))) return st;
+    }
+    if ((st = cid_running(cd, &run))) return st;
+    if (cd->b->profiling) collect_times();
+    if (running) *running = run;
+    return QCXMS_OK;
+}
+
+int qcxms_cid_read(qcxms_cid* cd, double* xyz, double* velo, double* grad, double* qat, int32_t* list,
+                   double* axyz, qcxms_cid_result* results)
+{
+    if (!cd) return fail("null ensemble");
+    qcxms_batch* b = cd->b;
+    const size_t na = b->natoms;
+    if (xyz) HIPCHK(hipMemcpyAsync(xyz, b->d.xyz, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, b->stream));
+    if (velo) HIPCHK(hipMemcpyAsync(velo, cd->c.velo, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, b->stream));
+    if (grad) HIPCHK(hipMemcpyAsync(grad, b->d.grad, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, b->stream));
+    if (qat) HIPCHK(hipMemcpyAsync(qat, b->d.qat, sizeof(double) * na, hipMemcpyDeviceToHost, b->stream));
+    if (list) HIPCHK(hipMemcpyAsync(list, cd->c.list, sizeof(int) * na, hipMemcpyDeviceToHost, b->stream));
+    std::vector<qx::CidTraj> tr(cd->ntraj);
+    std::vector<double> av(3 * na), store(3 * na);
+    HIPCHK(hipMemcpyAsync(tr.data(), cd->c.traj, sizeof(qx::CidTraj) * cd->ntraj, hipMemcpyDeviceToHost, b->stream));
+    if (axyz) {
+        HIPCHK(hipMemcpyAsync(av.data(), cd->c.avxyz, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipMemcpyAsync(store.data(), cd->c.store, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, b->stream));
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int s = 0; s < cd->ntraj; ++s) {
+        const qx::CidTraj& t = tr[s];
+        const size_t off = (size_t)b->info[s].atom_off;
+        if (axyz) {
+            // the structure cid returns (cid.f90:1097-1102)
+            for (int k = 0; k < 3 * t.nuc0; ++k) axyz[3 * off + k] = 0.0;
+            for (int k = 0; k < 3 * t.nuc; ++k)
+                axyz[3 * off + k] = t.check_fragmented > 1 ? store[3 * off + k]
+                                                           : av[3 * off + k] / (t.xyzavg_dump ? t.xyzavg_dump : 1);
+        }
+        if (results) {
+            qcxms_cid_result& r = results[s];
+            r.nstep = t.nstep;
+            r.nfrag = t.nfrag;
+            r.done = t.done;
+            r.gradfail = t.gradfail;
+            r.collided = t.collided;
+            r.fragmented = t.fragmented;
+            r.total_steps = t.total_steps;
+            r.collision_step = t.collision_step;
+            r.check_fragmented = t.check_fragmented;
+            r.count_average = t.count_average;
+            r.ttime = t.ttime;
+            r.epot = t.epot;
+            r.ekin = t.ekin;
+            r.new_velo = t.new_velo;
+            r.avgt = t.avgt;
+            r.atlast = t.atlast;
+            r.ecom = t.ecom;
+            r.velo_diff = t.velo_diff;
+            r.lowest_com = t.lowest;
+            for (int f = 0; f < 10; ++f) {
+                r.fragT[f] = t.fragT[f];
+                r.eint[f] = t.eint[f];
+            }
+        }
+    }
+    return QCXMS_OK;
+}
+
+qcxms_batch* qcxms_cid_batch(qcxms_cid* cd) { return cd ? cd->b : nullptr; }
 
 }  // extern "C"

@@ -82,22 +82,43 @@ constexpr int kMaxThreads = 1024;
 constexpr size_t kMaxLds = 160 * 1024;
 constexpr int kTimeoutSeconds = 300;
 
-// Barrier whose participant count can shrink while it is in use.
+// Barrier whose participant count can shrink while it is in use.  A block
+// barrier also checks that every thread of one barrier generation arrives
+// from the same source line: a __syncthreads reached from two places in one
+// round means control flow around a barrier that is not uniform across the
+// block, which the GPU does not report (its barrier counts arrivals from
+// anywhere) and which can desynchronise the waves or hang them.
 struct Barrier {
     std::mutex m;
     std::condition_variable cv;
     int expected = 0, arrived = 0;
     unsigned long gen = 0;
+    const char* site_file = nullptr;
+    int site_line = 0;
+    unsigned site_thread = 0;
     void reset(int n)
     {
         std::lock_guard<std::mutex> l(m);
         expected = n;
         arrived = 0;
     }
-    void wait(const char* what)
+    void wait(const char* what, const char* file = nullptr, int line = 0)
     {
         std::unique_lock<std::mutex> l(m);
         const unsigned long g = gen;
+        if (file) {
+            if (arrived == 0) {
+                site_file = file;
+                site_line = line;
+                site_thread = threadIdx.x;
+            } else if (line != site_line || std::strcmp(file, site_file) != 0) {
+                std::fprintf(stderr,
+                             "cpu emulation: divergent %s barrier (block %u): thread %u at %s:%d, "
+                             "thread %u at %s:%d\n",
+                             what, blockIdx.x, site_thread, site_file, site_line, threadIdx.x, file, line);
+                std::abort();
+            }
+        }
         if (++arrived == expected) {
             arrived = 0;
             ++gen;
@@ -266,7 +287,7 @@ inline void run_or_record(std::function<void()> f)
 
 }  // namespace qx_emu
 
-#define __syncthreads() qx_emu::g_block.wait("block")
+#define __syncthreads() qx_emu::g_block.wait("block", __FILE__, __LINE__)
 #define __shfl_xor(v, m, w) qx_emu::shfl_xor((v), (m), (w))
 #define __builtin_amdgcn_mfma_f64_16x16x4f64 qx_emu::mfma_f64_16x16x4
 #define __ffsll(x) __builtin_ffsll(x)

@@ -7,6 +7,7 @@
 //   output: int32 rc, running; double xyz[3 nat], velo[3 nat], qat[nat];
 //           per trajectory: int32 nstep, exit_reason; double epot, ekin
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "../../include/qcxms_gpu.h"
@@ -41,6 +42,9 @@ int main(int argc, char** argv)
     p.max_steps = nsteps;
     p.use_graph = hdr[3];
     p.fused = hdr[2];
+    // SCC thresholds of the GPU tests when given (QX_EMU_ECONV, QX_EMU_PCONV)
+    if (const char* e = std::getenv("QX_EMU_ECONV")) p.econv = std::atof(e);
+    if (const char* e = std::getenv("QX_EMU_PCONV")) p.pconv = std::atof(e);
     qcxms_md* md = qcxms_md_create(nt, off.data(), num.data(), chg.data(), mass.data(), imass.data(), xyz.data(),
                                    velo.data(), velof.data(), eimp.data(), tadd.data(), &p);
     int32_t rc = md ? 0 : -1, running = 0;

@@ -30,6 +30,16 @@ class Gfn2Info(C.Structure):
                 + [(n, C.c_int) for n in ("iterations", "converged", "nao", "nsh", "nel_int")])
 
 
+class CidState(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("nuc", "nuc0", "chrg", "ntot", "collided", "count_average")]
+                + [(n, C.c_double) for n in ("tstep", "etemp", "tinit", "beta", "velo_cm", "ttime")]
+                + [(n, C.c_int) for n in ("nstep", "nfrag", "gradfail", "fragmented", "total_steps",
+                                          "step_counter", "check_fragmented", "collision_step", "done")]
+                + [(n, C.c_double) for n in ("epot", "ekin", "lowest_com", "new_velo", "avgt",
+                                             "atlast", "ecom", "velo_diff")]
+                + [("fragT", C.c_double * 10), ("eint", C.c_double * 10)])
+
+
 class MdState(C.Structure):
     _fields_ = ([(n, C.c_int) for n in ("nat", "chrg", "nfragexit", "max_steps", "isec")]
                 + [(n, C.c_double) for n in ("tstep", "tadd", "eimp", "ax", "ieetemp", "etempin")]
@@ -68,6 +78,8 @@ def oracle():
         L.orc_md_run_avg.argtypes = [C.c_void_p, ip, ip, dp, dp, dp, dp, dp, dp, ip, EGRAD_FN,
                                      C.c_void_p, C.c_int, dp, dp]
         L.orc_md_set_trace.argtypes = [dp, C.c_int]
+        L.orc_cid_run.argtypes = [C.c_void_p, ip, dp, dp, dp, dp, dp, ip, dp, EGRAD_FN, C.c_void_p,
+                                  C.c_int]
         _oracle = L
     return _oracle
 
@@ -408,3 +420,30 @@ def ei_gen_dict(g):
     d["asave"] = g.asave.decode()
     d["res"] = [g.res[k].value.decode() for k in range(g.nres)]
     return d
+
+
+def cid_run(system, charge=1, tstep=0.5 * 41.3413733365614, etemp=5000.0, ntot=15000, opts=None,
+            steps_limit=0, collided=0, count_average=0):
+    """The oracle's CID collision MD (cid_oracle.c) of one system from
+    qcxms_amd.cid.cid_setup.  Returns dict(state, xyz, velo, grad, qat, list,
+    axyz)."""
+    L = oracle()
+    num = np.ascontiguousarray(system["num"], dtype=np.int32)
+    nuc0, nuc = len(num), int(system["nion"])
+    xyz = np.ascontiguousarray(system["xyz"], dtype=np.float64).copy()
+    velo = np.ascontiguousarray(system["velo"], dtype=np.float64).copy()
+    mass = np.ascontiguousarray(system["mass"], dtype=np.float64)
+    grad = np.zeros((nuc0, 3))
+    q = np.zeros(nuc0)
+    lst = np.zeros(nuc, dtype=np.int32)
+    axyz = np.zeros((nuc, 3))
+    st = CidState()
+    st.nuc, st.nuc0, st.chrg, st.ntot = nuc, nuc0, int(charge), int(ntot)
+    st.collided, st.count_average = int(collided), int(count_average)
+    st.tstep, st.etemp, st.tinit, st.beta = tstep, etemp, float(system["tinit"]), float(system["beta"])
+    st.velo_cm, st.ttime = 0.0, 0.0
+    o = opts if opts is not None else default_opts()
+    fn = C.cast(L.orc_gfn2_md_egrad, EGRAD_FN)
+    L.orc_cid_run(C.byref(st), ptr(num, ip), ptr(mass), ptr(xyz), ptr(velo), ptr(grad), ptr(q),
+                  ptr(lst, ip), ptr(axyz), fn, C.cast(C.pointer(o), C.c_void_p), int(steps_limit))
+    return dict(state=st, xyz=xyz, velo=velo, grad=grad, qat=q, list=lst, axyz=axyz)
