@@ -319,6 +319,42 @@ def run_cid(ntraj, steps, warmup):
                        "fragmented": sum(r.fragmented for r in res)}}
 
 
+# algorithmic HBM bytes of the MD bookkeeping scans per atom and per
+# trajectory record (reads + writes of the arrays each kernel touches once)
+SCAN_BYTES = {
+    # qat, avchrg, xyz, avxyz, grad, velo, mass read; avchrg, avxyz, velo, xyz written
+    "md_pre": (8 + 8 + 24 + 24 + 24 + 24 + 8) + (8 + 24 + 24 + 24),
+    # grad, qat (checkqc), velo, velof, mass (heating), num, xyz read; velo, list written
+    "md_post": (24 + 8 + 24 + 8 + 8 + 4 + 24) + (24 + 4),
+}
+TRAJ_RECORD_BYTES = 2 * 336  # the trajectory record (MdTraj, 336 bytes), read and written
+
+
+def scan_kernels(ntraj, steps=20, warmup=5):
+    """HBM rate of the integrator/bookkeeping scans (k_md_pre: leap-frog and
+    averages; k_md_post: checkqc, heating, connectivity), timed with HIP
+    events in a short run of the per-step kernels of the same workload."""
+    import qcxms_amd
+    trajs = workload(ntraj)
+    natoms = sum(len(t["num"]) for t in trajs)
+    md = qcxms_amd.MdEnsemble(trajs, max_steps=5000, use_graph=False, fused=False)
+    md.run(warmup)
+    md.set_profiling(True)
+    for k in SCAN_BYTES:
+        md.kernel_time(k)
+    md.run(steps)
+    out = {}
+    for k, per_atom in SCAN_BYTES.items():
+        ms, n = md.kernel_time(k)
+        us = 1e3 * ms / max(n, 1)
+        b = natoms * per_atom + ntraj * TRAJ_RECORD_BYTES
+        out["k_" + k] = {"avg_us": us, "bytes_per_launch": b, "GB/s": b / (us * 1e-6) / 1e9 if us > 0 else None,
+                         "frac_of_hbm_peak": (b / (us * 1e-6) / 1e9) / HBM_PEAK_GBS if us > 0 else None}
+    md.set_profiling(False)
+    md.close()
+    return out
+
+
 def run_one_per_launch(steps, warmup, fused=False):
     import qcxms_amd
     trajs = workload(1)
@@ -549,6 +585,9 @@ def main():
                               "unit": "trajectory-steps/s",
                               "form": "per-step kernels, each MD step replayed as one hipGraph"},
     }
+    # the integrator and connectivity scans are latency-bound: their HBM rate
+    # beside the roofline of the dominant kernel
+    line["bookkeeping_scans"] = scan_kernels(args.ntraj)
     if args.steady_steps > 0:
         # the same workload past heating (tadd = 100 fs = 200 steps), so that a
         # short timed window inside the heating phase is shown beside the rate
