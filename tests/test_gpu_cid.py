@@ -90,3 +90,25 @@ def test_gradient_failure_ends_the_run():
     st = ob.cid_run(sysd, charge=1, opts=ORC)["state"]
     assert running == 0
     assert (r.gradfail, r.nstep, r.done) == (st.gradfail, st.nstep, 1) == (1, 0, 1)
+
+
+def test_octane_with_n2_global_memory_path_follows_the_oracle():
+    """C8H18+ + N2 (28 atoms, 58 basis functions: SCC matrices in global
+    memory, MFMA eigen refinement), the system of the CID benchmark, over 60
+    steps of the approach."""
+    from qcxms_amd import workloads as W
+    t = W.md_ensemble(1, base_seed=1, temp=500.0, tadd_fs=100.0)[0]
+    sysd = q().cid_setup(t["num"], t["xyz"], t["velo"], t["mass"], CC.TSTEP, gas="N2", elab_ev=40.0,
+                         rand=(0.4, 0.6, 0.2, 0.8))
+    assert sum(1 if z == 1 else 4 for z in sysd["num"]) == 58
+    e = q().CidEnsemble([sysd], charge=1, **GPU)
+    e.run(60)
+    out = e.read()
+    e.close()
+    ref = ob.cid_run(sysd, charge=1, opts=ORC, steps_limit=60)
+    r, st = out["results"][0], ref["state"]
+    dx = np.abs(out["xyz"] - ref["xyz"]).max()
+    print(f"octane + N2: dE {abs(r.epot - st.epot):.2e} dx {dx:.2e}", flush=True)
+    assert r.nstep == st.nstep == 60
+    assert abs(r.epot - st.epot) <= 1e-8 and dx <= 1e-8
+    assert np.abs(out["velo"] - ref["velo"]).max() <= 1e-10
