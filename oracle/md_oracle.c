@@ -325,8 +325,19 @@ void orc_md_set_trace(double *buf, int cap)
     md_trace_cap = buf ? cap : 0;
 }
 
-/* md.f90:34-708 restricted to the EI production branch (it>0, method<3,
- * icoll=0, no restart).  egrad follows iniqm.f90:457-679 for prog=8: the
+/* Optional per-step coordinates (per thread): 3 * nat doubles after the
+ * leap-frog of every loop iteration, at most cap records. */
+static _Thread_local double *md_xyz_trace;
+static _Thread_local int md_xyz_trace_cap;
+void orc_md_set_xyz_trace(double *buf, int cap)
+{
+    md_xyz_trace = buf;
+    md_xyz_trace_cap = buf ? cap : 0;
+}
+
+/* md.f90:34-708 restricted to the production branch (it>0, no restart) of
+ * EI (method<3, icoll=0) and of CID after a collision (st->cid: method=3,
+ * icoll>=1).  egrad follows iniqm.f90:457-679 for prog=8: the
  * callback returns E, grad, qat; checkqc is applied here.  steps_limit
  * bounds the number of loop iterations run by this call (<=0: no bound). */
 int orc_md_run(orc_md_state *st, const int32_t *iat, const int32_t *imass, const double *mass,
@@ -346,12 +357,15 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
                    double *avg_chrg)
 {
     const int nuc = st->nat;
-    const int more = 250, avdump = 50;
+    const int more = 250, avdump = 50, cnt_steps = 50;
+    const int cid = st->cid;
     /* later generations exit at two fragments (main.F90:2255) */
     const int nfragexit = st->isec > 1 ? 2 : st->nfragexit;
     double ekin, t, epot;
     double *avchrg = (double *)calloc((size_t)nuc, sizeof(double));
     double *avxyz = (double *)calloc(3 * (size_t)nuc, sizeof(double));
+    double *avxyz2 = (double *)calloc(3 * (size_t)nuc, sizeof(double));
+    double *store = (double *)calloc(3 * (size_t)nuc, sizeof(double));
     st->mdok = 0;
     st->nfrag = 1;
     st->fragstate = 0;
@@ -370,8 +384,28 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
         st->exit_reason = ORC_MD_ERR_SCF;
         free(avchrg);
         free(avxyz);
+        free(avxyz2);
+        free(store);
         return 0;
     }
+    /* the ion's centre-of-mass motion (md.f90:247-255): with CID its kinetic
+     * energy is taken out of Ekin, at the start and after every step */
+    double old_cm[3] = {0, 0, 0}, summass = 0.0, tm = 0.0;
+    for (int i = 0; i < nuc; ++i) {
+        for (int c = 0; c < 3; ++c) old_cm[c] = old_cm[c] + mass[i] * xyz[3 * i + c];
+        tm = tm + mass[i];
+    }
+    for (int c = 0; c < 3; ++c) old_cm[c] = old_cm[c] / tm;
+    for (int i = 0; i < nuc; ++i) summass = summass + mass[i];
+    double new_velo = st->new_velo;
+    const double vv0 = new_velo * ORC_MSTOAU;
+    const double e_kin0 = 0.5 * summass * (vv0 * vv0);
+    const double e_kin_diff0 = ekin - e_kin0;
+    double new_temp = (2.0 * e_kin_diff0) / (3.0 * ORC_KB * nuc);
+    if (cid) ekin = e_kin_diff0;
+    int check_fragmented = 1, count_average = st->count_average, cnt = 0, save_natf[10] = {0};
+    int max_steps = st->max_steps;
+    const int add_steps = orc_cid_add_steps(nuc);
     /* ttime carries over from an earlier generation (main.F90 passes it through md) */
     double tav = 0, epav = 0, ekav = 0, edum = 0, eerror = 0, atlast = 0, dtime = 0, ttime = st->ttime;
     int nstep = 0, morestep = 0, fconst = 0, kdump = avdump, nfrag = 1;
@@ -398,7 +432,7 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
         }
         eerror = eav - epot - ekin;
         if (nfrag == 1) fragT[0] = tav / nstep;
-        int err1 = (epot == 0.0), err2 = (fabs(eerror) > (double)0.1f);
+        int err1 = (epot == 0.0), err2 = (fabs(eerror) > (cid ? (double)0.2f : (double)0.1f));
         if (err1 || err2) {
             st->mdok = (nfrag > 1 && nfrag <= 4) || st->isec > 1;
             st->exit_reason = err1 ? ORC_MD_ERR_SCF : ORC_MD_ERR_DRIFT;
@@ -412,7 +446,7 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
         }
         for (int i = 0; i < nuc; ++i) avchrg[i] = avchrg[i] + achrg[i];
         for (int i = 0; i < 3 * nuc; ++i) avxyz[i] = avxyz[i] + xyz[i];
-        atlast = atlast + t;
+        atlast = atlast + (cid ? new_temp : t);
 
         orc_leapfrog(nuc, grad, mass, tstep, xyz, velo, &ekin);
         ttime = ttime + tstep / ORC_FSTOAU;
@@ -425,21 +459,26 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
         if (nfrag == 1) morestep = 0;
         if (nfrag > 1 && dtime < 1.0e-6) dtime = ttime / (double)1000.0f;
 
-        if (nstep <= nadd && nfrag == 1) {
-            int k = orc_impactscale(nuc, velo, mass, velof, st->eimp, fadd * nstep, ekinstart);
-            if (k >= 20000) {
-                st->exit_reason = ORC_MD_HEAT_FAIL;
-                break;
+        if (!cid) {
+            /* heating and the electronic temperature: EI only (md.f90:438-445) */
+            if (nstep <= nadd && nfrag == 1) {
+                int k = orc_impactscale(nuc, velo, mass, velof, st->eimp, fadd * nstep, ekinstart);
+                if (k >= 20000) {
+                    st->exit_reason = ORC_MD_HEAT_FAIL;
+                    break;
+                }
             }
+            double dum = st->eimp - st->eimp * (double)(float)nstep / (double)(float)nadd;
+            etemp = orc_setetemp(nfrag, dum, st->ax, st->ieetemp);
         }
-        double dum = st->eimp - st->eimp * (double)(float)nstep / (double)(float)nadd;
-        etemp = orc_setetemp(nfrag, dum, st->ax, st->ieetemp);
 
         orc_fragment_structure(nuc, iat, xyz, 3.0, 1, 0, list);
         int32_t nf;
         orc_fragmass(nuc, iat, list, mass, imass, &nf, fragm, fragat);
         nfrag = nf;
         if (nfrag > 1) orc_intenergy(nuc, list, mass, velo, nfrag, fragT);
+        if (md_xyz_trace && nstep <= md_xyz_trace_cap)
+            memcpy(md_xyz_trace + 3 * (size_t)nuc * (size_t)(nstep - 1), xyz, sizeof(double) * 3 * (size_t)nuc);
         if (md_trace && nstep <= md_trace_cap) {
             double *r = md_trace + ORC_MD_TRACE * (size_t)(nstep - 1);
             r[0] = nstep;
@@ -454,6 +493,69 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
         if (nfrag > 6) {
             st->exit_reason = ORC_MD_NFRAG6;
             break;
+        }
+        if (cid) {
+            /* the ion's speed, Ekin without it (md.f90:466-478) */
+            double cm[3] = {0, 0, 0};
+            double tmm = 0.0;
+            for (int i = 0; i < nuc; ++i) {
+                for (int c = 0; c < 3; ++c) cm[c] = cm[c] + mass[i] * xyz[3 * i + c];
+                tmm = tmm + mass[i];
+            }
+            for (int c = 0; c < 3; ++c) cm[c] = cm[c] / tmm;
+            const double d0 = cm[0] - old_cm[0], d1 = cm[1] - old_cm[1], d2 = cm[2] - old_cm[2];
+            const double cm_out = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            new_velo = (cm_out / tstep) / ORC_MSTOAU;
+            for (int c = 0; c < 3; ++c) old_cm[c] = cm[c];
+            const double vv = new_velo * ORC_MSTOAU;
+            const double e_kin = 0.5 * summass * (vv * vv);
+            const double e_kin_diff = ekin - e_kin;
+            new_temp = (2.0 * e_kin_diff) / (3.0 * ORC_KB * nuc);
+            ekin = e_kin_diff;
+            /* the averaged fragment structure and the extension (md.f90:496-627) */
+            if (nfrag > check_fragmented) {
+                count_average = 1;
+                check_fragmented = nfrag;
+                max_steps = nstep + add_steps;
+            }
+            if (nfrag < check_fragmented && count_average) {
+                cnt = 0;
+                memset(avxyz2, 0, sizeof(double) * 3 * (size_t)nuc);
+                memset(store, 0, sizeof(double) * 3 * (size_t)nuc);
+                count_average = 0;
+                check_fragmented = 1;
+            }
+            if (count_average) {
+                cnt = cnt + 1;
+                for (int k = 0; k < 3 * nuc; ++k) {
+                    avxyz2[k] = avxyz2[k] + xyz[k];
+                    store[k] = avxyz2[k] / cnt;
+                }
+                int natf[10] = {0};
+                for (int i = 0; i < nuc; ++i)
+                    if (list[i] >= 1 && list[i] <= 10) natf[list[i] - 1] += 1;
+                for (int f = 0; f < nfrag && f < 10; ++f) {
+                    if (cnt == 1) save_natf[f] = natf[f];
+                    if (natf[f] != save_natf[f]) {
+                        cnt = 0;
+                        memset(store, 0, sizeof(double) * 3 * (size_t)nuc);
+                        memset(avxyz2, 0, sizeof(double) * 3 * (size_t)nuc);
+                        break;
+                    }
+                }
+                if (cnt == cnt_steps) {
+                    memset(avxyz2, 0, sizeof(double) * 3 * (size_t)nuc);
+                    cnt = 0;
+                    count_average = 0;
+                }
+            }
+            if (nstep >= max_steps) {
+                st->fragstate = 1;
+                st->mdok = 1;
+                st->exit_reason = ORC_MD_MAXSTEP;
+                break;
+            }
+            continue;
         }
         if (nfrag > nfragexit) {
             st->fragstate = 1;
@@ -504,8 +606,14 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass, c
     for (int i = 0; i < nuc; ++i)
         if (avg_chrg) avg_chrg[i] = (nstep > 0 && kdump) ? avchrg[i] / kdump : 0.0;
     for (int i = 0; i < 3 * nuc; ++i)
-        if (axyz) axyz[i] = (nstep > 0 && kdump) ? avxyz[i] / kdump : 0.0;
+        if (axyz)
+            axyz[i] = check_fragmented > 1 ? store[i] : ((nstep > 0 && kdump) ? avxyz[i] / kdump : 0.0);
+    st->check_fragmented = check_fragmented;
+    st->count_average = count_average;
+    st->new_velo = new_velo;
     free(avchrg);
     free(avxyz);
+    free(avxyz2);
+    free(store);
     return nstep;
 }

@@ -21,6 +21,7 @@ extern "C" {
 #define ORC_AUTOAA  0.52917726
 #define ORC_AUTOEV  27.21138505
 #define ORC_FSTOAU  41.3413733365614
+#define ORC_MSTOAU  (1.0 / 2.18769126364e+06) /* xtb_mctc_convert.f90:63 */
 
 double orc_aatoau(void);
 double orc_amutoau(void);
@@ -63,6 +64,13 @@ typedef struct {
     int nstep, nfrag, fragstate, mdok, exit_reason;
     double tav, epav, ekav, ttime, dtime, atlast, epot, ekin, eerror, etemp;
     double fragm[10], fragT[10];
+    /* CID mode (md.f90 with method = 3, icoll >= 1: the MD after a
+     * collision): cid != 0 selects it; new_velo (m/s) is the ion's
+     * centre-of-mass speed handed over by cid() and returned at the end;
+     * count_average is kept by the reference between calls (SAVE);
+     * check_fragmented is returned */
+    int cid, count_average, check_fragmented;
+    double new_velo;
 } orc_md_state;
 
 /* exit_reason codes shared with the product's MD engine */
@@ -84,6 +92,8 @@ int orc_md_run_avg(orc_md_state *st, const int32_t *iat, const int32_t *imass,
 /* per-step records of orc_md_run on this thread (buf NULL: off) */
 #define ORC_MD_TRACE 8
 void orc_md_set_trace(double *buf, int cap);
+/* per-step coordinates (3 * nat doubles a step) of orc_md_run on this thread */
+void orc_md_set_xyz_trace(double *buf, int cap);
 
 /* ---- CID: the collision MD of cid.f90:724-1111 (one collision, icoll = 1) ----
  * The system holds the ion's nuc atoms, then the gas atoms (nuc0 in all; the

@@ -46,7 +46,9 @@ class MdState(C.Structure):
                 + [(n, C.c_int) for n in ("nstep", "nfrag", "fragstate", "mdok", "exit_reason")]
                 + [(n, C.c_double) for n in ("tav", "epav", "ekav", "ttime", "dtime", "atlast",
                                              "epot", "ekin", "eerror", "etemp")]
-                + [("fragm", C.c_double * 10), ("fragT", C.c_double * 10)])
+                + [("fragm", C.c_double * 10), ("fragT", C.c_double * 10)]
+                + [(n, C.c_int) for n in ("cid", "count_average", "check_fragmented")]
+                + [("new_velo", C.c_double)])
 
 
 EGRAD_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, ip, dp, C.c_int, C.c_double, dp, dp, dp)
@@ -78,6 +80,7 @@ def oracle():
         L.orc_md_run_avg.argtypes = [C.c_void_p, ip, ip, dp, dp, dp, dp, dp, dp, ip, EGRAD_FN,
                                      C.c_void_p, C.c_int, dp, dp]
         L.orc_md_set_trace.argtypes = [dp, C.c_int]
+        L.orc_md_set_xyz_trace.argtypes = [dp, C.c_int]
         L.orc_cid_run.argtypes = [C.c_void_p, ip, dp, dp, dp, dp, dp, ip, dp, EGRAD_FN, C.c_void_p,
                                   C.c_int]
         _oracle = L
@@ -227,9 +230,11 @@ def impactscale(lib, prefix, velo, mass, velof, eimp, ff, e0):
 
 def md_run(traj, steps_limit, tstep=0.5 * 41.3413733365614, max_steps=2000, nfragexit=3,
            etempin=-1.0, opts=None, charge=1, ieetemp=0.0, ax=0.0, isec=1, egrad=None,
-           trace_steps=0):
-    """Oracle EI production MD (md.f90:285-682) with GFN2 forces for one
-    trajectory dict (num, xyz, velo, mass, eimp, tadd[, velof, imass])."""
+           trace_steps=0, cid=False, new_velo=0.0, count_average=0):
+    """Oracle production MD (md.f90:285-682) with GFN2 forces for one
+    trajectory dict (num, xyz, velo, mass, eimp, tadd[, velof, imass]);
+    cid: the MD after a CID collision (method 3, icoll >= 1) with the ion's
+    centre-of-mass speed new_velo (m/s)."""
     L = oracle()
     num = np.ascontiguousarray(traj["num"], dtype=np.int32)
     nat = len(num)
@@ -246,6 +251,7 @@ def md_run(traj, steps_limit, tstep=0.5 * 41.3413733365614, max_steps=2000, nfra
     st.isec = int(isec)
     st.tstep, st.tadd, st.eimp = tstep, float(traj["tadd"]), float(traj["eimp"])
     st.ax, st.ieetemp, st.etempin = float(ax), float(ieetemp), etempin
+    st.cid, st.count_average, st.new_velo = int(bool(cid)), int(count_average), float(new_velo)
     o = opts if opts is not None else default_opts()
     # egrad: optional Python callback (ctx, nat, iat, xyz, chrg, etemp, e, g, q)
     # wrapped as EGRAD_FN, for tests that force an exit through the energy
@@ -255,15 +261,19 @@ def md_run(traj, steps_limit, tstep=0.5 * 41.3413733365614, max_steps=2000, nfra
     # per-step records: nstep, etemp for the next egrad, epot, ekin, nfrag,
     # fragT(1), fragT(2), eerror
     trace = np.full((max(int(trace_steps), 1), 8), np.nan)
+    xtrace = np.full((max(int(trace_steps), 1), nat, 3), np.nan)
     if trace_steps:
         L.orc_md_set_trace(ptr(trace), int(trace_steps))
+        L.orc_md_set_xyz_trace(ptr(xtrace), int(trace_steps))
     L.orc_md_run_avg(C.byref(st), ptr(num, ip), ptr(imass, ip), ptr(mass), ptr(velof), ptr(xyz),
                      ptr(velo), ptr(grad), ptr(qat), ptr(lst, ip), fn,
                      C.cast(C.pointer(o), C.c_void_p), int(steps_limit), ptr(axyz), ptr(achrg))
     if trace_steps:
         L.orc_md_set_trace(None, 0)
+        L.orc_md_set_xyz_trace(None, 0)
     return dict(state=st, xyz=xyz, velo=velo, grad=grad, qat=qat, list=lst, axyz=axyz,
-                achrg=achrg, trace=trace[:st.nstep] if trace_steps else None)
+                achrg=achrg, trace=trace[:st.nstep] if trace_steps else None,
+                xyz_trace=xtrace[:st.nstep] if trace_steps else None)
 
 
 def gfn2_egrad_batch(systems, xyz_list, etemp, opts=None, nthreads=None):

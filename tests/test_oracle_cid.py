@@ -79,3 +79,58 @@ def test_failed_gradient_stops_before_the_first_step():
     s["xyz"][-1] = s["xyz"][s["nion"] - 1] + np.array([0.2, 0.0, 0.0])
     st = ob.cid_run(s, charge=1, opts=OPTS)["state"]
     assert st.gradfail == 1 and st.nstep == 0
+
+
+# ---- the production MD in its CID mode (md.f90, method 3, icoll >= 1) ----
+MD_ETEMP = 5000.0
+
+
+def md_cid(name, **kw):
+    tr, v, max_steps = CC.md_cases()[name]
+    return ob.md_run(tr, 0, max_steps=max_steps, nfragexit=1, etempin=MD_ETEMP, opts=OPTS, cid=True,
+                     new_velo=v, **kw)
+
+
+def test_md_cid_mode_takes_out_the_flight():
+    """md.f90:247-255, 466-487: with the ion's speed handed over, Ekin and
+    the temperature are those of the motion inside the ion.  The internal
+    dynamics do not depend on a uniform drift (the energy is translation
+    invariant), so the flying ion reports the resting one's energies, its
+    positions are the resting ones shifted by v t, and the speed measured
+    from the centre of mass each step is the one handed over."""
+    a, b = md_cid("c2_rest"), md_cid("c2_drift")
+    sa, sb = a["state"], b["state"]
+    assert sa.nstep == sb.nstep == 120 and sa.exit_reason == sb.exit_reason == 7
+    assert abs(sa.ekin - sb.ekin) <= 1e-9 and abs(sa.epot - sb.epot) <= 1e-9
+    assert abs(sa.atlast - sb.atlast) <= 1e-6 * sa.atlast
+    assert abs(sb.new_velo - 1500.0) <= 1e-6 and sa.new_velo < 1e-3
+    tr = CC.md_cases()["c2_drift"][0]
+    shift = (tr["velo"] - CC.md_cases()["c2_rest"][0]["velo"])[0] * 120 * CC.TSTEP
+    assert np.abs(b["xyz"] - a["xyz"] - shift).max() <= 1e-8
+    # without the speed handed over, the flight's kinetic energy 1/2 M v^2
+    # (0.013 Eh) stays in the first step's energy; Eerror is measured against
+    # the running mean of Epot + Ekin (no heating: nadd = 0), so after 120
+    # steps it is off by 1/2 M v^2 / 120
+    c = ob.md_run(tr, 0, max_steps=120, nfragexit=1, etempin=MD_ETEMP, opts=OPTS, cid=True, new_velo=0.0)
+    e_flight = 0.5 * tr["mass"].sum() * (1500.0 * CC.MSTOAU) ** 2
+    assert abs(sb.eerror - sa.eerror) <= 1e-9
+    assert abs((c["state"].eerror - sb.eerror) - e_flight / 120) <= 1e-10
+
+
+def test_md_cid_mode_averages_the_fragments_and_extends_the_run():
+    """md.f90:496-627, 672-698: a new fragment count starts the averaged
+    structure over the next cnt_steps = 50 steps and moves the end to
+    add_steps after it (500 steps for 11 atoms); a count that falls back
+    drops the average; the EI exits at nfragexit do not apply; the structure
+    md returns is the average over the 50 steps after the last split.  The
+    propane ion splits at step 103, joins at 113, splits at 348, joins at
+    367 and splits for good at 434."""
+    r = md_cid("c3_frag", trace_steps=3000)
+    st, tr = r["state"], r["trace"]
+    nf = tr[:, 4].astype(int)
+    assert list(np.nonzero(np.diff(nf))[0] + 2) == [103, 113, 348, 367, 434] and nf[0] == 1
+    f = 434
+    assert st.nstep == f + ob.oracle().orc_cid_add_steps(11) and st.exit_reason == 7
+    assert st.nfrag == 2 and st.check_fragmented == 2 and st.count_average == 0
+    want = r["xyz_trace"][f - 1:f + 49].mean(axis=0)
+    assert np.abs(r["axyz"] - want).max() <= 1e-12
