@@ -327,7 +327,9 @@ SCAN_BYTES = {
     # grad, qat (checkqc), velo, velof, mass (heating), num, xyz read; velo, list written
     "md_post": (24 + 8 + 24 + 8 + 8 + 4 + 24) + (24 + 4),
 }
-TRAJ_RECORD_BYTES = 2 * 336  # the trajectory record (MdTraj, 336 bytes), read and written
+# the EI fields of the trajectory record (328 of MdTraj's 440 bytes; the
+# CID-mode fields are read as one flag), read and written
+TRAJ_RECORD_BYTES = 2 * 328
 
 
 def scan_kernels(ntraj, steps=20, warmup=5):

@@ -197,6 +197,12 @@ typedef struct {
     double ttime, dtime, atlast, epot, ekin, eerror, etemp;
     double fragm[10];        /* fragment masses (amu), compacted as fragmass */
     double fragT[10];        /* fragment temperatures (K) */
+    /* CID mode (qcxms_md_set_cid; 0 otherwise): the highest fragment count
+     * since the last reset (check_fragmented), the averaging flag md keeps
+     * for the next call (count_average, SAVE in the reference) and the ion's
+     * last centre-of-mass speed in m/s (new_velo, handed back to cid()) */
+    int32_t check_fragmented, count_average;
+    double new_velo;
 } qcxms_md_result;
 
 /* exit_reason codes: 0 running, 1 SCF/checkqc failure (E = 0), 2 energy drift
@@ -227,9 +233,11 @@ int qcxms_md_run_group(qcxms_md **mds, int32_t n, int32_t nsteps, int32_t *runni
 int qcxms_md_read(qcxms_md *md, double *xyz, double *velo, double *grad, double *qat,
                   int32_t *list, qcxms_md_result *results);
 /* Averages over the last avdump (50) steps that md returns to main
- * (md.f90:355-363, 691-698, EI branch): axyz[3*natoms] = average geometry
- * (Bohr), the structure the fragment IPs are computed at, and
- * achrg[natoms] = average Mulliken charges.  Either pointer may be NULL. */
+ * (md.f90:355-363, 691-698): axyz[3*natoms] = average geometry (Bohr), the
+ * structure the fragment IPs are computed at (in the CID mode after a new
+ * fragment count: the structure averaged over the 50 steps after it,
+ * store_avxyz), and achrg[natoms] = average Mulliken charges.  Either
+ * pointer may be NULL. */
 int qcxms_md_read_averages(qcxms_md *md, double *axyz, double *achrg);
 /* Timing hooks of the ensemble's underlying batch (see qcxms_batch_kernel_time). */
 qcxms_batch *qcxms_md_batch(qcxms_md *md);
@@ -239,6 +247,19 @@ qcxms_batch *qcxms_md_batch(qcxms_md *md);
  * (src/main.F90:2255) and an SCF or drift failure keeps the run
  * (src/md.f90:329).  Default 1. */
 int qcxms_md_set_generation(qcxms_md *md, const int32_t *isec);
+/* The CID mode of the production MD (md.f90 with method = 3 after a
+ * collision, icoll >= 1; the call main.F90 makes after each cid()):
+ * trajectories with cid[s] != 0 take the ion's flight out of the kinetic
+ * energy (velo_cm[s]: the ion's centre-of-mass speed in m/s, new_velo of
+ * cid(); Ekin and the temperatures are those of the motion inside the ion,
+ * md.f90:247-255, 464-487), get no heating and no setetemp, a drift limit of
+ * 0.2 Eh, and no fragment exits: a new fragment count starts the averaged
+ * structure over the next 50 steps and moves the end to add_steps after it
+ * (cid.f90:232-234, md.f90:496-627); the run ends at max_steps.
+ * count_average[s] is the averaging flag the reference keeps between calls
+ * (may be NULL: 0).  Set once per trajectory, before its first step. */
+int qcxms_md_set_cid(qcxms_md *md, const int32_t *cid, const double *velo_cm,
+                     const int32_t *count_average);
 
 /* ---- fragmentation generations: the EI loop of src/main.F90:2208-2345 ----
  * Each trajectory runs its MD (md.f90); when it ends with mdok,

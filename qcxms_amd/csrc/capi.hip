@@ -909,7 +909,8 @@ qcxms_md* md_build(const qcxms_md* old, const MdLayout& L, const qcxms_md_params
         (st = mdalloc(md, &d_mass, (size_t)na, L.mass.data())) ||
         (st = mdalloc(md, &d_imass, (size_t)na, im.data())) || (st = mdalloc(md, &d_num, (size_t)na, nm.data())) ||
         (st = mdalloc(md, &md->m.list, (size_t)na, lst.data())) || (st = mdalloc(md, &md->m.avchrg, (size_t)na)) ||
-        (st = mdalloc(md, &md->m.avxyz, 3 * (size_t)na)) ||
+        (st = mdalloc(md, &md->m.avxyz, 3 * (size_t)na)) || (st = mdalloc(md, &md->m.avxyz2, 3 * (size_t)na)) ||
+        (st = mdalloc(md, &md->m.store, 3 * (size_t)na)) ||
         (st = mdalloc(md, &md->m.active, (size_t)ntraj, act.data())) ||
         (st = mdalloc(md, &d_src, (size_t)ntraj, L.src.data()))) {
         delete md;
@@ -1512,6 +1513,33 @@ void qcxms_md_destroy(qcxms_md* md)
 
 qcxms_batch* qcxms_md_batch(qcxms_md* md) { return md ? md->b : nullptr; }
 
+int qcxms_md_set_cid(qcxms_md* md, const int32_t* cid, const double* velo_cm, const int32_t* count_average)
+{
+    if (!md || !cid || !velo_cm) return fail("null argument");
+    for (int s = 0; s < md->ntraj; ++s)
+        if (cid[s] && !(velo_cm[s] >= 0.0 && velo_cm[s] < 1.0e12))
+            return fail("velo_cm must be a finite speed >= 0 (m/s)");
+    qcxms_batch* b = md->b;
+    std::vector<qx::MdTraj> tr(md->ntraj);
+    HIPCHK(hipMemcpyAsync(tr.data(), md->m.traj, sizeof(qx::MdTraj) * md->ntraj, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int s = 0; s < md->ntraj; ++s) {
+        if (!cid[s]) continue;
+        if (tr[s].nstep != 0 || tr[s].cid) return fail("the CID mode is set once, before the first step");
+    }
+    for (int s = 0; s < md->ntraj; ++s) {
+        if (!cid[s]) continue;
+        tr[s].cid = 1;
+        tr[s].new_velo = velo_cm[s];
+        tr[s].count_average = count_average ? (count_average[s] != 0) : 0;
+    }
+    HIPCHK(hipMemcpyAsync(md->m.traj, tr.data(), sizeof(qx::MdTraj) * md->ntraj, hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(qx::k_md_cid_start, dim3(md->ntraj), dim3(64), 0, b->stream, b->d, md->m);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return QCXMS_OK;
+}
+
 int qcxms_md_set_generation(qcxms_md* md, const int32_t* isec)
 {
     if (!md || !isec) return fail("null argument");
@@ -1661,6 +1689,9 @@ int qcxms_md_read(qcxms_md* md, double* xyz, double* velo, double* grad, double*
                 r.fragm[f] = t.fragm[f];
                 r.fragT[f] = t.fragT[f];
             }
+            r.check_fragmented = t.cid ? t.check_fragmented : 0;
+            r.count_average = t.count_average;
+            r.new_velo = t.new_velo;
         }
     return QCXMS_OK;
 }
@@ -1670,21 +1701,27 @@ int qcxms_md_read_averages(qcxms_md* md, double* axyz, double* achrg)
     if (!md) return fail("null ensemble");
     qcxms_batch* b = md->b;
     const size_t na = b->natoms;
-    std::vector<double> sx(3 * na), sq(na);
+    std::vector<double> sx(3 * na), sq(na), so(3 * na);
     std::vector<qx::MdTraj> tr(md->ntraj);
     HIPCHK(hipMemcpyAsync(sx.data(), md->m.avxyz, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(so.data(), md->m.store, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipMemcpyAsync(sq.data(), md->m.avchrg, sizeof(double) * na, hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipMemcpyAsync(tr.data(), md->m.traj, sizeof(qx::MdTraj) * md->ntraj, hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
-    // md.f90:691-698 (EI: axyz = avxyz / kdump); a trajectory that has not
+    // md.f90:691-698 (axyz = avxyz / kdump, or with CID after a new fragment
+    // count the averaged structure store_avxyz); a trajectory that has not
     // completed a step yet has nothing averaged and gets zeros
     for (int s = 0; s < md->ntraj; ++s) {
         const int kd = tr[s].nstep > 0 ? tr[s].kdump : 0;
+        const bool stored = tr[s].cid && tr[s].check_fragmented > 1;
         const SysInfo& si = b->info[s];
         for (int i = si.atom_off; i < si.atom_off + si.nat; ++i) {
             if (achrg) achrg[i] = kd ? sq[i] / kd : 0.0;
             if (axyz)
-                for (int c = 0; c < 3; ++c) axyz[3 * (size_t)i + c] = kd ? sx[3 * (size_t)i + c] / kd : 0.0;
+                for (int c = 0; c < 3; ++c) {
+                    const size_t k = 3 * (size_t)i + c;
+                    axyz[k] = stored ? so[k] : (kd ? sx[k] / kd : 0.0);
+                }
         }
     }
     return QCXMS_OK;

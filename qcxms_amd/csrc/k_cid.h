@@ -24,7 +24,6 @@
 
 namespace qx {
 
-constexpr double kMsToAu = 1.0 / 2.18769126364e+06;  // xtb_mctc_convert.f90:63
 constexpr double kAutoEvCid = 27.21138505;           // xtb_mctc_convert.f90:27
 
 struct CidTraj {
@@ -59,22 +58,6 @@ struct CidPtrs {
     int* active;         // per trajectory: 1 while running
 };
 
-// utility.f90:541-562 (sequential)
-QX_DEV void cid_com(int nuc, const double* mass, const double* x, double cm[3])
-{
-    double tm = 0.0;
-    cm[0] = cm[1] = cm[2] = 0.0;
-    for (int i = 0; i < nuc; ++i) {
-        cm[0] = cm[0] + mass[i] * x[3 * i];
-        cm[1] = cm[1] + mass[i] * x[3 * i + 1];
-        cm[2] = cm[2] + mass[i] * x[3 * i + 2];
-        tm = tm + mass[i];
-    }
-    cm[0] = cm[0] / tm;
-    cm[1] = cm[1] / tm;
-    cm[2] = cm[2] / tm;
-}
-
 // cid.f90:1124-1139: distance of the last atom to cm
 QX_DEV double cid_dist(int nuc0, const double* x, const double cm[3])
 {
@@ -94,7 +77,7 @@ __global__ __launch_bounds__(64) void k_cid_init(BatchPtrs B, CidPtrs C)
     const int ok = B.status[sys] == 0 && checkqc(t.nuc0, e, B.grad + 3 * (size_t)off, B.qat + off, si.charge);
     t.epot = e;
     double cm[3];
-    cid_com(t.nuc, C.mass + off, B.xyz + 3 * (size_t)off, cm);
+    md_center_of_mass(t.nuc, C.mass + off, B.xyz + 3 * (size_t)off, cm);
     for (int c = 0; c < 3; ++c) t.old_cm[c] = cm[c];
     t.lowest = t.highest = cid_dist(t.nuc0, B.xyz + 3 * (size_t)off, cm);
     if (!ok) {
@@ -154,7 +137,7 @@ __global__ __launch_bounds__(64) void k_cid_post(BatchPtrs B, CidPtrs C, CidPara
         s_stop = !ok;
         if (ok) {
             double cm[3];
-            cid_com(nuc, mass, x, cm);
+            md_center_of_mass(nuc, mass, x, cm);
             const double d0 = cm[0] - t.old_cm[0], d1 = cm[1] - t.old_cm[1], d2 = cm[2] - t.old_cm[2];
             const double cm_out = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
             for (int c = 0; c < 3; ++c) t.old_cm[c] = cm[c];
@@ -256,7 +239,7 @@ __global__ __launch_bounds__(64) void k_cid_post(BatchPtrs B, CidPtrs C, CidPara
     if (t.distance_dump == p.dumpdist) {
         t.distance_dump = 0;
         double cm[3];
-        cid_com(nuc, mass, x, cm);
+        md_center_of_mass(nuc, mass, x, cm);
         const double d = cid_dist(t.nuc0, x, cm);
         if (d > t.highest) t.highest = d;
         if (d < t.lowest) t.lowest = d;

@@ -32,6 +32,15 @@ struct MdTraj {
     double ekin, epot, tav, epav, ekav, edum, eerror, ttime, dtime, atlast, ekinstart, etemp,
         eimp, fadd, tadd;
     double fragm[10], fragT[10];
+    // CID mode (md.f90 with method 3 after a collision, icoll >= 1): cid
+    // selects it; count_average is the reference's SAVE flag, handed in and
+    // back; check_fragmented, cnt, save_natf the averaged-structure count;
+    // add_steps the extension after a new fragment count (cid.f90:232-234)
+    int cid, check_fragmented, count_average, cnt, add_steps;
+    int save_natf[10];
+    // the ion's centre-of-mass speed (m/s, handed in by cid() and back),
+    // the temperature without the flight, the ion's mass and last centre
+    double new_velo, new_temp, summass, old_cm[3];
 };
 
 struct MdParams {
@@ -50,7 +59,37 @@ struct MdPtrs {
     double* avchrg;  // running charge sums (md.f90:355-361)
     double* avxyz;   // running coordinate sums, 3 per atom (md.f90:357,363)
     int* active;     // per trajectory: 1 while running
+    double* avxyz2;  // CID mode: coordinate sums since the count started, 3 per atom
+    double* store;   // CID mode: the averaged structure (store_avxyz), 3 per atom
 };
+
+constexpr double kMsToAu = 1.0 / 2.18769126364e+06;  // m/s in au (xtb_mctc_convert.f90:63)
+constexpr int kCntSteps = 50;                        // md.f90 cnt_steps
+
+// cid.f90:232-234
+QX_DEV int cid_add_steps(int nuc)
+{
+    int a = 0;
+    if (nuc > 10) a = (nuc / 10) * 500;
+    if (nuc >= 40) a = (nuc / 10) * 1000;
+    return a;
+}
+
+// utility.f90:541-562 (sequential sums) of nat atoms
+QX_DEV void md_center_of_mass(int nat, const double* mass, const double* x, double cm[3])
+{
+    double tm = 0.0;
+    cm[0] = cm[1] = cm[2] = 0.0;
+    for (int i = 0; i < nat; ++i) {
+        cm[0] = cm[0] + mass[i] * x[3 * i];
+        cm[1] = cm[1] + mass[i] * x[3 * i + 1];
+        cm[2] = cm[2] + mass[i] * x[3 * i + 2];
+        tm = tm + mass[i];
+    }
+    cm[0] = cm[0] / tm;
+    cm[1] = cm[1] / tm;
+    cm[2] = cm[2] / tm;
+}
 
 // utility.f90:69-86
 QX_DEV double setetemp(int nfrag, double eimp, double ax, double ieetemp)
@@ -286,6 +325,30 @@ __global__ __launch_bounds__(64) void k_md_init(BatchPtrs B, MdPtrs M, MdParams 
     }
 }
 
+// CID mode start (md.f90:247-255) of the trajectories with t.cid set before
+// their first step: the ion's mass and centre, and Ekin without the kinetic
+// energy of the flight at the speed new_velo handed over by cid()
+__global__ __launch_bounds__(64) void k_md_cid_start(BatchPtrs B, MdPtrs M)
+{
+    const int sys = blockIdx.x;
+    if (threadIdx.x != 0 || !M.active[sys]) return;
+    MdTraj& t = M.traj[sys];
+    if (!t.cid) return;
+    const SysInfo si = B.info[sys];
+    const int nat = si.nat, off = si.atom_off;
+    md_center_of_mass(nat, M.mass + off, B.xyz + 3 * (size_t)off, t.old_cm);
+    double summass = 0.0;
+    for (int i = 0; i < nat; ++i) summass = summass + M.mass[off + i];
+    t.summass = summass;
+    const double vv = t.new_velo * kMsToAu;
+    const double e_kin_diff = t.ekin - 0.5 * summass * (vv * vv);
+    t.new_temp = (2.0 * e_kin_diff) / (3.0 * kKB * nat);
+    t.ekin = e_kin_diff;
+    t.add_steps = cid_add_steps(nat);
+    t.check_fragmented = 1;
+    t.cnt = 0;
+}
+
 // Re-layout of a running ensemble: system s of the new batch (Bn, Mn) takes
 // the whole state of system src[s] of the old one (Bo, Mo), the same
 // molecule at another place: coordinates, gradient, charges, velocities,
@@ -315,6 +378,8 @@ __global__ __launch_bounds__(64) void k_md_repack(BatchPtrs Bo, MdPtrs Mo, Batch
         Bn.grad[3 * an + i] = Bo.grad[3 * ao + i];
         Mn.velo[3 * an + i] = Mo.velo[3 * ao + i];
         Mn.avxyz[3 * an + i] = Mo.avxyz[3 * ao + i];
+        Mn.avxyz2[3 * an + i] = Mo.avxyz2[3 * ao + i];
+        Mn.store[3 * an + i] = Mo.store[3 * ao + i];
     }
     for (int i = lane; i < sn.nx; i += 64) {
         Bn.x[sn.x_off + i] = Bo.x[so.x_off + i];
@@ -344,9 +409,84 @@ struct MdLds {
     double term[3 * QX_MAX_NAT];
     double atoms[5 * QX_MAX_NAT];
     unsigned long long adj[QX_MAX_NAT];
-    int s_exit, s_more;
+    int s_exit, s_more, s_cidop, s_cidcnt;
     double s_T;
 };
+
+// CID mode: the averaged structure over the cnt_steps steps after a new
+// fragment count (md.f90:496-627).  Thread 0 keeps the count and decides; every
+// thread then updates its coordinates of the sums (avxyz2) and the average
+// (store_avxyz).  Called by all threads of the block after the connectivity
+// (list) of this step.
+QX_DEV void md_cid_average(const BatchPtrs& B, const MdPtrs& M, int sys, int nfrag, MdLds& Lm)
+{
+    enum { kZero = 1, kAdd = 2, kDrop = 4, kFinal = 8 };
+    const SysInfo si = B.info[sys];
+    const int nat = si.nat, off = si.atom_off;
+    MdTraj& t = M.traj[sys];
+    if (threadIdx.x == 0) {
+        int op = 0, cnt_used = 0;
+        if (nfrag > t.check_fragmented) {
+            t.count_average = 1;
+            t.check_fragmented = nfrag;
+            t.max_steps = t.nstep + t.add_steps;
+        }
+        // a count that falls back drops the average
+        if (nfrag < t.check_fragmented && t.count_average) {
+            t.cnt = 0;
+            op |= kZero;
+            t.count_average = 0;
+            t.check_fragmented = 1;
+        }
+        if (t.count_average) {
+            t.cnt = t.cnt + 1;
+            cnt_used = t.cnt;
+            op |= kAdd;
+            int natf[10];
+            for (int f = 0; f < 10; ++f) natf[f] = 0;
+            for (int i = 0; i < nat; ++i) {
+                const int l = M.list[off + i];
+                if (l >= 1 && l <= 10) natf[l - 1] += 1;
+            }
+            // a fragment whose atom count changed restarts the count
+            for (int f = 0; f < nfrag && f < 10; ++f) {
+                if (t.cnt == 1) t.save_natf[f] = natf[f];
+                if (natf[f] != t.save_natf[f]) {
+                    t.cnt = 0;
+                    op |= kDrop;
+                    break;
+                }
+            }
+            if (t.cnt == kCntSteps) {
+                op |= kFinal;
+                t.cnt = 0;
+                t.count_average = 0;
+            }
+        }
+        Lm.s_cidop = op;
+        Lm.s_cidcnt = cnt_used;
+    }
+    __syncthreads();
+    const int op = Lm.s_cidop;
+    if (!op) return;
+    const double cnt = (double)Lm.s_cidcnt;
+    for (int k = threadIdx.x; k < 3 * nat; k += blockDim.x) {
+        const size_t i = 3 * (size_t)off + k;
+        double a2 = (op & kZero) ? 0.0 : M.avxyz2[i];
+        double st = (op & kZero) ? 0.0 : M.store[i];
+        if (op & kAdd) {
+            a2 = a2 + B.xyz[i];
+            st = a2 / cnt;
+        }
+        if (op & kDrop) {
+            a2 = 0.0;
+            st = 0.0;
+        }
+        if (op & kFinal) a2 = 0.0;  // the average stays in store
+        M.avxyz2[i] = a2;
+        M.store[i] = st;
+    }
+}
 
 // one MD loop iteration up to the force call (md.f90:285-395): time,
 // averages, drift and SCF checks, running sums, leap-frog.  Any block size
@@ -376,7 +516,8 @@ QX_DEV void md_pre_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& p
         }
         t.eerror = eav - t.epot - t.ekin;
         if (t.nfrag == 1) t.fragT[0] = t.tav / nstep;
-        const int err1 = (t.epot == 0.0), err2 = (fabs(t.eerror) > (double)0.1f);
+        // REAL(4) limits: 0.1 Eh, 0.2 Eh with CID (md.f90:324-328)
+        const int err1 = (t.epot == 0.0), err2 = (fabs(t.eerror) > (t.cid ? (double)0.2f : (double)0.1f));
         s_exit = 0;
         if (err1 || err2) {
             t.mdok = (t.nfrag > 1 && t.nfrag <= 4) || t.isec > 1;
@@ -387,7 +528,8 @@ QX_DEV void md_pre_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& p
             t.kdump = 0;
             t.atlast = 0.0;
         }
-        s_T = T;
+        // aTlast sums the temperature without the flight with CID (md.f90:365-366)
+        s_T = t.cid ? t.new_temp : T;
     }
     __syncthreads();
     // the flag is cleared only after every wavefront has passed the entry
@@ -434,7 +576,8 @@ QX_DEV void md_post_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& 
         t.kdump = t.kdump + 1;
         if (t.nfrag == 1) t.morestep = 0;
         if (t.nfrag > 1 && t.dtime < 1.0e-6) t.dtime = t.ttime / (double)1000.0f;
-        s_more = (t.nstep <= t.nadd && t.nfrag == 1);
+        // heating and setetemp are EI only (md.f90:438-445)
+        s_more = (!t.cid && t.nstep <= t.nadd && t.nfrag == 1);
     }
     __syncthreads();
     // heating: impactscale (impact.f90:12-55) with ff = fadd * nstep
@@ -454,12 +597,16 @@ QX_DEV void md_post_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& 
     double fragm[10];
     const int nfrag = dev_fragments(nat, M.num + off, B.xyz + 3 * (size_t)off, M.mass + off,
                                     M.list + off, fragm, adj);
+    // t.cid does not change during a run: the test is uniform over the block
+    if (t.cid && nfrag <= 6) md_cid_average(B, M, sys, nfrag, Lm);
     if (lane != 0) return;
-    // electronic temperature for the next egrad (md.f90:443-445): setetemp
-    // runs before fragment_structure, so it sees the previous step's nfrag
-    const double dum = t.eimp - t.eimp * (double)(float)t.nstep / (double)(float)t.nadd;
-    t.etemp = setetemp(t.nfrag, dum, p.ax, p.ieetemp);
-    ((SysInfo*)B.info)[sys].kt = t.etemp * gfn2::kKtoAu;
+    if (!t.cid) {
+        // electronic temperature for the next egrad (md.f90:443-445): setetemp
+        // runs before fragment_structure, so it sees the previous step's nfrag
+        const double dum = t.eimp - t.eimp * (double)(float)t.nstep / (double)(float)t.nadd;
+        t.etemp = setetemp(t.nfrag, dum, p.ax, p.ieetemp);
+        ((SysInfo*)B.info)[sys].kt = t.etemp * gfn2::kKtoAu;
+    }
     for (int f = 0; f < 10; ++f) t.fragm[f] = fragm[f];
     t.nfrag = nfrag;
     // fragment temperatures (md.f90:715-741)
@@ -486,6 +633,29 @@ QX_DEV void md_post_system(const BatchPtrs& B, const MdPtrs& M, const MdParams& 
     int reason = kRunning;
     if (nfrag > 6) {
         reason = kNfrag6;
+    } else if (t.cid) {
+        // the ion's speed from its centre's move over the step, and Ekin
+        // without the flight (md.f90:464-487)
+        double cm[3];
+        md_center_of_mass(nat, M.mass + off, B.xyz + 3 * (size_t)off, cm);
+        const double d0 = cm[0] - t.old_cm[0], d1 = cm[1] - t.old_cm[1], d2 = cm[2] - t.old_cm[2];
+        const double cm_out = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        t.new_velo = (cm_out / p.tstep) / kMsToAu;
+        t.old_cm[0] = cm[0];
+        t.old_cm[1] = cm[1];
+        t.old_cm[2] = cm[2];
+        const double vv = t.new_velo * kMsToAu;
+        const double e_kin = 0.5 * t.summass * (vv * vv);
+        const double e_kin_diff = t.ekin - e_kin;
+        t.new_temp = (2.0 * e_kin_diff) / (3.0 * kKB * nat);
+        t.ekin = e_kin_diff;
+        // no fragment exits: the run ends at max_steps, which a new fragment
+        // count moves to add_steps later (md.f90:504, 672-679)
+        if (t.nstep >= t.max_steps) {
+            t.fragstate = 1;
+            t.mdok = 1;
+            reason = kMaxStep;
+        }
     } else if (nfrag > nfragexit) {
         t.fragstate = 1;
         t.mdok = 1;

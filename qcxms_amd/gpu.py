@@ -293,7 +293,8 @@ class MdResult(C.Structure):
                                           "running", "scc_iterations", "pad")]
                 + [(n, C.c_double) for n in ("tav", "epav", "ekav", "ttime", "dtime", "atlast",
                                              "epot", "ekin", "eerror", "etemp")]
-                + [("fragm", C.c_double * 10), ("fragT", C.c_double * 10)])
+                + [("fragm", C.c_double * 10), ("fragT", C.c_double * 10)]
+                + [("check_fragmented", C.c_int32), ("count_average", C.c_int32), ("new_velo", C.c_double)])
 
 
 EXIT_REASONS = {0: "running", 1: "scf/checkqc failure", 2: "energy drift", 3: "more than 6 fragments",
@@ -315,6 +316,7 @@ def _md_sigs(L):
     L.qcxms_md_batch.restype = C.c_void_p
     L.qcxms_md_batch.argtypes = [C.c_void_p]
     L.qcxms_md_set_generation.argtypes = [C.c_void_p, _ip]
+    L.qcxms_md_set_cid.argtypes = [C.c_void_p, _ip, _dp, _ip]
     L.qcxms_fragments_batch.argtypes = [C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
     L.qcxms_leapfrog_batch.argtypes = [C.c_int32, _ip, _dp, _dp, C.c_double, _dp, _dp, _dp]
     L.qcxms_impactscale_batch.argtypes = [C.c_int32, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
@@ -326,7 +328,10 @@ class MdEnsemble:
 
     trajectories: list of dicts with num, xyz [nat,3] (Bohr), velo [nat,3]
     (au), mass [nat] (m_e), eimp (Eh), tadd (au); optional charge, imass,
-    velof, isec (fragmentation generation, default 1).  lanes > 1 splits the trajectories into that many contiguous
+    velof, isec (fragmentation generation, default 1), and for the MD after
+    a CID collision (md.f90 with method 3, icoll >= 1; qcxms_md_set_cid)
+    cid=True with new_velo (the ion's centre-of-mass speed, m/s) and
+    count_average (0/1).  lanes > 1 splits the trajectories into that many contiguous
     groups, each an ensemble on its own stream, advanced together
     (qcxms_md_run_group) so that the device overlaps their kernels.  fused
     runs whole MD iterations per trajectory in one persistent launch.
@@ -378,6 +383,13 @@ class MdEnsemble:
             if np.any(isec != 1) and L.qcxms_md_set_generation(self._hs[-1], _p(isec, _ip)):
                 self.close()
                 raise RuntimeError("qcxms_md_set_generation failed: " + last_error())
+            cid = np.array([int(bool(t.get("cid", False))) for t in part], dtype=np.int32)
+            if cid.any():
+                vcm = np.array([float(t.get("new_velo", 0.0)) for t in part])
+                cavg = np.array([int(t.get("count_average", 0)) for t in part], dtype=np.int32)
+                if L.qcxms_md_set_cid(self._hs[-1], _p(cid, _ip), _p(vcm), _p(cavg, _ip)):
+                    self.close()
+                    raise RuntimeError("qcxms_md_set_cid failed: " + last_error())
         self._h = self._hs[0]
 
     def close(self):

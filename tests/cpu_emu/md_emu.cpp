@@ -5,7 +5,8 @@
 //           int32 num[nat], charge[ntraj], imass[nat]; double mass[nat],
 //           xyz[3 nat], velo[3 nat], velof[nat], eimp[ntraj], tadd[ntraj]
 //   output: int32 rc, running; double xyz[3 nat], velo[3 nat], qat[nat];
-//           per trajectory: int32 nstep, exit_reason; double epot, ekin
+//           per trajectory: int32 nstep, exit_reason; double epot, ekin;
+//           int32 nfrag, check_fragmented; double new_velo
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -45,12 +46,28 @@ int main(int argc, char** argv)
     // SCC thresholds of the GPU tests when given (QX_EMU_ECONV, QX_EMU_PCONV)
     if (const char* e = std::getenv("QX_EMU_ECONV")) p.econv = std::atof(e);
     if (const char* e = std::getenv("QX_EMU_PCONV")) p.pconv = std::atof(e);
+    if (const char* e = std::getenv("QX_EMU_ETEMP")) p.etempin = std::atof(e);
+    if (const char* e = std::getenv("QX_EMU_NFRAGEXIT")) p.nfragexit = std::atoi(e);
+    // CID mode of every trajectory with these centre-of-mass speeds (m/s),
+    // comma-separated, one per trajectory (QX_EMU_CID)
+    std::vector<int32_t> cid;
+    std::vector<double> vcm;
+    if (const char* e = std::getenv("QX_EMU_CID")) {
+        for (const char* c = e; *c;) {
+            char* end;
+            vcm.push_back(std::strtod(c, &end));
+            c = *end ? end + 1 : end;
+        }
+        if ((int)vcm.size() != nt) return 2;
+        cid.assign(nt, 1);
+    }
     qcxms_md* md = qcxms_md_create(nt, off.data(), num.data(), chg.data(), mass.data(), imass.data(), xyz.data(),
                                    velo.data(), velof.data(), eimp.data(), tadd.data(), &p);
     int32_t rc = md ? 0 : -1, running = 0;
     std::vector<qcxms_md_result> res(nt);
     std::vector<double> q(nat);
-    if (md) {
+    if (md && !cid.empty()) rc = qcxms_md_set_cid(md, cid.data(), vcm.data(), nullptr);
+    if (md && !rc) {
         rc = qcxms_md_run(md, nsteps + 1, &running);
         if (!rc) rc = qcxms_md_read(md, xyz.data(), velo.data(), nullptr, q.data(), nullptr, res.data());
         qcxms_md_destroy(md);
@@ -68,6 +85,9 @@ int main(int argc, char** argv)
         std::fwrite(&res[t].exit_reason, sizeof(int32_t), 1, o);
         std::fwrite(&res[t].epot, sizeof(double), 1, o);
         std::fwrite(&res[t].ekin, sizeof(double), 1, o);
+        std::fwrite(&res[t].nfrag, sizeof(int32_t), 1, o);
+        std::fwrite(&res[t].check_fragmented, sizeof(int32_t), 1, o);
+        std::fwrite(&res[t].new_velo, sizeof(double), 1, o);
     }
     std::fclose(o);
     return 0;

@@ -62,9 +62,10 @@ def egrad_batch(systems, xyz_list, etemp, method=2, timeout=1800, stored=False, 
 EMU_MD = os.path.join(EMU_DIR, "_build", "md_emu")
 
 
-def md_run(trajs, nsteps, fused, use_graph=1, timeout=1800):
+def md_run(trajs, nsteps, fused, use_graph=1, timeout=1800, env=None):
     """MD ensemble through the emulated library: (rc, running, xyz, velo, qat,
-    [(nstep, exit_reason, epot, ekin)])."""
+    [(nstep, exit_reason, epot, ekin, nfrag, check_fragmented, new_velo)]).
+    env: extra environment of the driver (QX_EMU_* settings)."""
     nt = len(trajs)
     off = np.cumsum([0] + [len(t["num"]) for t in trajs]).astype(np.int32)
     cat = lambda k, dt: np.concatenate([np.asarray(t[k], dtype=dt).reshape(-1) for t in trajs])
@@ -83,8 +84,8 @@ def md_run(trajs, nsteps, fused, use_graph=1, timeout=1800):
             np.ones(nat).tofile(f)
             np.array([t["eimp"] for t in trajs], dtype=np.float64).tofile(f)
             np.array([t["tadd"] for t in trajs], dtype=np.float64).tofile(f)
-        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
-        p = subprocess.run([EMU_MD, fin, fout], capture_output=True, text=True, timeout=timeout, env=env)
+        penv = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", **(env or {}))
+        p = subprocess.run([EMU_MD, fin, fout], capture_output=True, text=True, timeout=timeout, env=penv)
         if p.returncode != 0:
             raise RuntimeError(f"emulated MD failed ({p.returncode}):\n{p.stderr[-6000:]}")
         raw = open(fout, "rb").read()
@@ -97,5 +98,7 @@ def md_run(trajs, nsteps, fused, use_graph=1, timeout=1800):
     for _ in range(nt):
         ns, ex = np.frombuffer(raw, np.int32, 2, o); o += 8
         ep, ek = np.frombuffer(raw, np.float64, 2, o); o += 16
-        res.append((int(ns), int(ex), float(ep), float(ek)))
+        nf, chk = np.frombuffer(raw, np.int32, 2, o); o += 8
+        vcm = float(np.frombuffer(raw, np.float64, 1, o)[0]); o += 8
+        res.append((int(ns), int(ex), float(ep), float(ek), int(nf), int(chk), vcm))
     return int(rc), int(running), xyz, velo, q, res
