@@ -421,36 +421,42 @@ QX_DEV bool eig_rotate(double* A, double* Cv, int m, int nrow, int lda, double t
     const int tid = threadIdx.x;
     if (offdiag_max(A, m, lda, red) < tol) return true;
     for (int it = 0; it < maxit; ++it) {
+        // one angle per pair i < j (K_ji = -K_ij): rows r and m - 1 - r of
+        // the upper triangle together hold m - 1 pairs, so pair b maps to
+        // (r, c) = (b / (m - 1), b % (m - 1)) without a square root (m even)
         double kmax = 0.0;
-        for (int p = tid; p < m * m; p += kNT) {
-            const int i = p / m, j = p - i * m;
-            double k = 0.0;
-            if (i != j) {
-                const int lo = i < j ? i : j, hi = i < j ? j : i;
-                const double a = A[(size_t)lo * lda + hi];
-                const double d = A[(size_t)hi * lda + hi] - A[(size_t)lo * lda + lo];
-                const double r = sqrt(d * d + 4.0 * a * a);
-                const double t = r > 0.0 ? 2.0 * (d >= 0.0 ? a : -a) / (fabs(d) + r) : 0.0;
-                const double th = atan(t);
-                k = i < j ? th : -th;
-            }
-            Y[(size_t)i * lda + j] = k;
-            kmax = fmax(kmax, fabs(k));
+        for (int b = tid; b < (m / 2) * (m - 1); b += kNT) {
+            const int r = b / (m - 1), c = b - r * (m - 1);
+            const bool first = c < m - 1 - r;
+            const int i = first ? r : m - 1 - r;
+            const int j = first ? r + 1 + c : m - r + (c - (m - 1 - r));
+            const double a = A[(size_t)i * lda + j];
+            const double d = A[(size_t)j * lda + j] - A[(size_t)i * lda + i];
+            const double rr = sqrt(d * d + 4.0 * a * a);
+            const double t = rr > 0.0 ? 2.0 * (d >= 0.0 ? a : -a) / (fabs(d) + rr) : 0.0;
+            const double th = atan(t);
+            Y[(size_t)i * lda + j] = th;
+            Y[(size_t)j * lda + i] = -th;
+            kmax = fmax(kmax, fabs(th));
         }
+        for (int i = tid; i < m; i += kNT) Y[(size_t)i * lda + i] = 0.0;
         kmax = block_max(kmax, red);
         if (kmax > 0.1) return false;
-        if (kmax > 1e-4) {
+        // Y = I + K + K^2/2 has Y^T Y = I + K^4/4: with max |K| <= 1e-5 that
+        // is below rounding and no Newton-Schulz step is needed
+        const bool quad = kmax > 1e-4 || kmax <= 1e-5;
+        if (quad) {
             mfma_gemm<false, false>(m, m, m, Y, lda, Y, lda, Z, lda, nullptr);
             __syncthreads();
         }
         for (int p = tid; p < m * m; p += kNT) {
             const int i = p / m, j = p - i * m;
             const size_t o = (size_t)i * lda + j;
-            Y[o] = (i == j ? 1.0 : 0.0) + Y[o] + (kmax > 1e-4 ? 0.5 * Z[o] : 0.0);
+            Y[o] = (i == j ? 1.0 : 0.0) + Y[o] + (quad ? 0.5 * Z[o] : 0.0);
         }
         __syncthreads();
         double* R = Y;
-        const int nns = kmax > 0.005 ? 2 : 1;
+        const int nns = kmax > 0.005 ? 2 : (kmax > 1e-5 ? 1 : 0);
         for (int s = 0; s < nns; ++s) {
             mfma_gemm<true, false>(m, m, m, R, lda, R, lda, Z, lda, nullptr);
             __syncthreads();
@@ -583,14 +589,19 @@ QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, con
     QX_PT(22, pt);
     double eaes = 0.0, eaxc = 0.0, es2 = 0.0, es3 = 0.0, ed4 = 0.0;
     if (wave == 0) {
-        const int i = lane;
+        // up to 32 atoms: lanes i and i + 32 take the two halves of the
+        // partners of atom i and add their sums; more atoms: one lane per atom
+        const bool split = nat <= 32;
+        const int i = split ? (lane & 31) : lane, h = split ? (lane >> 5) : 0;
+        const int jm = split ? (nat + 1) / 2 : nat;
+        const int j0 = h ? jm : 0, j1 = (split && !h) ? jm : nat;
+        double v0 = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
+        double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
         if (i < nat) {
             const int nn = nat * nat;
             const double* ag = B.aesg + (size_t)si.pair_off / QX_PAIR_STRIDE * QX_AES_COMP + i;
-            double v0 = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
-            double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
 #pragma unroll 4
-            for (int j = 0; j < nat; ++j) {
+            for (int j = j0; j < j1; ++j) {
                 const double* g = ag + (size_t)j * nat;
                 const double g0 = g[0], g1 = g[nn], g2 = g[2 * nn], g3 = g[3 * nn], g4 = g[4 * nn],
                              g5 = g[5 * nn], g6 = g[6 * nn], g7 = g[7 * nn], g8 = g[8 * nn],
@@ -610,6 +621,20 @@ QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, con
                 w4 += qj * g7;
                 w5 += qj * g8;
             }
+        }
+        if (split) {
+            v0 += __shfl_xor(v0, 32, 64);
+            d0 += __shfl_xor(d0, 32, 64);
+            d1 += __shfl_xor(d1, 32, 64);
+            d2 += __shfl_xor(d2, 32, 64);
+            w0 += __shfl_xor(w0, 32, 64);
+            w1 += __shfl_xor(w1, 32, 64);
+            w2 += __shfl_xor(w2, 32, 64);
+            w3 += __shfl_xor(w3, 32, 64);
+            w4 += __shfl_xor(w4, 32, 64);
+            w5 += __shfl_xor(w5, 32, 64);
+        }
+        if (i < nat && h == 0) {
             w1 *= 2.0;
             w3 *= 2.0;
             w4 *= 2.0;
@@ -644,8 +669,11 @@ QX_DEV void potentials(const SysInfo& si, const BatchPtrs& B, SysShared& ss, con
             ss.yr[k] = y;
             ed4 += -0.5 * ss.zr[k] * y;
         }
+        // the shell Coulomb rows on the threads the D4 rows leave free
         const double* G = B.gam + si.gam_off;
-        for (int a = t0; a < nsh; a += nt) {
+        const int ga = nd + nsh <= nt ? nd : 0;
+        for (int a = t0 - ga; a < nsh; a += nt) {
+            if (a < 0) break;
             double v = 0.0;
 #pragma unroll 8
             for (int b = 0; b < nsh; ++b) v += G[(size_t)b * nsh + a] * qsh[b];
