@@ -359,7 +359,9 @@ typedef struct {
     int32_t mchrg;      /* in/out: total charge (write_fragments.f90:302-310) */
     int32_t nfrag_ok;   /* in/out: run still counted (0 after more than 5 fragments) */
     double chrgcont;    /* in/out: charge carried by the continued fragment */
-    int32_t nfrag, tcont, ipok, nres, stat, pad;  /* out; tcont 0: no continuation */
+    int32_t nfrag, tcont, ipok, nres, stat;  /* out; tcont 0: no continuation */
+    int32_t icoll;      /* in: collision number of a CID run (> 0: the records carry
+                           it, write_fragments.f90:411-420); 0: EI records */
     double fragchrg3[10]; /* IP-Boltzmann fragment charges (fractional) */
     double fragchrg[10];  /* summed Mulliken charges qat per fragment */
     double fragm[10];     /* fragment masses (amu) */
@@ -368,7 +370,8 @@ typedef struct {
     char res[10][128];    /* records for qcxms.res, nres of them */
 } qcxms_fragment_report;
 
-/* manage_fragments (src/write_fragments.f90:12-457, EI records) for many
+/* manage_fragments (src/write_fragments.f90:12-457; EI records, or CID
+ * records with the collision number when report[t].icoll > 0) for many
  * finished trajectories: fragment_structure + fragmass on the final
  * coordinates xyz (list[natoms] out), IPs of the fragments at the averaged
  * coordinates axyz (qcxms_md_read_averages), Boltzmann charges at
@@ -426,6 +429,13 @@ qcxms_cid *qcxms_cid_create(int32_t ntraj, const int32_t *offsets, const int32_t
                             const double *tinit, const double *beta, const double *velo_cm,
                             const qcxms_cid_params *params);
 void qcxms_cid_destroy(qcxms_cid *cd);
+/* The two flags cid() keeps from one call to the next of a trajectory
+ * (cid.f90:125,147: locals with an initial value, hence SAVE): collided[s]
+ * (a collision was seen in an earlier call, so a fragmentation extends the
+ * run at once) and count_average[s] (the averaged-structure count still
+ * running).  Both start at 0; set them before the first step of a later
+ * collision (icoll > 1).  Either pointer may be NULL (left unchanged). */
+int qcxms_cid_set_saved(qcxms_cid *cd, const int32_t *collided, const int32_t *count_average);
 /* advance every running trajectory by up to nsteps steps; *running = those
  * still going */
 int qcxms_cid_run(qcxms_cid *cd, int32_t nsteps, int32_t *running);

@@ -2023,7 +2023,8 @@ int qcxms_manage_fragments_batch(int32_t ntraj, const int32_t* offsets, const in
                     cnt[nl] = fragel[(j - 1) * kMaxElem + e];
                     ++nl;
                 }
-            const std::string rec = res_record(r.fragchrg3[j - 1], r.mchrg, itrj[t], -1, isec, j, nl, zel, cnt);
+            const std::string rec = res_record(r.fragchrg3[j - 1], r.mchrg, itrj[t], r.icoll > 0 ? r.icoll : -1, isec, j, nl,
+                                                   zel, cnt);
             char* dst = (r.tcont > 0 && r.tcont == j) ? r.asave : r.res[r.nres++];
             std::snprintf(dst, sizeof r.asave, "%s", rec.c_str());
         }
@@ -2552,6 +2553,24 @@ qcxms_cid* qcxms_cid_create(int32_t ntraj, const int32_t* offsets, const int32_t
 }
 
 void qcxms_cid_destroy(qcxms_cid* cd) { delete cd; }
+
+int qcxms_cid_set_saved(qcxms_cid* cd, const int32_t* collided, const int32_t* count_average)
+{
+    if (!cd) return fail("null ensemble");
+    qcxms_batch* b = cd->b;
+    std::vector<qx::CidTraj> tr(cd->ntraj);
+    HIPCHK(hipMemcpyAsync(tr.data(), cd->c.traj, sizeof(qx::CidTraj) * cd->ntraj, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int s = 0; s < cd->ntraj; ++s)
+        if (tr[s].nstep != 0) return fail("the saved flags are set before the first step");
+    for (int s = 0; s < cd->ntraj; ++s) {
+        if (collided) tr[s].collided = collided[s] != 0;
+        if (count_average) tr[s].count_average = count_average[s] != 0;
+    }
+    HIPCHK(hipMemcpyAsync(cd->c.traj, tr.data(), sizeof(qx::CidTraj) * cd->ntraj, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return QCXMS_OK;
+}
 
 int qcxms_cid_run(qcxms_cid* cd, int32_t nsteps, int32_t* running)
 {

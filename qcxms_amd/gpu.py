@@ -637,7 +637,7 @@ def impactscale_batch(velos, masses, velofs, eimp, ff, e0):
 class FragmentReport(C.Structure):
     """qcxms_fragment_report: one trajectory's manage_fragments state."""
     _fields_ = ([("mchrg", C.c_int32), ("nfrag_ok", C.c_int32), ("chrgcont", C.c_double)]
-                + [(n, C.c_int32) for n in ("nfrag", "tcont", "ipok", "nres", "stat", "pad")]
+                + [(n, C.c_int32) for n in ("nfrag", "tcont", "ipok", "nres", "stat", "icoll")]
                 + [("fragchrg3", C.c_double * 10), ("fragchrg", C.c_double * 10),
                    ("fragm", C.c_double * 10), ("fragip", C.c_double * 30),
                    ("asave", C.c_char * 128), ("res", (C.c_char * 128) * 10)])
@@ -679,9 +679,10 @@ def fragment_ips_batch(nums, axyzs, lists, mchrgs):
 
 
 def manage_fragments_batch(nums, xyzs, axyzs, qats, masses, atlast, itrj, isec, maxsec, btf,
-                           mchrg, chrgcont, nfrag_ok=None, imasses=None):
-    """manage_fragments (src/write_fragments.f90:12-457, EI) for many finished
-    trajectories.  Returns (lists, [FragmentReport])."""
+                           mchrg, chrgcont, nfrag_ok=None, imasses=None, icoll=None):
+    """manage_fragments (src/write_fragments.f90:12-457) for many finished
+    trajectories; icoll (per trajectory) gives CID records with the
+    collision number, none EI records.  Returns (lists, [FragmentReport])."""
     L = library()
     _post_sigs(L)
     n = len(nums)
@@ -695,6 +696,7 @@ def manage_fragments_batch(nums, xyzs, axyzs, qats, masses, atlast, itrj, isec, 
         rep[t].mchrg = int(mchrg[t])
         rep[t].chrgcont = float(chrgcont[t])
         rep[t].nfrag_ok = 1 if nfrag_ok is None else int(nfrag_ok[t])
+        rep[t].icoll = 0 if icoll is None else int(icoll[t])
     lst = np.zeros(len(num), dtype=np.int32)
     if L.qcxms_manage_fragments_batch(n, _p(off, _ip), _p(num, _ip), _p(cat(xyzs, np.float64)),
                                       _p(cat(axyzs, np.float64)), _p(cat(qats, np.float64)),

@@ -57,3 +57,13 @@ def md_cases():
     out["c3_frag"] = (dict(num=num, xyz=xyz, velo=W.maxwell_boltzmann(mass, 6000.0, 3), mass=mass,
                            eimp=0.0, tadd=0.0), 0.0, 400)
     return out
+
+
+def run_ions():
+    """Ions for whole CID runs: a butane ion (two of them) and a pentane ion
+    at 500 K, hit by N2 at 60 eV."""
+    ions = []
+    for seed, nheavy in ((3, 4), (5, 4), (7, 5)):
+        num, xyz, velo, mass = ion(seed, nheavy)
+        ions.append(dict(num=num, xyz=xyz, velo=velo, mass=mass))
+    return ions

@@ -134,3 +134,52 @@ def test_md_cid_mode_averages_the_fragments_and_extends_the_run():
     assert st.nfrag == 2 and st.check_fragmented == 2 and st.count_average == 0
     want = r["xyz_trace"][f - 1:f + 49].mean(axis=0)
     assert np.abs(r["axyz"] - want).max() <= 1e-12
+
+
+def test_later_collision_setup_closed_forms():
+    """cid.f90:179-676 with icoll > 1: the ion keeps its velocities (flight
+    included) and moves onto its centre of mass; the gas sits at the old
+    centre + direc * max(v dt 2 step_dist, 17) plus 0.7 x the off-axis
+    offsets; beta from the ion's mass; the thermal temperature from all
+    velocities."""
+    num, xyz, velo, mass = CC.ion(5, 4)
+    xyz = xyz + np.array([0.3, -0.2, 0.5])
+    direc = np.array([0.6, 0.0, 0.8])
+    rand = (0.25, 0.5, 0.7, 0.2)
+    for v_ms in (9000.0, 200.0):
+        s = CID.cid_setup_next(num, xyz, velo, mass, CC.TSTEP, v_ms, direc, gas="N2", rand=rand)
+        nuc = len(num)
+        cm_old = CID.center_of_mass(mass, xyz)
+        assert np.abs(CID.center_of_mass(mass, s["xyz"][:nuc])).max() < 1e-12
+        assert np.array_equal(s["velo"][:nuc], velo) and np.abs(s["velo"][nuc:]).max() == 0.0
+        start = max(v_ms * CC.MSTOAU * 2 * CC.TSTEP * min(2 * nuc * 10, 800) * W.AUTOAA, 17.0)
+        # lmin = 0.7 picks the highest y (times f), lpos = 0.2 the lowest x (times g)
+        highy, lowx = xyz[:, 1].max(), xyz[:, 0].min()
+        want = cm_old + direc * start + np.array([0.7 * lowx * 0.5, 0.7 * highy * 0.25, 0.0])
+        assert np.abs(s["xyz"][-1] - want).max() < 1e-12
+        assert abs(s["xyz"][-2, 2] - s["xyz"][-1, 2] - 1.09 * W.AATOAU) < 1e-12
+        mgas = 14.007 * W.AMUTOAU
+        assert abs(s["beta"] - mgas / (mgas + mass.sum())) < 1e-15
+        ek = 0.5 * float((mass[:, None] * velo * velo).sum())
+        assert abs(s["tinit"] - ek / (1.5 * nuc * W.KB)) < 1e-9 * s["tinit"]
+        assert abs(s["eimpact_ev"] - 0.5 * mass.sum() * (v_ms * CC.MSTOAU) ** 2 * W.AUTOEV) < 1e-12
+
+
+def test_mfp_steps_and_continuation_rules():
+    """main.F90:1816-1829: nmax = 100 nuc, cut to 75 / 60 / 50 % from the
+    third / fourth / fifth MD on, within 1000..10000; main.F90:1683-1735: the
+    continued fragment's atoms in order, their nuclear-charge-weighted centre
+    at the origin, velocities kept."""
+    from qcxms_amd.cid_run import continue_fragment, nmax_mfp
+    assert [nmax_mfp(n, 2) for n in (5, 14, 60, 150)] == [1000, 1400, 6000, 10000]
+    assert [nmax_mfp(40, s) for s in (2, 3, 4, 5, 9)] == [4000, 3000, 2400, 2000, 2000]
+    assert nmax_mfp(14, 3) == 1050 and nmax_mfp(14, 4) == 1000
+    num, xyz, velo, mass = CC.ion(5, 4)
+    lst = np.where(np.arange(len(num)) % 3 == 0, 2, 1).astype(np.int32)
+    n2, x2, v2, m2, i2 = continue_fragment(num, xyz, velo, mass, np.full(len(num), -1), lst, 2)
+    sel = lst == 2
+    assert np.array_equal(n2, num[sel]) and np.array_equal(v2, velo[sel]) and np.array_equal(m2, mass[sel])
+    assert np.abs((n2[:, None] * x2).sum(0)).max() < 1e-12
+    assert np.abs((x2 - xyz[sel]) - (x2 - xyz[sel])[0]).max() < 1e-15
+    n0, x0, _, _, _ = continue_fragment(num, xyz, velo, mass, np.full(len(num), -1), lst, 0)
+    assert len(n0) == len(num) and np.abs((num[:, None] * x0).sum(0)).max() < 1e-12
