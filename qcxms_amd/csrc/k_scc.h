@@ -267,7 +267,8 @@ QX_DEV void load_block(const double* A, int lda, int2 ip, int2 ir, double b[4])
 }
 
 // Parallel cyclic threshold Jacobi on symmetric A (m x m, m even, leading
-// dimension lda).  Pairs with |a_pq| < tol are not rotated; a sweep ends the
+// dimension lda).  Pairs with |a_pq| < tol, or whose rotation has |sin| <=
+// min_s, are not rotated; a sweep ends the
 // loop when no off-diagonal element of the result reaches tol.  Each rotation
 // is also applied to columns p, q of the nrow x m matrix Cv (leading
 // dimension lda), so Cv ends as Cv V with A = V^T A_in V diagonal.
@@ -277,7 +278,7 @@ QX_DEV void load_block(const double* A, int lda, int2 ip, int2 ir, double b[4])
 // every round, so their parameters and entries are fetched in two LDS round
 // trips (parameters, then data) before any store.
 QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, int maxsweep,
-                  double2* rot, int2* rpq, double* red, int* s_any)
+                  double2* rot, int2* rpq, double* red, int* s_any, double min_s = 0.0)
 {
     const int half = m / 2;
     if (offdiag_max(A, m, lda, red) < tol) return 0;
@@ -310,7 +311,8 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
                 double2 cs = make_double2(1.0, 0.0);
                 if (fabs(apq) >= tol) {
                     cs = jacobi_rotation(app, aqq, apq);
-                    s_any[f] = 1;
+                    if (fabs(cs.y) > min_s) s_any[f] = 1;
+                    else cs = make_double2(1.0, 0.0);
                 }
                 rot[k] = cs;
                 rpq[k] = make_int2(p, q);
@@ -401,26 +403,36 @@ QX_DEV int jacobi(double* A, double* Cv, int m, int nrow, int lda, double tol, i
     return sweep;
 }
 
+constexpr double kCoarseSin = 0.02;  // |sin| of the rotations a coarse sweep applies
+constexpr int kCoarseSweeps = 3;      // coarse sweeps before the refinement gives up
+
 // Simultaneous-rotation refinement of a nearly diagonal symmetric A (m x m,
 // leading dimension lda) by FP64 MFMA products.  Every pair (i, j) gets the
 // angle of the Jacobi rotation that would zero a_ij alone; with K the
 // antisymmetric matrix of the angles, Y = I + K (+ K^2 / 2 when max |K| >
-// 1e-4) is orthogonal up to O(K^2) (O(K^4)), and one or two Newton-Schulz
-// steps R <- R (3 I - R^T R) / 2 make it orthogonal to rounding.  Then
-// A <- R^T A R and Cv <- Cv R (nrow x m).  The off-diagonal decreases
-// quadratically (errors of the simultaneous angles are second order in the
-// off-diagonal over the eigenvalue gaps).  Returns true when the largest
-// off-diagonal magnitude is below tol after at most maxit rounds; false,
-// possibly after some rounds, when an angle exceeds 0.1 (near-degenerate
-// pairs with a relatively large coupling: the series would need more terms)
-// or the rounds did not suffice, and the caller finishes with Jacobi.  Y, Z,
-// T: m x m scratch (leading dimension lda).  Called by all threads.
+// 1e-4 or <= 1e-5) is orthogonal up to O(K^2) (O(K^4)), and none, one or two
+// Newton-Schulz steps R <- R (3 I - R^T R) / 2 make it orthogonal to
+// rounding.  Then A <- R^T A R and Cv <- Cv R (nrow x m).  The off-diagonal
+// decreases quadratically (errors of the simultaneous angles are second
+// order in the off-diagonal over the eigenvalue gaps).  An angle above 0.1
+// (near-degenerate pairs with a relatively large coupling, typically at the
+// first SCC iteration after the geometry moved: the series would need more
+// terms) calls for a coarse Jacobi sweep that rotates only the pairs whose
+// rotation has |sin| > kCoarseSin, up to kCoarseSweeps times; from the first
+// one on, pairs with a coupling below tol are left alone, as Jacobi leaves
+// them.  Returns true when the largest off-diagonal magnitude is below tol
+// after at most maxit rounds (besides the coarse sweeps); false, possibly
+// after some rounds, when the angles stay above 0.1 or the rounds did not
+// suffice, and the caller finishes with Jacobi.  Y, Z, T: m x m scratch
+// (leading dimension lda); rot, rpq, s_any: the Jacobi buffers.  Called by
+// all threads.
 QX_DEV bool eig_rotate(double* A, double* Cv, int m, int nrow, int lda, double tol, int maxit, double* Y,
-                       double* Z, double* T, double* red)
+                       double* Z, double* T, double* red, double2* rot, int2* rpq, int* s_any)
 {
     const int tid = threadIdx.x;
     if (offdiag_max(A, m, lda, red) < tol) return true;
-    for (int it = 0; it < maxit; ++it) {
+    int coarse = 0;
+    for (int it = 0; it < maxit + coarse; ++it) {
         // one angle per pair i < j (K_ji = -K_ij): rows r and m - 1 - r of
         // the upper triangle together hold m - 1 pairs, so pair b maps to
         // (r, c) = (b / (m - 1), b % (m - 1)) without a square root (m even)
@@ -434,14 +446,25 @@ QX_DEV bool eig_rotate(double* A, double* Cv, int m, int nrow, int lda, double t
             const double d = A[(size_t)j * lda + j] - A[(size_t)i * lda + i];
             const double rr = sqrt(d * d + 4.0 * a * a);
             const double t = rr > 0.0 ? 2.0 * (d >= 0.0 ? a : -a) / (fabs(d) + rr) : 0.0;
-            const double th = atan(t);
+            // after a coarse sweep, pairs below tol are left alone as Jacobi leaves them
+            const double th = (coarse && fabs(a) < tol) ? 0.0 : atan(t);
             Y[(size_t)i * lda + j] = th;
             Y[(size_t)j * lda + i] = -th;
             kmax = fmax(kmax, fabs(th));
         }
         for (int i = tid; i < m; i += kNT) Y[(size_t)i * lda + i] = 0.0;
         kmax = block_max(kmax, red);
-        if (kmax > 0.1) return false;
+        if (kmax > 0.1) {
+            // strongly coupled pairs (near-degenerate orbitals after the
+            // geometry moved): one Jacobi sweep over the pairs whose rotation
+            // exceeds kCoarseSin, up to kCoarseSweeps times, then the
+            // simultaneous rotations again
+            if (coarse == kCoarseSweeps) return false;
+            ++coarse;
+            jacobi(A, Cv, m, nrow, lda, tol, 1, rot, rpq, red, s_any, kCoarseSin);
+            QX_PH_ADD(29, 1);
+            continue;
+        }
         // Y = I + K + K^2/2 has Y^T Y = I + K^4/4: with max |K| <= 1e-5 that
         // is below rounding and no Newton-Schulz step is needed
         const bool quad = kmax > 1e-4 || kmax <= 1e-5;
@@ -863,7 +886,7 @@ QX_DEV void scc_body(int sys, const SysInfo& si, const BatchPtrs& B, const SccOp
         double jtol = fmin(1e-6, 1e-3 * errp);
         if (jtol < opt.jacobi_tol) jtol = opt.jacobi_tol;
         int nsweep = 0;
-        if (EIG && bf.E && eig_rotate(A, Cc, m, n, lb, jtol, 3, Ft, P, bf.E, ss.red)) {
+        if (EIG && bf.E && eig_rotate(A, Cc, m, n, lb, jtol, 3, Ft, P, bf.E, ss.red, rot, rpq, s_any)) {
             QX_PH_ADD(15, 1);
         } else {
             nsweep = jacobi(A, Cc, m, n, lb, jtol, 50, rot, rpq, ss.red, s_any);
