@@ -11,6 +11,6 @@ constexpr int kTuneTable[][3] = {
     {56, 0, 1},
     {256, 1, 0},
 };
-constexpr int kTuneMixed[2] = {1, 0};
+constexpr int kTuneMixed[2] = {1, 1};
 
 }  // namespace qx
