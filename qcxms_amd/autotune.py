@@ -15,7 +15,7 @@ keeps the fastest.  Output:
 DIR/tuning.json (every timing) and DIR/tuning_table.h, the table the library
 compiles in (qcxms_amd/csrc/tuning_table.h).
 
-Run on the GPU:  python -m qcxms_amd.autotune --out gpurun_out/tune
+Run on the GPU:  python -m qcxms_amd.autotune --out DIR
 """
 import argparse
 import json
@@ -99,7 +99,7 @@ def header(table, mixed):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default="gpurun_out/tune")
+    ap.add_argument("--out", default="tune_out")
     ap.add_argument("--reps", type=int, default=5)
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)

@@ -48,7 +48,7 @@ def summarize(logdir):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out", "race_audit"))
+    ap.add_argument("--out", default=os.path.join(HERE, "_build", "race_audit"))
     args = ap.parse_args()
     subprocess.run(["make", "-s", "-C", HERE, "tsan"], check=True)
     eb.EMU_BIN = os.path.join(HERE, "_build", "egrad_emu_tsan")
