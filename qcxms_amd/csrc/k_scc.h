@@ -510,6 +510,62 @@ QX_DEV bool eig_rotate(double* A, double* Cv, int m, int nrow, int lda, double t
     return false;
 }
 
+// The Broyden weights of up to kWaveSolve secant pairs on wavefront 0: solve
+// (B) x = diag(om) gm for the n x n symmetric positive definite B (leading
+// dimension lb) and return gm <- diag(om) x.  Lane i keeps row i of B in
+// registers; a right-looking Cholesky factor B = L L^T (column j: lane j's
+// pivot, the other lanes' multipliers, rank-1 update of the trailing rows by
+// lane broadcasts), then L y = diag(om) gm forward and L^T x = y backward.
+// Called by the 64 threads of wavefront 0.
+constexpr int kWaveSolve = 16;
+QX_DEV void broyden_solve_wave(const double* B, int lb, double* gm, const double* om, int n)
+{
+    const int i = threadIdx.x & 63;
+    double r[kWaveSolve];
+#pragma unroll
+    for (int k = 0; k < kWaveSolve; ++k)
+        r[k] = (i < n && k < n) ? B[(size_t)i * lb + k] : (i == k ? 1.0 : 0.0);
+    double g = i < n ? gm[i] * om[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kWaveSolve; ++j) {
+        if (j < n) {
+            const double d = sqrt(__shfl(r[j], j, 64));
+            if (i == j) r[j] = d;
+            else if (i > j) r[j] = r[j] / d;
+#pragma unroll
+            for (int k = j + 1; k < kWaveSolve; ++k) {
+                const double lkj = __shfl(r[j], k, 64);
+                if (i >= k) r[k] = r[k] - r[j] * lkj;
+            }
+        }
+    }
+    // forward: L y = g
+#pragma unroll
+    for (int k = 0; k < kWaveSolve; ++k) {
+        if (k < n) {
+            const double yk = __shfl(g, k, 64) / __shfl(r[k], k, 64);
+            if (i == k) g = yk;
+            else if (i > k) g = g - r[k] * yk;
+        }
+    }
+    // backward: L^T x = y (lane i takes L[k][i] from lane k)
+#pragma unroll
+    for (int k = kWaveSolve - 1; k >= 0; --k) {
+        if (k < n) {
+            const double xk = __shfl(g, k, 64) / __shfl(r[k], k, 64);
+            double lki = 0.0;
+#pragma unroll
+            for (int c = 0; c < kWaveSolve; ++c) {
+                const double v = __shfl(r[c], k, 64);
+                if (c == i) lki = v;
+            }
+            if (i == k) g = xk;
+            else if (i < k) g = g - lki * xk;
+        }
+    }
+    if (i < n) gm[i] = g * om[i];
+}
+
 // Fermi occupations per spin on wavefront 0 for nalpha electrons; returns -TS.
 // The chemical potential is found by Newton steps safeguarded by a bracket
 // (bisection whenever a step leaves it), starting from mu_io when it lies
@@ -1071,7 +1127,9 @@ QX_DEV void scc_body(int sys, const SysInfo& si, const BatchPtrs& B, const SccOp
             bt[(size_t)i * lbt + j] = om[i] * om[j] * ha[(size_t)i * mem + j] + (i == j ? 1e-4 : 0.0);
         }
         __syncthreads();
-        if (tid == 0) {
+        if (it1 <= kWaveSolve) {
+            if (tid < 64) broyden_solve_wave(bt, lbt, gm, om, it1);
+        } else if (tid == 0) {
             for (int i = 0; i < it1; ++i) gm[i] *= om[i];
             for (int j = 0; j < it1; ++j) {
                 double d = bt[(size_t)j * lbt + j];

@@ -2,7 +2,7 @@
 // the kernels on the host (built with clang++ -fsanitize=address).
 //
 // Every GPU thread of a block is an OS thread; blocks of a launch run one
-// after the other.  __syncthreads is a block barrier, __shfl_xor and the
+// after the other.  __syncthreads is a block barrier, __shfl_xor, __shfl and the
 // FP64 MFMA 16x16x4 tile product exchange operands through per-wavefront
 // slots between two wavefront barriers.  Threads that leave a kernel early
 // drop out of the barriers, as exited wavefronts do on the GPU.  Dynamic LDS
@@ -174,6 +174,25 @@ inline T shfl_xor(T v, int mask, int width)
     return out;
 }
 
+// __shfl: every lane receives the value of lane src (within width lanes)
+template <class T>
+inline T shfl(T v, int src, int width)
+{
+    static_assert(sizeof(T) <= sizeof(double), "shuffle of a wider type");
+    const int l = threadIdx.x & 63;
+    Wave& w = wave();
+    double slot = 0.0;
+    std::memcpy(&slot, &v, sizeof(T));
+    w.a[l] = slot;
+    w.bar.wait("wavefront shuffle");
+    const int from = (l & ~(width - 1)) | (src & (width - 1));
+    const double got = w.a[from];
+    w.bar.wait("wavefront shuffle");
+    T out;
+    std::memcpy(&out, &got, sizeof(T));
+    return out;
+}
+
 // v_readlane: every lane of the wave receives lane l's value; l must be the
 // same in all lanes of the wave.
 inline int readlane(int v, int l)
@@ -289,6 +308,7 @@ inline void run_or_record(std::function<void()> f)
 
 #define __syncthreads() qx_emu::g_block.wait("block", __FILE__, __LINE__)
 #define __shfl_xor(v, m, w) qx_emu::shfl_xor((v), (m), (w))
+#define __shfl(v, l, w) qx_emu::shfl((v), (l), (w))
 #define __builtin_amdgcn_mfma_f64_16x16x4f64 qx_emu::mfma_f64_16x16x4
 #define __ffsll(x) __builtin_ffsll(x)
 #define __builtin_amdgcn_readlane(v, l) qx_emu::readlane((v), (l))
